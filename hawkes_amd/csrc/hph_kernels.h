@@ -1,0 +1,53 @@
+// Launch interface between the engine (hph_engine.cpp) and the HIP kernels (hph_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hph {
+namespace kern {
+
+constexpr int kRowsPerBlock = 256; // one event row per thread, 4 wavefronts of 64
+constexpr int kTile = 256;         // events per LDS tile
+constexpr int kMaxDim = 8;
+constexpr int kResultWidth = 7;    // loglik + 6 gradient components
+
+// Event data of one dataset on the device, time-sorted, structure of arrays.
+struct Events {
+    const double *x;   // D planes of npad doubles: x[d * npad + i]
+    const double *t;   // npad doubles
+    const int *perm;   // sorted position -> caller's index
+    int n;             // events
+    int npad;          // n rounded up to kRowsPerBlock
+    int dim;           // D
+    double t_last;     // the caller's last time (T of the model)
+};
+
+// Work decomposition of the pair pass.
+struct PairGrid {
+    int row_blocks;    // ceil(n / kRowsPerBlock)
+    int chunks;        // column chunks per row block
+    int chunk_len;     // events per chunk (multiple of kTile)
+};
+
+PairGrid plan_pair_grid(int n, int K, int target_blocks);
+
+// Pair pass: per (k, chunk, row) sums of exp terms and their moments.
+//   acc 0: sum exp(-(tauX^2 r^2 + tauT^2 dt^2)/2)      (acc 1: * r^2, acc 2: * dt^2)
+//   acc 3: sum_{dt>0} exp(-omega dt - sigma^2 r^2/2)   (acc 4: * r^2, acc 5: * dt)
+// Without the gradient only acc 0 and acc 3 are formed (stored as slots 0 and 1).
+// part layout: [K][chunks][nacc][npad].
+void launch_pair(const Events &ev, const double *params, int K, const PairGrid &g, bool grad, bool fp32,
+                 double *part, hipStream_t s);
+
+// Per-row log-likelihood terms and gradient terms, reduced per block in double-double.
+// block_out layout: [K][row_blocks][7][2]. probs (may be null) receives p_i for k == 0 in the
+// caller's order.
+void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
+                     const double *part, double *block_out, double *probs, hipStream_t s);
+
+// Sums the block partials per k in a fixed order: result layout [K][7] (only [k][0] without grad).
+void launch_reduce(const Events &ev, int K, const PairGrid &g, bool grad, const double *block_out,
+                   double *result, hipStream_t s);
+
+} // namespace kern
+} // namespace hph

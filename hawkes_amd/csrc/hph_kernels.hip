@@ -1,0 +1,340 @@
+// HIP kernels of the Hawkes likelihood engine, written for gfx950 (CDNA4, wave64).
+//
+// The O(N^2) pass restates the per-row rate sum of src/NewHawkes.hpp:520-547 (ratesLoop) and
+// src/backend/opencl/OpenCLHawkes.hpp:632-702 (computeLikContribs), with three changes of
+// method:
+//   * one event row per lane, the column events staged in LDS tiles and read by broadcast;
+//   * the two Gaussians of the background term share one exp, and so do the Gaussian and the
+//     exponential of the excitation term: pdf(a)pdf(b) = exp(-(a^2+b^2)/2)/(2 pi);
+//   * events are time-sorted, so the excitation term (dt > 0 only) is skipped for whole tiles
+//     that lie after a wavefront's rows and needs no mask for tiles wholly before them.
+// Per-row terms (log, exp and Phi integral terms, src/NewHawkes.hpp:652-655) and the exact
+// gradient are formed in a second kernel; all sums over rows are fixed-order double-double
+// reductions, so results are bitwise reproducible run to run.
+
+#include "hph_exp_coeffs.h"
+#include "hph_kernels.h"
+
+#include <stdexcept>
+
+namespace hph {
+namespace kern {
+
+// exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
+// Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
+// fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of
+// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp keeps k inside int range; for
+// x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for x <= 709.
+__device__ __forceinline__ double exp_fp64(double x) {
+    x = fmax(x, -1000.0);
+    const double shifter = 6755399441055744.0; // 1.5 * 2^52
+    const double kf = __builtin_fma(x, HPH_LOG2E, shifter);
+    const double k = kf - shifter;
+    const int ki = (int)(unsigned)__builtin_bit_cast(unsigned long long, kf);
+    double r = __builtin_fma(-k, HPH_LN2_HI, x);
+    r = __builtin_fma(-k, HPH_LN2_LO, r);
+    double p = HPH_EXP_C10;
+    p = __builtin_fma(p, r, HPH_EXP_C9);
+    p = __builtin_fma(p, r, HPH_EXP_C8);
+    p = __builtin_fma(p, r, HPH_EXP_C7);
+    p = __builtin_fma(p, r, HPH_EXP_C6);
+    p = __builtin_fma(p, r, HPH_EXP_C5);
+    p = __builtin_fma(p, r, HPH_EXP_C4);
+    p = __builtin_fma(p, r, HPH_EXP_C3);
+    p = __builtin_fma(p, r, HPH_EXP_C2);
+    p = __builtin_fma(p, r, HPH_EXP_C1);
+    p = __builtin_fma(p, r, HPH_EXP_C0);
+    return __builtin_amdgcn_ldexp(p, ki);
+}
+
+enum { kSNone = 0, kSFull = 1, kSMixed = 2 };
+
+// Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.
+template <int D> struct TileStride {
+    static constexpr int value = (D + 2) & ~1;
+};
+
+// One LDS tile against this lane's row. SMODE says whether the excitation term is absent
+// (every column is at or after the row block), unmasked (every column strictly before it) or
+// masked per pair (the diagonal band, and ties).
+template <int D, bool GRAD, int SMODE>
+__device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D],
+                                          double ti, double cBx, double cBt, double cSx, double nomega,
+                                          double (&acc)[6]) {
+    constexpr int ST = TileStride<D>::value;
+#pragma unroll 2
+    for (int j = 0; j < cnt; ++j) {
+        const double *e = tile + j * ST;
+        double r2;
+        {
+            const double d0 = xi[0] - e[0];
+            r2 = d0 * d0;
+        }
+#pragma unroll
+        for (int d = 1; d < D; ++d) {
+            const double dd = xi[d] - e[d];
+            r2 = __builtin_fma(dd, dd, r2);
+        }
+        const double dt = ti - e[D];
+        const double dt2 = dt * dt;
+        const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
+        acc[0] += eb;
+        if (GRAD) {
+            acc[1] = __builtin_fma(eb, r2, acc[1]);
+            acc[2] = __builtin_fma(eb, dt2, acc[2]);
+        }
+        if (SMODE != kSNone) {
+            double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+            if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
+            acc[3] += es;
+            if (GRAD) {
+                acc[4] = __builtin_fma(es, r2, acc[4]);
+                acc[5] = __builtin_fma(es, dt, acc[5]);
+            }
+        }
+    }
+}
+
+// Grid: x = row_blocks * chunks (later row blocks first: they carry more excitation work),
+// y = parameter vector k. Block: 256 lanes, one row each.
+template <int D, bool GRAD>
+__global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const double *__restrict__ params,
+                                                             PairGrid g, double *__restrict__ part) {
+    constexpr int ST = TileStride<D>::value;
+    constexpr int NACC = GRAD ? 6 : 2;
+    __shared__ double tile[kTile * ST];
+
+    const int k = blockIdx.y;
+    const int rb = g.row_blocks - 1 - (int)(blockIdx.x / g.chunks);
+    const int c = (int)(blockIdx.x % g.chunks);
+    const double *pk = params + 6 * k;
+    const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
+    const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
+    const double cSx = -0.5 * sigma * sigma, nomega = -omega;
+
+    const int tid = threadIdx.x;
+    const int n = ev.n, npad = ev.npad;
+    const int i = rb * kRowsPerBlock + tid;
+    const int ic = min(i, n - 1);
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = ev.x[(size_t)d * npad + ic];
+    const double ti = ev.t[ic];
+
+    const int wave_first = __builtin_amdgcn_readfirstlane(rb * kRowsPerBlock + (tid & ~63));
+    const bool wave_live = wave_first < n;
+    const double t_lo = ev.t[min(wave_first, n - 1)];
+    const double t_hi = ev.t[min(wave_first + 63, n - 1)];
+
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int j_begin = c * g.chunk_len;
+    const int j_end = min(n, j_begin + g.chunk_len);
+    for (int j0 = j_begin; j0 < j_end; j0 += kTile) {
+        const int cnt = min(kTile, j_end - j0);
+        __syncthreads();
+        if (tid < cnt) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) tile[tid * ST + d] = ev.x[(size_t)d * npad + j0 + tid];
+            tile[tid * ST + D] = ev.t[j0 + tid];
+        }
+        __syncthreads();
+        if (wave_live) {
+            const double tf = tile[D], tl = tile[(cnt - 1) * ST + D];
+            int mode = tl < t_lo ? kSFull : (tf >= t_hi ? kSNone : kSMixed);
+            mode = __builtin_amdgcn_readfirstlane(mode);
+            if (mode == kSFull)
+                tile_pass<D, GRAD, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
+            else if (mode == kSNone)
+                tile_pass<D, GRAD, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
+            else
+                tile_pass<D, GRAD, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
+        }
+    }
+    double *out = part + ((size_t)k * g.chunks + c) * NACC * npad;
+    if (GRAD) {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) out[(size_t)a * npad + i] = acc[a];
+    } else {
+        out[i] = acc[0];
+        out[npad + i] = acc[3];
+    }
+}
+
+// ---- double-double helpers (Knuth TwoSum) ----
+__device__ __forceinline__ double2 dd_add(double2 a, double2 b) {
+    const double s = a.x + b.x;
+    const double bb = s - a.x;
+    double e = (a.x - (s - bb)) + (b.x - bb);
+    e += a.y + b.y;
+    const double hi = s + e;
+    return make_double2(hi, e - (hi - s));
+}
+
+__device__ __forceinline__ double ipow(double x, int e) {
+    double r = 1.0;
+    for (int q = 0; q < e; ++q) r *= x;
+    return r;
+}
+
+// Fixed-order tree reduction of one double-double per thread over a 256-thread block.
+__device__ double2 block_reduce_dd(double2 v, double2 *sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = kRowsPerBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = dd_add(sh[tid], sh[tid + s]);
+        __syncthreads();
+    }
+    const double2 r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Per-row terms: ll_i = log(lambda_i) + theta (e^{-omega (T - t_i)} - 1)
+//                       - mu0 (Phi(tauT (T - t_i)) - Phi(-tauT t_i))   (src/NewHawkes.hpp:652-655)
+// and d ll_i / d(sigma, tauX, tauT, omega, theta, mu0).
+template <bool GRAD>
+__global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, const double *__restrict__ params,
+                                                                 PairGrid g, const double *__restrict__ part,
+                                                                 double *__restrict__ block_out,
+                                                                 double *__restrict__ probs) {
+    constexpr int NACC = GRAD ? 6 : 2;
+    __shared__ double2 sh[kRowsPerBlock];
+    const int k = blockIdx.y;
+    const int rb = blockIdx.x;
+    const int i = rb * kRowsPerBlock + threadIdx.x;
+    const int n = ev.n, npad = ev.npad, D = ev.dim;
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < g.chunks; ++c) {
+            const double *base = part + ((size_t)k * g.chunks + c) * NACC * npad;
+#pragma unroll
+            for (int a = 0; a < NACC; ++a) s[a] += base[(size_t)a * npad + i];
+        }
+        const double sB = s[0], sS = GRAD ? s[3] : s[1];
+        const double *pk = params + 6 * k;
+        const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3], theta = pk[4], mu0 = pk[5];
+        const double inv_2pi = 0.15915494309189533577;
+        const double inv_sqrt_2pi = 0.39894228040143267794;
+        const double sigmaD = ipow(sigma, D), tauXD = ipow(tauX, D);
+        const double bg = mu0 * tauXD * tauT * inv_2pi * sB;
+        const double se = theta * omega * sigmaD * inv_sqrt_2pi * sS;
+        const double lam = bg + se;
+        const double ti = ev.t[i];
+        const double dT = ev.t_last - ti;
+        const double eT = exp(-omega * dT);
+        const double phiA = 0.5 * erfc(-tauT * dT * M_SQRT1_2);
+        const double phiB = 0.5 * erfc(tauT * ti * M_SQRT1_2);
+        v[0] = log(lam) + theta * (eT - 1.0) - mu0 * (phiA - phiB);
+        if (GRAD) {
+            const double pB = sB * inv_2pi, pBr = s[1] * inv_2pi, pBd = s[2] * inv_2pi;
+            const double pS = sS * inv_sqrt_2pi, pSr = s[4] * inv_sqrt_2pi, pSd = s[5] * inv_sqrt_2pi;
+            const double inv = 1.0 / lam;
+            const double zA = tauT * dT, zB = tauT * ti;
+            const double pdfA = exp(-0.5 * zA * zA) * inv_sqrt_2pi;
+            const double pdfB = exp(-0.5 * zB * zB) * inv_sqrt_2pi;
+            v[1] = theta * omega * ipow(sigma, D - 1) * (D * pS - sigma * sigma * pSr) * inv;
+            v[2] = mu0 * tauT * ipow(tauX, D - 1) * (D * pB - tauX * tauX * pBr) * inv;
+            v[3] = mu0 * tauXD * (pB - tauT * tauT * pBd) * inv - mu0 * (dT * pdfA + ti * pdfB);
+            v[4] = theta * sigmaD * (pS - omega * pSd) * inv - theta * dT * eT;
+            v[5] = omega * sigmaD * pS * inv + (eT - 1.0);
+            v[6] = tauXD * tauT * pB * inv - (phiA - phiB);
+        }
+        if (probs != nullptr && k == 0) probs[ev.perm[i]] = se / (bg + se);
+    }
+    for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
+        const double2 r = block_reduce_dd(make_double2(v[q], 0.0), sh);
+        if (threadIdx.x == 0) {
+            double *o = block_out + (((size_t)k * g.row_blocks + rb) * kResultWidth + q) * 2;
+            o[0] = r.x;
+            o[1] = r.y;
+        }
+    }
+}
+
+// One block per k: fixed-order double-double sum of the block partials, plus the
+// N (D-1) log(1/sqrt(2 pi)) constant (src/NewHawkes.hpp:659).
+__global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, PairGrid g, int width,
+                                                               const double *__restrict__ block_out,
+                                                               double *__restrict__ result) {
+    __shared__ double2 sh[kRowsPerBlock];
+    const int k = blockIdx.x;
+    for (int q = 0; q < width; ++q) {
+        double2 a = make_double2(0.0, 0.0);
+        for (int b = threadIdx.x; b < g.row_blocks; b += kRowsPerBlock) {
+            const double *o = block_out + (((size_t)k * g.row_blocks + b) * kResultWidth + q) * 2;
+            a = dd_add(a, make_double2(o[0], o[1]));
+        }
+        double2 r = block_reduce_dd(a, sh);
+        if (threadIdx.x == 0) {
+            if (q == 0) {
+                const double c = -0.91893853320467274178 * (double)ev.n * (double)(ev.dim - 1);
+                r = dd_add(r, make_double2(c, 0.0));
+            }
+            result[(size_t)k * kResultWidth + q] = r.x + r.y;
+        }
+    }
+}
+
+// ---------------- host launchers ----------------
+
+PairGrid plan_pair_grid(int n, int K, int target_blocks) {
+    PairGrid g;
+    g.row_blocks = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int max_chunks = (n + kTile - 1) / kTile;
+    long long per_k = (long long)K * g.row_blocks;
+    int want = (int)((target_blocks + per_k - 1) / per_k);
+    if (want < 1) want = 1;
+    int chunks = want < max_chunks ? want : max_chunks;
+    const int per_chunk = (n + chunks - 1) / chunks;
+    g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
+    g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
+    return g;
+}
+
+template <int D>
+static void launch_pair_d(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
+                          double *part, hipStream_t s) {
+    dim3 grid((unsigned)(g.row_blocks * g.chunks), (unsigned)K);
+    if (grad)
+        hipLaunchKernelGGL((pair_kernel<D, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
+    else
+        hipLaunchKernelGGL((pair_kernel<D, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
+}
+
+void launch_pair(const Events &ev, const double *params, int K, const PairGrid &g, bool grad, bool fp32,
+                 double *part, hipStream_t s) {
+    if (fp32) throw std::invalid_argument("FP32-bulk pair pass is not built in this library");
+    switch (ev.dim) {
+    case 1: launch_pair_d<1>(ev, params, K, g, grad, part, s); break;
+    case 2: launch_pair_d<2>(ev, params, K, g, grad, part, s); break;
+    case 3: launch_pair_d<3>(ev, params, K, g, grad, part, s); break;
+    case 4: launch_pair_d<4>(ev, params, K, g, grad, part, s); break;
+    case 5: launch_pair_d<5>(ev, params, K, g, grad, part, s); break;
+    case 6: launch_pair_d<6>(ev, params, K, g, grad, part, s); break;
+    case 7: launch_pair_d<7>(ev, params, K, g, grad, part, s); break;
+    case 8: launch_pair_d<8>(ev, params, K, g, grad, part, s); break;
+    default: throw std::invalid_argument("embedding dimension must be 1..8");
+    }
+}
+
+void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
+                     const double *part, double *block_out, double *probs, hipStream_t s) {
+    dim3 grid((unsigned)g.row_blocks, (unsigned)K);
+    if (grad)
+        hipLaunchKernelGGL((finalize_kernel<true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
+                           block_out, probs);
+    else
+        hipLaunchKernelGGL((finalize_kernel<false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
+                           block_out, probs);
+}
+
+void launch_reduce(const Events &ev, int K, const PairGrid &g, bool grad, const double *block_out,
+                   double *result, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, g, grad ? kResultWidth : 1,
+                       block_out, result);
+}
+
+} // namespace kern
+} // namespace hph

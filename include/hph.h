@@ -1,0 +1,104 @@
+/*
+ * hph.h -- C ABI of the MI355X Hawkes likelihood engine (libhph.so).
+ *
+ * Drop-in boundary for the engine API of suchard-group/hawkes (hpHawkes):
+ * every entry point below replaces one virtual of hph::AbstractHawkes
+ * (src/AbstractHawkes.hpp:32-60) or the factory (src/factory.cpp:32-118), as a
+ * foreign binding (Rcpp .Call, JNI, ctypes, cgo) would bind it. Plain pointers and
+ * sizes only; the caller owns every buffer, which is borrowed for the call only.
+ *
+ * Every function returns HPH_OK (0) or an error code; hph_last_error() gives the
+ * message of the calling thread's last failure. No call aborts the process.
+ * One engine must not be used by two threads at once; distinct engines are independent.
+ */
+#ifndef HPH_H
+#define HPH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Flag bits, same values as hph::Flags (src/flags.h:6-13). */
+#define HPH_FLAG_FLOAT (1L << 2)  /* FP32-bulk pair arithmetic, FP64 accumulation */
+#define HPH_FLAG_TBB (1L << 3)    /* accepted; CPU-only, no effect here */
+#define HPH_FLAG_OPENCL (1L << 4) /* GPU engine: this library's HIP engine */
+#define HPH_FLAG_SSE (1L << 5)    /* accepted; CPU-only, no effect here */
+#define HPH_FLAG_AVX (1L << 6)    /* accepted; CPU-only, no effect here */
+#define HPH_FLAG_AVX512 (1L << 7) /* accepted; CPU-only, no effect here */
+
+/* Status codes. */
+#define HPH_OK 0
+#define HPH_ERR_INVALID_ARGUMENT 1
+#define HPH_ERR_NOT_INITIALIZED 2
+#define HPH_ERR_DEVICE 3
+#define HPH_ERR_OUT_OF_MEMORY 4
+#define HPH_ERR_UNSUPPORTED 5
+
+/* Number of model parameters: (sigmaXprec, tauXprec, tauTprec, omega, theta, mu0). */
+#define HPH_PARAMETER_COUNT 6
+
+typedef struct hph_engine hph_engine;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *hph_last_error(void);
+
+/* Replaces hph::factory(dim, N, flags, device, threads) (src/factory.cpp:32) and the engine
+ * constructor (src/AbstractHawkes.hpp:34). device is the 0-based HIP device; -1 means device 0
+ * (the reference's "no device" value, src/hpHawkes.cpp:65). threads is accepted and unused. */
+int hph_create(int embedding_dimension, int location_count, long flags, int device, int threads,
+               hph_engine **out);
+
+/* Frees the engine and its device memory (replaces the shared_ptr release). NULL is a no-op. */
+int hph_destroy(hph_engine *engine);
+
+/* Replaces AbstractHawkes::setTimesData(double*, size_t) (src/AbstractHawkes.hpp:49).
+ * length must equal location_count. Times need not be sorted; T is the last element. */
+int hph_set_times_data(hph_engine *engine, const double *data, size_t length);
+
+/* Replaces AbstractHawkes::updateLocations(int, double*, size_t) (src/AbstractHawkes.hpp:43).
+ * location_index == -1: all locations, row-major N x D, length == N*D.
+ * location_index >= 0 : one location, length == D. */
+int hph_update_locations(hph_engine *engine, int location_index, const double *location, size_t length);
+
+/* Replaces AbstractHawkes::setParameters(double*, size_t) (src/AbstractHawkes.hpp:50).
+ * length must be 6: (sigmaXprec, tauXprec, tauTprec, omega, theta, mu0); values finite. */
+int hph_set_parameters(hph_engine *engine, const double *data, size_t length);
+
+/* Replaces AbstractHawkes::getSumOfLikContribs() (src/AbstractHawkes.hpp:42): the log-likelihood. */
+int hph_get_sum_of_lik_contribs(hph_engine *engine, double *out);
+
+/* New (the reference's getLogLikelihoodGradient is commented out, src/AbstractHawkes.hpp:44):
+ * d loglik / d(sigmaXprec, tauXprec, tauTprec, omega, theta, mu0); length must be 6. */
+int hph_get_log_like_gradient(hph_engine *engine, double *out, size_t length);
+
+/* Log-likelihood and gradient from one pass: *loglik and grad[0..5]. */
+int hph_get_log_likelihood_and_gradient(hph_engine *engine, double *loglik, double *grad);
+
+/* Replaces AbstractHawkes::getProbsSelfExcite(double*, size_t) (src/AbstractHawkes.hpp:45);
+ * length must equal location_count; result in the caller's event order. */
+int hph_get_probs_self_excite(hph_engine *engine, double *out, size_t length);
+
+/* Replace storeState / restoreState / acceptState (src/AbstractHawkes.hpp:46-48): snapshot of
+ * parameters, locations and the last log-likelihood; restore brings the snapshot back. */
+int hph_store_state(hph_engine *engine);
+int hph_restore_state(hph_engine *engine);
+int hph_accept_state(hph_engine *engine);
+
+/* Replaces AbstractHawkes::getInternalDimension() (src/AbstractHawkes.hpp:51). */
+int hph_get_internal_dimension(hph_engine *engine, int *out);
+
+/* ---- Batched proposals (extension; no counterpart in the reference) ----
+ * K parameter vectors evaluated against the engine's data in one launch. params is K x 6,
+ * row-major; count must be K*6. The single-vector parameters set by hph_set_parameters are
+ * not changed. */
+int hph_set_parameters_batch(hph_engine *engine, const double *params, size_t count);
+/* loglik[K]; grad (K x 6, may be NULL for log-likelihood only). */
+int hph_get_log_likelihood_batch(hph_engine *engine, double *loglik, double *grad, size_t K);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPH_H */

@@ -1,0 +1,63 @@
+/*
+ * hph_engine.hpp -- C++ engine interface of the MI355X Hawkes likelihood engine.
+ *
+ * Same virtual signatures as hph::AbstractHawkes (src/AbstractHawkes.hpp:32-60) and the same
+ * factory signature (src/AbstractHawkes.hpp:64, src/factory.cpp:32), so C++ callers written
+ * against the reference recompile against this header. Additions: getLogLikeGradient and the
+ * batched-proposal calls. Errors are reported as exceptions derived from std::exception
+ * (std::invalid_argument for bad lengths or values, std::runtime_error for device failures);
+ * the C ABI in hph.h turns them into status codes.
+ */
+#ifndef HPH_ENGINE_HPP
+#define HPH_ENGINE_HPP
+
+#include <cstddef>
+#include <memory>
+#include <vector>
+
+namespace hph {
+
+/* Flag bits (src/flags.h:6-13). */
+enum Flags { FLOAT = 1 << 2, TBB = 1 << 3, OPENCL = 1 << 4, SSE = 1 << 5, AVX = 1 << 6, AVX512 = 1 << 7 };
+
+class AbstractHawkes {
+  public:
+    AbstractHawkes(int embeddingDimension, int locationCount, long flags)
+        : embeddingDimension(embeddingDimension), locationCount(locationCount), flags(flags) {}
+
+    virtual ~AbstractHawkes() = default;
+
+    // Interface of src/AbstractHawkes.hpp:42-51
+    virtual double getSumOfLikContribs() = 0;
+    virtual void updateLocations(int, double *, size_t) = 0;
+    virtual void getProbsSelfExcite(double *, size_t) = 0;
+    virtual void storeState() = 0;
+    virtual void restoreState() = 0;
+    virtual void acceptState() = 0;
+    virtual void setTimesData(double *, size_t) = 0;
+    virtual void setParameters(double *, size_t) = 0;
+    virtual int getInternalDimension() = 0;
+
+    // Additions
+    virtual void getLogLikeGradient(double *, size_t) = 0;
+    virtual double getLogLikelihoodAndGradient(double *gradient6) = 0;
+    virtual void setParametersBatch(const double *, size_t) = 0;
+    virtual void getLogLikelihoodBatch(double *loglik, double *gradient, size_t K) = 0;
+
+  protected:
+    int embeddingDimension;
+    int locationCount;
+    long long pairCount() const { return (long long)locationCount * locationCount; }
+    long flags;
+    int updatedLocation = -1;
+};
+
+typedef std::shared_ptr<hph::AbstractHawkes> SharedPtr;
+
+/* src/factory.cpp:32. Every flag combination returns the HIP engine (FLOAT selects the
+ * FP32-bulk mode); device is the 0-based HIP device, -1 meaning device 0. */
+SharedPtr factory(int dim1, int dim2, long flags, int device, int threads);
+
+} // namespace hph
+
+#endif // HPH_ENGINE_HPP
