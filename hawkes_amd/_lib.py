@@ -45,6 +45,8 @@ SIGNATURES = {
     "hph_get_internal_dimension": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "hph_set_parameters_batch": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_get_log_likelihood_batch": (ctypes.c_int, [_vp, _dp, _dp, _sz]),
+    "hph_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hph_get_timing": (ctypes.c_int, [_vp, _dp, _sz]),
 }
 
 _lib = None

@@ -111,6 +111,8 @@ class HipHawkes : public AbstractHawkes {
         h_stage_.release();
         h_result_.release();
         h_params_.release();
+        for (auto &e : ev_)
+            if (e) (void)hipEventDestroy(e);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -241,6 +243,19 @@ class HipHawkes : public AbstractHawkes {
         }
     }
 
+    void setTiming(bool enabled) override {
+        check(hipSetDevice(device_), "hipSetDevice");
+        if (enabled && !ev_[0])
+            for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+        timing_ = enabled;
+    }
+
+    void getTiming(double *out, size_t length) override {
+        if (out == nullptr || length < 2) throw std::invalid_argument("timing output needs 2 values");
+        out[0] = last_ms_[0];
+        out[1] = last_ms_[1];
+    }
+
   private:
     void invalidate() { ll_valid_ = grad_valid_ = false; }
 
@@ -307,13 +322,16 @@ class HipHawkes : public AbstractHawkes {
                              stream_),
               "upload params");
         const kern::Events ev = events();
+        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
         kern::launch_pair(ev, d_params_.as<double>(), K, g, grad, fp32_, d_part_.as<double>(), stream_);
         check(hipGetLastError(), "pair kernel launch");
+        if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(), d_block_.as<double>(),
                               probs_host ? d_probs_.as<double>() : nullptr, stream_);
         check(hipGetLastError(), "finalize kernel launch");
         kern::launch_reduce(ev, K, g, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
         check(hipGetLastError(), "reduce kernel launch");
+        if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * (size_t)K * kern::kResultWidth,
                              hipMemcpyDeviceToHost, stream_),
               "download result");
@@ -322,6 +340,13 @@ class HipHawkes : public AbstractHawkes {
                                  stream_),
                   "download probs");
         check(hipStreamSynchronize(stream_), "stream sync");
+        if (timing_) {
+            float a = 0.f, b = 0.f;
+            check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "hipEventElapsedTime");
+            check(hipEventElapsedTime(&b, ev_[1], ev_[2]), "hipEventElapsedTime");
+            last_ms_[0] = a;
+            last_ms_[1] = b;
+        }
     }
 
     void evaluate_single(bool grad, double *probs_host) {
@@ -357,6 +382,10 @@ class HipHawkes : public AbstractHawkes {
     double stored_params_[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> stored_locs_;
     double stored_ll_ = 0.0;
+
+    bool timing_ = false;
+    hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+    double last_ms_[2] = {0.0, 0.0};
 
     DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_;
     HostBuf h_stage_, h_result_, h_params_;
@@ -510,6 +539,16 @@ int hph_set_parameters_batch(hph_engine *e, const double *params, size_t count) 
 int hph_get_log_likelihood_batch(hph_engine *e, double *loglik, double *grad, size_t K) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->getLogLikelihoodBatch(loglik, grad, K); });
+}
+
+int hph_set_timing(hph_engine *e, int enabled) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->setTiming(enabled != 0); });
+}
+
+int hph_get_timing(hph_engine *e, double *out, size_t length) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->getTiming(out, length); });
 }
 
 } // extern "C"

@@ -102,6 +102,16 @@ class Engine:
         check(_lib.lib().hph_get_log_likelihood_batch(self._h, dptr(ll), dptr(g) if gradient else None, K))
         return (ll, g) if gradient else ll
 
+    # -- kernel timing (HIP events on the engine's stream) --
+    def set_timing(self, enabled=True):
+        check(_lib.lib().hph_set_timing(self._h, 1 if enabled else 0))
+
+    def timing(self):
+        """(pair-pass ms, finalize + reduction ms) of the last evaluation."""
+        out = np.zeros(2)
+        check(_lib.lib().hph_get_timing(self._h, dptr(out), 2))
+        return float(out[0]), float(out[1])
+
     # -- state (src/AbstractHawkes.hpp:46-48) --
     def store_state(self):
         check(_lib.lib().hph_store_state(self._h))

@@ -97,6 +97,15 @@ int hph_set_parameters_batch(hph_engine *engine, const double *params, size_t co
 /* loglik[K]; grad (K x 6, may be NULL for log-likelihood only). */
 int hph_get_log_likelihood_batch(hph_engine *engine, double *loglik, double *grad, size_t K);
 
+/* ---- Kernel timing (extension; the reference's MICRO_BENCHMARK timers,
+ * src/backend/opencl/OpenCLHawkes.hpp:38, 377-429, measured here with HIP events on the
+ * engine's stream) ----
+ * enabled != 0 records events around the kernels of every later evaluation.
+ * hph_get_timing: out[0] = pair-pass kernel ms of the last evaluation, out[1] = per-row
+ * finalize + reduction kernels ms; length must be >= 2. */
+int hph_set_timing(hph_engine *engine, int enabled);
+int hph_get_timing(hph_engine *engine, double *out, size_t length);
+
 #ifdef __cplusplus
 }
 #endif

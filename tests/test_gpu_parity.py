@@ -1,0 +1,225 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle and the golden vectors.
+
+Tolerances (FP64 mode):
+  * log-likelihood: relative error <= 1e-12;
+  * gradient component k: |g - g_oracle| <= 1e-12 * max(|g_oracle|, 1e-3 * S_k), where S_k is
+    the oracle's sum of |per-row terms| of that component (gradients cancel near the MLE);
+  * self-excitation probabilities: relative error <= 1e-12 per event (exact 0 for events with
+    no earlier event).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from hawkes_amd import Engine, HphError, _lib, synth
+
+pytestmark = pytest.mark.gpu
+
+LL_TOL = 1e-12
+GRAD_TOL = 1e-12
+PROB_TOL = 1e-12
+
+
+def make_engine(locs, times, params):
+    locs = np.asarray(locs, dtype=np.float64)
+    if locs.ndim == 1:
+        locs = locs[:, None]
+    e = Engine(locs.shape[1], locs.shape[0])
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters(params)
+    return e
+
+
+def grad_err(g, ref):
+    g, go, scale = np.asarray(g), ref["grad"], ref["scale"][1:]
+    return np.abs(g - go) / np.maximum(np.maximum(np.abs(go), 1e-3 * scale), 1e-300)
+
+
+def assert_matches_oracle(locs, times, params, probs=True, precision="long", name=""):
+    ref = oracle.total(locs, times, params, precision=precision, probs=probs)
+    e = make_engine(locs, times, params)
+    ll, g = e.log_likelihood_and_gradient()
+    assert rel_err(ll, ref["ll"]) <= LL_TOL, (name, ll, ref["ll"])
+    assert grad_err(g, ref).max() <= GRAD_TOL, (name, grad_err(g, ref))
+    assert e.log_likelihood() == ll
+    if probs:
+        p = e.probs_self_excite()
+        np.testing.assert_allclose(p, ref["probs"], rtol=PROB_TOL, atol=0, err_msg=name)
+    e.close()
+    return ref
+
+
+def test_golden_vectors(golden):
+    for c in golden:
+        print("golden case", c["name"], flush=True)
+        e = make_engine(c["locations"], c["times"], c["params"])
+        ll, g = e.log_likelihood_and_gradient()
+        ref = oracle.total(c["locations"], c["times"], c["params"])
+        assert rel_err(ll, c["ll"]) <= LL_TOL, c["name"]
+        gold = np.array(c["grad"])
+        err = np.abs(g - gold) / np.maximum(np.maximum(np.abs(gold), 1e-3 * ref["scale"][1:]), 1e-300)
+        assert err.max() <= GRAD_TOL, (c["name"], err)
+        np.testing.assert_allclose(e.probs_self_excite(), c["probs"], rtol=PROB_TOL, atol=0, err_msg=c["name"])
+        e.close()
+
+
+def test_uniform_n1000_config():
+    locs, times, p = synth.workload("uniform_n1000")
+    assert_matches_oracle(locs, times, p, name="uniform_n1000")
+
+
+def test_cluster_process_small():
+    locs, times = synth.cluster_process(2800, side=np.sqrt(4000 / 100), seed=777)
+    assert_matches_oracle(locs, times, synth.P0, name="cluster_4k")
+
+
+@pytest.mark.parametrize("D", [1, 3, 5, 8])
+def test_dimensions(D):
+    rng = np.random.default_rng(100 + D)
+    n = 700
+    locs = rng.uniform(0, 5, (n, D))
+    times = np.sort(rng.uniform(0, 500, n))
+    p = np.array([0.9, 0.7, 0.05, 0.4, 0.8, 0.6])
+    assert_matches_oracle(locs, times, p, name="D%d" % D)
+
+
+def test_unsorted_times_and_ties():
+    rng = np.random.default_rng(21)
+    n = 900
+    locs = rng.uniform(0, 4, (n, 2))
+    times = np.round(rng.uniform(0, 300, n))  # many ties, not sorted; T = last element
+    p = np.array([1.1, 0.8, 0.02, 0.3, 0.9, 0.7])
+    assert_matches_oracle(locs, times, p, name="unsorted_ties")
+
+
+def test_row_counts_around_block_sizes():
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 63, 64, 65, 255, 256, 257, 513):
+        locs = rng.uniform(0, 3, (n, 2))
+        times = np.sort(rng.uniform(0, 100, n))
+        assert_matches_oracle(locs, times, [1.0, 0.9, 0.1, 0.5, 0.6, 0.9], name="n%d" % n)
+
+
+def test_batch_equals_single_calls_bitwise():
+    locs, times = synth.cluster_process(1500, seed=31)
+    P = synth.jitter_params(8)
+    e = make_engine(locs, times, P[0])
+    e.set_parameters_batch(P)
+    ll_b, g_b = e.log_likelihood_batch()
+    for k in range(8):
+        e.set_parameters(P[k])
+        ll, g = e.log_likelihood_and_gradient()
+        assert ll == ll_b[k]
+        assert np.array_equal(g, g_b[k])
+    ll_only = e.log_likelihood_batch(gradient=False)
+    assert np.array_equal(ll_only, ll_b)
+
+
+def test_repeat_calls_are_bitwise_identical():
+    locs, times = synth.cluster_process(3000, seed=9)
+    e = make_engine(locs, times, synth.P0)
+    a = e.log_likelihood_and_gradient()
+    pa = e.probs_self_excite()
+    for _ in range(3):
+        e.set_parameters(synth.P0)  # invalidates the cached result
+        b = e.log_likelihood_and_gradient()
+        assert a[0] == b[0] and np.array_equal(a[1], b[1])
+        assert np.array_equal(pa, e.probs_self_excite())
+
+
+def test_config2_full_parity_and_finite_differences():
+    """Cluster process, 35,000 background events (N ~ 49.5k): FP64 ll + 6-gradient against the
+    compensated double oracle, and the gradient against central finite differences of the GPU
+    log-likelihood (h = 1e-6 relative)."""
+    locs, times, p = synth.workload("cluster_b35k")
+    print("config2 N =", len(times), flush=True)
+    ref = oracle.total(locs, times, p, precision="double")
+    e = make_engine(locs, times, p)
+    ll, g = e.log_likelihood_and_gradient()
+    assert rel_err(ll, ref["ll"]) <= LL_TOL
+    assert grad_err(g, ref).max() <= GRAD_TOL
+    for k in range(6):
+        h = 1e-6 * p[k]
+        up, dn = p.copy(), p.copy()
+        up[k] += h
+        dn[k] -= h
+        e.set_parameters(up)
+        fu = e.log_likelihood()
+        e.set_parameters(dn)
+        fd = (fu - e.log_likelihood()) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-5 * max(abs(g[k]), 1e-3 * ref["scale"][1 + k]), (k, fd, g[k])
+
+
+def test_full_size_rows_against_oracle():
+    """N ~ 1e5 (70,000 background events): per-event probabilities of sampled rows against
+    the oracle's row sums over all N columns, and translation invariance of the total."""
+    locs, times, p = synth.workload("cluster_b70k")
+    n = len(times)
+    e = make_engine(locs, times, p)
+    probs = e.probs_self_excite()
+    ll, g = e.log_likelihood_and_gradient()
+    for lo in (0, n // 3, n - 64):
+        rows = oracle.rows(locs, times, p, lo, lo + 64, precision="double")
+        np.testing.assert_allclose(probs[lo:lo + 64], rows[:, 8], rtol=PROB_TOL, atol=0)
+    assert probs[0] == 0.0 and np.all((probs >= 0) & (probs <= 1))
+    e.update_locations(locs + np.array([1000.0, -250.0]))
+    ll2 = e.log_likelihood()
+    assert rel_err(ll2, ll) <= 1e-12
+
+
+def test_single_location_update_and_state():
+    locs, times = synth.cluster_process(900, seed=12)
+    p = synth.P0
+    e = make_engine(locs, times, p)
+    ll0 = e.log_likelihood()
+    e.store_state()
+    moved = locs.copy()
+    moved[17] += np.array([0.3, -0.2])
+    e.update_locations(moved[17], index=17)
+    ll1 = e.log_likelihood()
+    assert rel_err(ll1, oracle.total(moved, times, p)["ll"]) <= LL_TOL
+    e.restore_state()
+    assert e.log_likelihood() == ll0
+    e.set_parameters(p)
+    assert rel_err(e.log_likelihood(), ll0) <= 1e-15
+    e.accept_state()
+    assert e.internal_dimension() == 2
+
+
+def test_error_codes():
+    e = Engine(2, 10)
+    with pytest.raises(HphError) as ex:
+        e.log_likelihood()
+    assert ex.value.code == _lib.HPH_ERR_NOT_INITIALIZED
+    with pytest.raises(HphError) as ex:
+        e.set_times(np.zeros(9))
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    with pytest.raises(HphError) as ex:
+        e.set_parameters(np.zeros(5))
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    with pytest.raises(HphError) as ex:
+        e.set_parameters([1, 1, 1, 1, float("nan"), 1])
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    with pytest.raises(HphError) as ex:
+        e.update_locations(np.zeros(3), index=10)
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    e.close()
+
+
+def test_r_style_api():
+    import hawkes_amd as hph
+    locs, times, p = synth.workload("uniform_n1000")
+    eng = hph.createEngine(2, 1000, gpu=1)
+    with pytest.raises(RuntimeError):
+        hph.getLogLikelihood(eng)
+    eng = hph.updateLocations(eng, locs)
+    eng = hph.setTimesData(eng, times)
+    eng = hph.setParameters(eng, p)
+    ref = oracle.total(locs, times, p)
+    assert rel_err(hph.getLogLikelihood(eng), ref["ll"]) <= LL_TOL
+    assert grad_err(hph.getLogLikeGradient(eng), ref).max() <= GRAD_TOL
+    assert math.isfinite(hph.getProbsSelfExcite(eng).sum())
