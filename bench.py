@@ -26,9 +26,38 @@ METRIC = "loglik+grad pair-evals/s at N=1e5 FP64; K=64 batched vs plain speedup"
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 
-def flop_model(n, causal):
-    """Algorithmic FLOP of one LL + gradient evaluation: 35 N^2 + 28 C (exp = 20 FLOP, FMA = 2)."""
-    return 35.0 * n * n + 28.0 * causal
+def flop_model(n_background, n_excitation):
+    """Algorithmic FLOP of one LL + gradient evaluation: 35 per background pair + 28 per
+    excitation pair formed (exp = 20 FLOP, FMA = 2); 35 N^2 + 28 C when every pair is formed."""
+    return 35.0 * n_background + 28.0 * n_excitation
+
+
+def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256, zero_arg=746.0):
+    """Background and excitation pairs the pair kernel forms for sorted times: per (wavefront of
+    64 rows, tile of 256 columns), the tile is skipped where every term is exactly zero, the
+    excitation term is formed only for tiles not wholly after the rows (all pairs of the tile
+    in the masked diagonal case, as the kernel evaluates then masks)."""
+    t = np.sort(np.asarray(times))
+    n = t.size
+    w0 = np.arange(0, n, rows_per_wave)
+    w_lo, w_hi = t[w0], t[np.minimum(w0 + rows_per_wave - 1, n - 1)]
+    rows = np.minimum(w0 + rows_per_wave, n) - w0
+    j0 = np.arange(0, n, tile)
+    t_f, t_l = t[j0], t[np.minimum(j0 + tile - 1, n - 1)]
+    cols = np.minimum(j0 + tile, n) - j0
+    before = t_l[None, :] < w_lo[:, None]
+    after = t_f[None, :] >= w_hi[:, None]
+    s_on = ~after
+    b_on = np.ones_like(before)
+    if skip_zero:
+        gap = np.where(before, w_lo[:, None] - t_l[None, :],
+                       np.where(t_f[None, :] > w_hi[:, None], t_f[None, :] - w_hi[:, None], 0.0))
+        b_on = 0.5 * params[2] ** 2 * gap * gap < zero_arg
+        if params[3] >= 0:
+            s_on &= ~(before & (params[3] * gap >= zero_arg))
+    # the kernel runs whole 64-lane wavefronts; count lanes of real rows only
+    pair_counts = rows[:, None] * cols[None, :]
+    return float((pair_counts * b_on).sum()), float((pair_counts * s_on).sum())
 
 
 def load_traffic():
@@ -78,38 +107,43 @@ def main():
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from hawkes_amd import Engine, synth
+    from hawkes_amd import Engine, _lib, synth
 
     locs, times, p0 = synth.workload(args.workload)
     n = len(times)
     causal = synth.causal_pairs(times)
     params = synth.jitter_params(max(args.steps + args.warmup, 1))
-    e = Engine(2, n)
-    e.update_locations(locs)
-    e.set_times(times)
-    e.set_parameters(p0)
-    e.set_timing(True)
-    e.log_likelihood_and_gradient()  # uploads the data; first launch
 
-    for s in range(args.warmup):
-        e.set_parameters(params[s])
-        e.log_likelihood_and_gradient()
+    def run(flags, steps):
+        e = Engine(2, n, flags=flags)
+        e.update_locations(locs)
+        e.set_times(times)
+        e.set_parameters(p0)
+        e.set_timing(True)
+        e.log_likelihood_and_gradient()  # uploads the data; first launch
+        for s in range(args.warmup):
+            e.set_parameters(params[s])
+            e.log_likelihood_and_gradient()
+        pair_ms, fin_ms, formed = [], [], []
+        t0 = time.perf_counter()
+        for s in range(steps):
+            e.set_parameters(params[args.warmup + s])
+            e.log_likelihood_and_gradient()
+            a, b = e.timing()
+            pair_ms.append(a)
+            fin_ms.append(b)
+        wall = time.perf_counter() - t0
+        for s in range(steps):
+            formed.append(pairs_formed(times, params[args.warmup + s], not (flags & _lib.ALL_PAIRS)))
+        return e, wall, pair_ms, fin_ms, formed
 
-    pair_ms, fin_ms = [], []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        e.set_parameters(params[args.warmup + s])
-        e.log_likelihood_and_gradient()
-        a, b = e.timing()
-        pair_ms.append(a)
-        fin_ms.append(b)
-    wall = time.perf_counter() - t0
+    e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps)
     ms_per_step = 1e3 * wall / args.steps
     value = n * n / (wall / args.steps)
 
     pair_avg_ms = float(np.mean(pair_ms))
-    flop = flop_model(n, causal)
-    achieved = flop / (pair_avg_ms * 1e-3) / 1e12
+    flops = [flop_model(b, x) for b, x in formed]
+    achieved = float(np.mean([f / (ms * 1e-3) for f, ms in zip(flops, pair_ms)])) / 1e12
     roofline = {
         "bound": "valu",
         "achieved": achieved,
@@ -120,8 +154,22 @@ def main():
         "kernel": "pair_kernel<2,true>",
         "kernel_ms": pair_avg_ms,
         "finalize_reduce_ms": float(np.mean(fin_ms)),
-        "flop_per_launch": flop,
+        "flop_per_launch": float(np.mean(flops)),
+        "pairs_formed": {"background": float(np.mean([b for b, _ in formed])),
+                         "excitation": float(np.mean([x for _, x in formed]))},
     }
+
+    # Every pair formed (no exact-zero tile skipping): same results bitwise, more work.
+    ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4))
+    steps_a = len(pair_a)
+    flop_a = flop_model(*formed_a[0])
+    all_pairs = {
+        "value": n * n / (wall_a / steps_a),
+        "ms_per_step": 1e3 * wall_a / steps_a,
+        "kernel_ms": float(np.mean(pair_a)),
+        "roofline_frac": flop_a / (float(np.mean(pair_a)) * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
+    }
+    ea.close()
 
     batched = None
     if args.batch > 0:
@@ -157,6 +205,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "all_pairs": all_pairs,
     }
     print(json.dumps(out))
 

@@ -22,6 +22,7 @@ OPENCL = 1 << 4
 SSE = 1 << 5
 AVX = 1 << 6
 AVX512 = 1 << 7
+ALL_PAIRS = 1 << 16  # extension: no exact-zero tile skipping
 
 # Every symbol include/hph.h declares, with (restype, argtypes).
 _dp = ctypes.POINTER(ctypes.c_double)

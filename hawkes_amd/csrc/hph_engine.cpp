@@ -309,7 +309,8 @@ class HipHawkes : public AbstractHawkes {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
         if (data_dirty_) upload();
-        const kern::PairGrid g = kern::plan_pair_grid(locationCount, K, kTargetBlocks);
+        kern::PairGrid g = kern::plan_pair_grid(locationCount, K, kTargetBlocks);
+        g.skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         const int nacc = grad ? 6 : 2;
         d_params_.ensure(sizeof(double) * 6 * (size_t)K);
         d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);

@@ -27,7 +27,12 @@ struct PairGrid {
     int row_blocks;    // ceil(n / kRowsPerBlock)
     int chunks;        // column chunks per row block
     int chunk_len;     // events per chunk (multiple of kTile)
+    int skip_zero;     // 1: skip tiles whose every term underflows to exactly +0.0
 };
+
+// exp(x) is exactly +0.0 in FP64 for x <= -745.14 (both here and in the reference's exp);
+// a tile is skipped only when every one of its arguments is <= -kZeroExpArg.
+constexpr double kZeroExpArg = 746.0;
 
 PairGrid plan_pair_grid(int n, int K, int target_blocks);
 

@@ -54,10 +54,12 @@ template <int D> struct TileStride {
     static constexpr int value = (D + 2) & ~1;
 };
 
-// One LDS tile against this lane's row. SMODE says whether the excitation term is absent
-// (every column is at or after the row block), unmasked (every column strictly before it) or
-// masked per pair (the diagonal band, and ties).
-template <int D, bool GRAD, int SMODE>
+// One LDS tile against this lane's row. DOB says whether the background term is formed (it is
+// not when every background term of the tile is exactly zero). SMODE says whether the
+// excitation term is absent (every column at or after the row block, or every term exactly
+// zero), unmasked (every column strictly before it) or masked per pair (the diagonal band, and
+// ties).
+template <int D, bool GRAD, bool DOB, int SMODE>
 __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D],
                                           double ti, double cBx, double cBt, double cSx, double nomega,
                                           double (&acc)[6]) {
@@ -76,12 +78,14 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
             r2 = __builtin_fma(dd, dd, r2);
         }
         const double dt = ti - e[D];
-        const double dt2 = dt * dt;
-        const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
-        acc[0] += eb;
-        if (GRAD) {
-            acc[1] = __builtin_fma(eb, r2, acc[1]);
-            acc[2] = __builtin_fma(eb, dt2, acc[2]);
+        if (DOB) {
+            const double dt2 = dt * dt;
+            const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
+            acc[0] += eb;
+            if (GRAD) {
+                acc[1] = __builtin_fma(eb, r2, acc[1]);
+                acc[2] = __builtin_fma(eb, dt2, acc[2]);
+            }
         }
         if (SMODE != kSNone) {
             double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
@@ -140,14 +144,25 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
         __syncthreads();
         if (wave_live) {
             const double tf = tile[D], tl = tile[(cnt - 1) * ST + D];
-            int mode = tl < t_lo ? kSFull : (tf >= t_hi ? kSNone : kSMixed);
-            mode = __builtin_amdgcn_readfirstlane(mode);
-            if (mode == kSFull)
-                tile_pass<D, GRAD, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
-            else if (mode == kSNone)
-                tile_pass<D, GRAD, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
-            else
-                tile_pass<D, GRAD, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
+            int smode = tl < t_lo ? kSFull : (tf >= t_hi ? kSNone : kSMixed);
+            int dob = 1;
+            if (g.skip_zero) {
+                // Smallest |dt| between this wavefront's rows and the tile (times are sorted).
+                const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
+                // Background: arg <= -tauT^2 dt^2 / 2; excitation (tile before the rows only):
+                // arg <= -omega dt. Terms whose argument is <= -kZeroExpArg are exactly +0.0.
+                if (-cBt * gap * gap >= kZeroExpArg) dob = 0;
+                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= kZeroExpArg) smode = kSNone;
+            }
+            const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
+            switch (mode) {
+            case 4 + kSNone: tile_pass<D, GRAD, true, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            case 4 + kSFull: tile_pass<D, GRAD, true, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            case 4 + kSMixed: tile_pass<D, GRAD, true, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            case kSFull: tile_pass<D, GRAD, false, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            case kSMixed: tile_pass<D, GRAD, false, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            default: break; // every term of the tile is exactly zero
+            }
         }
     }
     double *out = part + ((size_t)k * g.chunks + c) * NACC * npad;
@@ -290,6 +305,7 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     const int per_chunk = (n + chunks - 1) / chunks;
     g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
     g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
+    g.skip_zero = 1;
     return g;
 }
 

@@ -27,6 +27,10 @@ extern "C" {
 #define HPH_FLAG_SSE (1L << 5)    /* accepted; CPU-only, no effect here */
 #define HPH_FLAG_AVX (1L << 6)    /* accepted; CPU-only, no effect here */
 #define HPH_FLAG_AVX512 (1L << 7) /* accepted; CPU-only, no effect here */
+/* Extension: evaluate every pair. By default, tiles of time-sorted events in which every exp
+ * argument is <= -746 (so every term is exactly +0.0 in FP64, as in the reference's exp) are
+ * skipped; the result is bitwise the same either way. */
+#define HPH_FLAG_ALL_PAIRS (1L << 16)
 
 /* Status codes. */
 #define HPH_OK 0

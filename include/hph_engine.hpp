@@ -18,7 +18,8 @@
 namespace hph {
 
 /* Flag bits (src/flags.h:6-13). */
-enum Flags { FLOAT = 1 << 2, TBB = 1 << 3, OPENCL = 1 << 4, SSE = 1 << 5, AVX = 1 << 6, AVX512 = 1 << 7 };
+enum Flags { FLOAT = 1 << 2, TBB = 1 << 3, OPENCL = 1 << 4, SSE = 1 << 5, AVX = 1 << 6, AVX512 = 1 << 7,
+             ALL_PAIRS = 1 << 16 /* extension: no exact-zero tile skipping */ };
 
 class AbstractHawkes {
   public:

@@ -119,6 +119,29 @@ def test_batch_equals_single_calls_bitwise():
     assert np.array_equal(ll_only, ll_b)
 
 
+@pytest.mark.parametrize("name,params", [
+    ("p0", synth.P0),
+    ("small_omega_large_tauT", [1.0, 1 / 1.6, 1 / 3.0, 0.05, 1.0, 1.0]),
+    ("large_omega", [1.0, 1 / 1.6, 1 / 336.0, 40.0, 1.0, 1.0]),
+])
+def test_zero_tile_skip_is_bitwise_exact(name, params):
+    """Skipping tiles whose terms all underflow to +0.0 gives bitwise the same results as
+    evaluating every pair (HPH_FLAG_ALL_PAIRS)."""
+    locs, times = synth.cluster_process(6000, side=20.0, horizon=43_800.0, seed=41)
+    out = []
+    for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
+        e = Engine(2, len(times), flags=flags)
+        e.update_locations(locs)
+        e.set_times(times)
+        e.set_parameters(params)
+        ll, g = e.log_likelihood_and_gradient()
+        out.append((ll, g, e.probs_self_excite()))
+        e.close()
+    assert out[0][0] == out[1][0], name
+    assert np.array_equal(out[0][1], out[1][1]), name
+    assert np.array_equal(out[0][2], out[1][2]), name
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
