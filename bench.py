@@ -112,7 +112,7 @@ def main():
     locs, times, p0 = synth.workload(args.workload)
     n = len(times)
     causal = synth.causal_pairs(times)
-    params = synth.jitter_params(max(args.steps + args.warmup, 1))
+    params = synth.jitter_params(max(args.steps, 3) + args.warmup)
 
     def run(flags, steps):
         e = Engine(2, n, flags=flags)

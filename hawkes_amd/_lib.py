@@ -24,6 +24,8 @@ AVX = 1 << 6
 AVX512 = 1 << 7
 ALL_PAIRS = 1 << 16  # extension: no exact-zero tile skipping
 
+PAIR_KERNEL_AUTO, PAIR_KERNEL_ROWWISE, PAIR_KERNEL_SYMMETRIC = 0, 1, 2
+
 # Every symbol include/hph.h declares, with (restype, argtypes).
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -48,6 +50,7 @@ SIGNATURES = {
     "hph_get_log_likelihood_batch": (ctypes.c_int, [_vp, _dp, _dp, _sz]),
     "hph_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_get_timing": (ctypes.c_int, [_vp, _dp, _sz]),
+    "hph_set_pair_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
 }
 
 _lib = None

@@ -106,7 +106,7 @@ class HipHawkes : public AbstractHawkes {
     ~HipHawkes() override {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
-        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_})
+        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_colpart_, &d_items_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -250,6 +250,14 @@ class HipHawkes : public AbstractHawkes {
         timing_ = enabled;
     }
 
+    void setPairKernel(int kind) override {
+        if (kind < HPH_PAIR_KERNEL_AUTO || kind > HPH_PAIR_KERNEL_SYMMETRIC)
+            throw std::invalid_argument("unknown pair kernel kind");
+        if (kind == HPH_PAIR_KERNEL_SYMMETRIC && embeddingDimension > 3)
+            throw std::invalid_argument("the symmetric pair kernel is built for dimensions 1..3");
+        kernel_ = kind;
+    }
+
     void getTiming(double *out, size_t length) override {
         if (out == nullptr || length < 2) throw std::invalid_argument("timing output needs 2 values");
         out[0] = last_ms_[0];
@@ -304,17 +312,23 @@ class HipHawkes : public AbstractHawkes {
         return ev;
     }
 
-    // Runs the three kernels for K parameter vectors; results land in h_result_ ([K][7]).
+    // Symmetric pass for D <= 3 once the data spans a few row blocks; the row-wise pass otherwise.
+    bool use_symmetric() const {
+        if (kernel_ == HPH_PAIR_KERNEL_ROWWISE) return false;
+        if (embeddingDimension > 3) return false;
+        if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC) return true;
+        return locationCount >= kSymMinEvents;
+    }
+
+    // Runs the pair pass, finalize and reduction kernels for K parameter vectors; results land in
+    // h_result_ ([K][7]).
     void run(const double *params, int K, bool grad, double *probs_host) {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
         if (data_dirty_) upload();
-        kern::PairGrid g = kern::plan_pair_grid(locationCount, K, kTargetBlocks);
-        g.skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
-        const int nacc = grad ? 6 : 2;
         d_params_.ensure(sizeof(double) * 6 * (size_t)K);
-        d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
-        d_block_.ensure(sizeof(double) * (size_t)K * g.row_blocks * kern::kResultWidth * 2);
+        const int fin_blocks = (locationCount + kern::kRowsPerBlock - 1) / kern::kRowsPerBlock;
+        d_block_.ensure(sizeof(double) * (size_t)K * fin_blocks * kern::kResultWidth * 2);
         d_result_.ensure(sizeof(double) * (size_t)K * kern::kResultWidth);
         h_result_.ensure(sizeof(double) * (size_t)K * kern::kResultWidth);
         h_params_.ensure(sizeof(double) * 6 * (size_t)K);
@@ -323,14 +337,45 @@ class HipHawkes : public AbstractHawkes {
                              stream_),
               "upload params");
         const kern::Events ev = events();
-        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-        kern::launch_pair(ev, d_params_.as<double>(), K, g, grad, fp32_, d_part_.as<double>(), stream_);
-        check(hipGetLastError(), "pair kernel launch");
-        if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
-        kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(), d_block_.as<double>(),
-                              probs_host ? d_probs_.as<double>() : nullptr, stream_);
-        check(hipGetLastError(), "finalize kernel launch");
-        kern::launch_reduce(ev, K, g, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
+        const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
+        double *probs_dev = probs_host ? d_probs_.as<double>() : nullptr;
+        if (use_symmetric()) {
+            if (!sym_valid_ || sym_K_ != K) {
+                sym_ = kern::plan_sym_grid(locationCount, embeddingDimension, K, kTargetBlocks);
+                d_items_.ensure(sizeof(int2) * sym_.items.size());
+                check(hipMemcpy(d_items_.ptr, sym_.items.data(), sizeof(int2) * sym_.items.size(),
+                                hipMemcpyHostToDevice),
+                      "upload work items");
+                sym_valid_ = true;
+                sym_K_ = K;
+            }
+            kern::SymGrid g = sym_.grid;
+            g.item = d_items_.as<int2>();
+            g.skip_zero = skip_zero;
+            d_part_.ensure(sizeof(double) * kern::sym_rowpart_doubles(g, K, grad));
+            d_colpart_.ensure(sizeof(double) * kern::sym_colpart_doubles(g, K, grad));
+            if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+            kern::launch_sym(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(), d_colpart_.as<double>(),
+                             stream_);
+            check(hipGetLastError(), "symmetric pair kernel launch");
+            if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+            kern::launch_sym_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
+                                      d_colpart_.as<double>(), d_block_.as<double>(), probs_dev, stream_);
+            check(hipGetLastError(), "finalize kernel launch");
+        } else {
+            kern::PairGrid g = kern::plan_pair_grid(locationCount, K, kTargetBlocks);
+            g.skip_zero = skip_zero;
+            const int nacc = grad ? 6 : 2;
+            d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
+            if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+            kern::launch_pair(ev, d_params_.as<double>(), K, g, grad, fp32_, d_part_.as<double>(), stream_);
+            check(hipGetLastError(), "pair kernel launch");
+            if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+            kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
+                                  d_block_.as<double>(), probs_dev, stream_);
+            check(hipGetLastError(), "finalize kernel launch");
+        }
+        kern::launch_reduce(ev, K, fin_blocks, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
         check(hipGetLastError(), "reduce kernel launch");
         if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * (size_t)K * kern::kResultWidth,
@@ -363,6 +408,11 @@ class HipHawkes : public AbstractHawkes {
     }
 
     static constexpr int kTargetBlocks = 4096;
+    static constexpr int kSymMinEvents = 4096;
+    int kernel_ = HPH_PAIR_KERNEL_AUTO;
+    kern::SymPlan sym_;
+    bool sym_valid_ = false;
+    int sym_K_ = 0;
 
     int device_;
     bool fp32_;
@@ -388,7 +438,7 @@ class HipHawkes : public AbstractHawkes {
     hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
     double last_ms_[2] = {0.0, 0.0};
 
-    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_;
+    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_, d_colpart_, d_items_;
     HostBuf h_stage_, h_result_, h_params_;
 };
 
@@ -545,6 +595,11 @@ int hph_get_log_likelihood_batch(hph_engine *e, double *loglik, double *grad, si
 int hph_set_timing(hph_engine *e, int enabled) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->setTiming(enabled != 0); });
+}
+
+int hph_set_pair_kernel(hph_engine *e, int kind) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->setPairKernel(kind); });
 }
 
 int hph_get_timing(hph_engine *e, double *out, size_t length) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace hph {
 namespace kern {
 
@@ -50,9 +52,37 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
                      const double *part, double *block_out, double *probs, hipStream_t s);
 
-// Sums the block partials per k in a fixed order: result layout [K][7] (only [k][0] without grad).
-void launch_reduce(const Events &ev, int K, const PairGrid &g, bool grad, const double *block_out,
-                   double *result, hipStream_t s);
+// Sums the per-block partials (nblocks finalize blocks of 256 rows) per k in a fixed order:
+// result layout [K][7] (only [k][0] without grad).
+void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,
+                   hipStream_t s);
+
+// ---- Symmetric pass (hph_sym.hip): background exps once per unordered pair ----
+struct SymGrid {
+    int A;               // event rows per lane
+    int R;               // rows per block (4 wavefronts x 64 lanes x A)
+    int nb;              // row blocks = ceil(n / R)
+    int L;               // earlier columns per work item (multiple of kTile)
+    int max_chunks;      // chunk slots of the row partials; the diagonal block uses slot max_chunks
+    int items;           // work items (block, chunk) and (block, -1) for the diagonal block
+    const int2 *item;    // device array of work items
+    long long col_stride;// column-partial doubles per parameter vector
+    int skip_zero;       // as PairGrid::skip_zero
+};
+
+struct SymPlan {
+    SymGrid grid;
+    std::vector<int2> items; // host copy of the work items, heaviest first
+};
+
+SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks);
+size_t sym_rowpart_doubles(const SymGrid &g, int K, bool grad);
+size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad);
+void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, double *rowpart,
+                double *colpart, hipStream_t s);
+void launch_sym_finalize(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
+                         const double *rowpart, const double *colpart, double *block_out, double *probs,
+                         hipStream_t s);
 
 } // namespace kern
 } // namespace hph

@@ -12,47 +12,13 @@
 // gradient are formed in a second kernel; all sums over rows are fixed-order double-double
 // reductions, so results are bitwise reproducible run to run.
 
-#include "hph_exp_coeffs.h"
-#include "hph_kernels.h"
+#include "hph_device.h"
 
 #include <stdexcept>
 
 namespace hph {
 namespace kern {
 
-// exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
-// Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
-// fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of
-// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp keeps k inside int range; for
-// x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for x <= 709.
-__device__ __forceinline__ double exp_fp64(double x) {
-    x = fmax(x, -1000.0);
-    const double shifter = 6755399441055744.0; // 1.5 * 2^52
-    const double kf = __builtin_fma(x, HPH_LOG2E, shifter);
-    const double k = kf - shifter;
-    const int ki = (int)(unsigned)__builtin_bit_cast(unsigned long long, kf);
-    double r = __builtin_fma(-k, HPH_LN2_HI, x);
-    r = __builtin_fma(-k, HPH_LN2_LO, r);
-    double p = HPH_EXP_C10;
-    p = __builtin_fma(p, r, HPH_EXP_C9);
-    p = __builtin_fma(p, r, HPH_EXP_C8);
-    p = __builtin_fma(p, r, HPH_EXP_C7);
-    p = __builtin_fma(p, r, HPH_EXP_C6);
-    p = __builtin_fma(p, r, HPH_EXP_C5);
-    p = __builtin_fma(p, r, HPH_EXP_C4);
-    p = __builtin_fma(p, r, HPH_EXP_C3);
-    p = __builtin_fma(p, r, HPH_EXP_C2);
-    p = __builtin_fma(p, r, HPH_EXP_C1);
-    p = __builtin_fma(p, r, HPH_EXP_C0);
-    return __builtin_amdgcn_ldexp(p, ki);
-}
-
-enum { kSNone = 0, kSFull = 1, kSMixed = 2 };
-
-// Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.
-template <int D> struct TileStride {
-    static constexpr int value = (D + 2) & ~1;
-};
 
 // One LDS tile against this lane's row. DOB says whether the background term is formed (it is
 // not when every background term of the tile is exactly zero). SMODE says whether the
@@ -175,36 +141,6 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
     }
 }
 
-// ---- double-double helpers (Knuth TwoSum) ----
-__device__ __forceinline__ double2 dd_add(double2 a, double2 b) {
-    const double s = a.x + b.x;
-    const double bb = s - a.x;
-    double e = (a.x - (s - bb)) + (b.x - bb);
-    e += a.y + b.y;
-    const double hi = s + e;
-    return make_double2(hi, e - (hi - s));
-}
-
-__device__ __forceinline__ double ipow(double x, int e) {
-    double r = 1.0;
-    for (int q = 0; q < e; ++q) r *= x;
-    return r;
-}
-
-// Fixed-order tree reduction of one double-double per thread over a 256-thread block.
-__device__ double2 block_reduce_dd(double2 v, double2 *sh) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int s = kRowsPerBlock / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] = dd_add(sh[tid], sh[tid + s]);
-        __syncthreads();
-    }
-    const double2 r = sh[0];
-    __syncthreads();
-    return r;
-}
-
 // Per-row terms: ll_i = log(lambda_i) + theta (e^{-omega (T - t_i)} - 1)
 //                       - mu0 (Phi(tauT (T - t_i)) - Phi(-tauT t_i))   (src/NewHawkes.hpp:652-655)
 // and d ll_i / d(sigma, tauX, tauT, omega, theta, mu0).
@@ -218,67 +154,42 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
     const int k = blockIdx.y;
     const int rb = blockIdx.x;
     const int i = rb * kRowsPerBlock + threadIdx.x;
-    const int n = ev.n, npad = ev.npad, D = ev.dim;
+    const int n = ev.n, npad = ev.npad;
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < n) {
-        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double a6[NACC];
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) a6[a] = 0.0;
         for (int c = 0; c < g.chunks; ++c) {
             const double *base = part + ((size_t)k * g.chunks + c) * NACC * npad;
 #pragma unroll
-            for (int a = 0; a < NACC; ++a) s[a] += base[(size_t)a * npad + i];
+            for (int a = 0; a < NACC; ++a) a6[a] += base[(size_t)a * npad + i];
         }
-        const double sB = s[0], sS = GRAD ? s[3] : s[1];
-        const double *pk = params + 6 * k;
-        const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3], theta = pk[4], mu0 = pk[5];
-        const double inv_2pi = 0.15915494309189533577;
-        const double inv_sqrt_2pi = 0.39894228040143267794;
-        const double sigmaD = ipow(sigma, D), tauXD = ipow(tauX, D);
-        const double bg = mu0 * tauXD * tauT * inv_2pi * sB;
-        const double se = theta * omega * sigmaD * inv_sqrt_2pi * sS;
-        const double lam = bg + se;
-        const double ti = ev.t[i];
-        const double dT = ev.t_last - ti;
-        const double eT = exp(-omega * dT);
-        const double phiA = 0.5 * erfc(-tauT * dT * M_SQRT1_2);
-        const double phiB = 0.5 * erfc(tauT * ti * M_SQRT1_2);
-        v[0] = log(lam) + theta * (eT - 1.0) - mu0 * (phiA - phiB);
+        double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (GRAD) {
-            const double pB = sB * inv_2pi, pBr = s[1] * inv_2pi, pBd = s[2] * inv_2pi;
-            const double pS = sS * inv_sqrt_2pi, pSr = s[4] * inv_sqrt_2pi, pSd = s[5] * inv_sqrt_2pi;
-            const double inv = 1.0 / lam;
-            const double zA = tauT * dT, zB = tauT * ti;
-            const double pdfA = exp(-0.5 * zA * zA) * inv_sqrt_2pi;
-            const double pdfB = exp(-0.5 * zB * zB) * inv_sqrt_2pi;
-            v[1] = theta * omega * ipow(sigma, D - 1) * (D * pS - sigma * sigma * pSr) * inv;
-            v[2] = mu0 * tauT * ipow(tauX, D - 1) * (D * pB - tauX * tauX * pBr) * inv;
-            v[3] = mu0 * tauXD * (pB - tauT * tauT * pBd) * inv - mu0 * (dT * pdfA + ti * pdfB);
-            v[4] = theta * sigmaD * (pS - omega * pSd) * inv - theta * dT * eT;
-            v[5] = omega * sigmaD * pS * inv + (eT - 1.0);
-            v[6] = tauXD * tauT * pB * inv - (phiA - phiB);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) s[a] = a6[a];
+        } else {
+            s[0] = a6[0];
+            s[3] = a6[NACC - 1];
         }
-        if (probs != nullptr && k == 0) probs[ev.perm[i]] = se / (bg + se);
+        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v);
+        if (probs != nullptr && k == 0) probs[ev.perm[i]] = p;
     }
-    for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
-        const double2 r = block_reduce_dd(make_double2(v[q], 0.0), sh);
-        if (threadIdx.x == 0) {
-            double *o = block_out + (((size_t)k * g.row_blocks + rb) * kResultWidth + q) * 2;
-            o[0] = r.x;
-            o[1] = r.y;
-        }
-    }
+    store_block_sums<GRAD>(v, sh, block_out, k, g.row_blocks, rb);
 }
 
 // One block per k: fixed-order double-double sum of the block partials, plus the
 // N (D-1) log(1/sqrt(2 pi)) constant (src/NewHawkes.hpp:659).
-__global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, PairGrid g, int width,
+__global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nblocks, int width,
                                                                const double *__restrict__ block_out,
                                                                double *__restrict__ result) {
     __shared__ double2 sh[kRowsPerBlock];
     const int k = blockIdx.x;
     for (int q = 0; q < width; ++q) {
         double2 a = make_double2(0.0, 0.0);
-        for (int b = threadIdx.x; b < g.row_blocks; b += kRowsPerBlock) {
-            const double *o = block_out + (((size_t)k * g.row_blocks + b) * kResultWidth + q) * 2;
+        for (int b = threadIdx.x; b < nblocks; b += kRowsPerBlock) {
+            const double *o = block_out + (((size_t)k * nblocks + b) * kResultWidth + q) * 2;
             a = dd_add(a, make_double2(o[0], o[1]));
         }
         double2 r = block_reduce_dd(a, sh);
@@ -346,10 +257,10 @@ void launch_finalize(const Events &ev, const double *params, int K, const PairGr
                            block_out, probs);
 }
 
-void launch_reduce(const Events &ev, int K, const PairGrid &g, bool grad, const double *block_out,
-                   double *result, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, g, grad ? kResultWidth : 1,
-                       block_out, result);
+void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, nblocks,
+                       grad ? kResultWidth : 1, block_out, result);
 }
 
 } // namespace kern

@@ -102,6 +102,10 @@ class Engine:
         check(_lib.lib().hph_get_log_likelihood_batch(self._h, dptr(ll), dptr(g) if gradient else None, K))
         return (ll, g) if gradient else ll
 
+    def set_pair_kernel(self, kind):
+        """_lib.PAIR_KERNEL_AUTO / PAIR_KERNEL_ROWWISE / PAIR_KERNEL_SYMMETRIC."""
+        check(_lib.lib().hph_set_pair_kernel(self._h, int(kind)))
+
     # -- kernel timing (HIP events on the engine's stream) --
     def set_timing(self, enabled=True):
         check(_lib.lib().hph_set_timing(self._h, 1 if enabled else 0))

@@ -110,6 +110,15 @@ int hph_get_log_likelihood_batch(hph_engine *engine, double *loglik, double *gra
 int hph_set_timing(hph_engine *engine, int enabled);
 int hph_get_timing(hph_engine *engine, double *out, size_t length);
 
+/* ---- Pair-pass selection (extension) ----
+ * AUTO: symmetric pass for dimensions 1..3 and at least 4096 events, row-wise pass otherwise.
+ * ROWWISE: every ordered pair, one event row per lane. SYMMETRIC: background terms once per
+ * unordered pair (dimensions 1..3). Results agree to rounding; each is deterministic. */
+#define HPH_PAIR_KERNEL_AUTO 0
+#define HPH_PAIR_KERNEL_ROWWISE 1
+#define HPH_PAIR_KERNEL_SYMMETRIC 2
+int hph_set_pair_kernel(hph_engine *engine, int kind);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,7 @@ class AbstractHawkes {
     virtual void setParametersBatch(const double *, size_t) = 0;
     virtual void getLogLikelihoodBatch(double *loglik, double *gradient, size_t K) = 0;
     virtual void setTiming(bool enabled) = 0;
+    virtual void setPairKernel(int kind) = 0;
     virtual void getTiming(double *out, size_t length) = 0;
 
   protected:

@@ -23,11 +23,16 @@ GRAD_TOL = 1e-12
 PROB_TOL = 1e-12
 
 
-def make_engine(locs, times, params):
+KERNELS = {"auto": _lib.PAIR_KERNEL_AUTO, "rowwise": _lib.PAIR_KERNEL_ROWWISE,
+           "symmetric": _lib.PAIR_KERNEL_SYMMETRIC}
+
+
+def make_engine(locs, times, params, kernel="auto", flags=_lib.OPENCL):
     locs = np.asarray(locs, dtype=np.float64)
     if locs.ndim == 1:
         locs = locs[:, None]
-    e = Engine(locs.shape[1], locs.shape[0])
+    e = Engine(locs.shape[1], locs.shape[0], flags=flags)
+    e.set_pair_kernel(KERNELS[kernel])
     e.update_locations(locs)
     e.set_times(times)
     e.set_parameters(params)
@@ -39,9 +44,9 @@ def grad_err(g, ref):
     return np.abs(g - go) / np.maximum(np.maximum(np.abs(go), 1e-3 * scale), 1e-300)
 
 
-def assert_matches_oracle(locs, times, params, probs=True, precision="long", name=""):
+def assert_matches_oracle(locs, times, params, probs=True, precision="long", name="", kernel="auto"):
     ref = oracle.total(locs, times, params, precision=precision, probs=probs)
-    e = make_engine(locs, times, params)
+    e = make_engine(locs, times, params, kernel=kernel)
     ll, g = e.log_likelihood_and_gradient()
     assert rel_err(ll, ref["ll"]) <= LL_TOL, (name, ll, ref["ll"])
     assert grad_err(g, ref).max() <= GRAD_TOL, (name, grad_err(g, ref))
@@ -53,10 +58,13 @@ def assert_matches_oracle(locs, times, params, probs=True, precision="long", nam
     return ref
 
 
-def test_golden_vectors(golden):
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_golden_vectors(golden, kernel):
     for c in golden:
-        print("golden case", c["name"], flush=True)
-        e = make_engine(c["locations"], c["times"], c["params"])
+        if kernel == "symmetric" and c["D"] > 3:
+            continue
+        print("golden case", c["name"], kernel, flush=True)
+        e = make_engine(c["locations"], c["times"], c["params"], kernel=kernel)
         ll, g = e.log_likelihood_and_gradient()
         ref = oracle.total(c["locations"], c["times"], c["params"])
         assert rel_err(ll, c["ll"]) <= LL_TOL, c["name"]
@@ -67,47 +75,53 @@ def test_golden_vectors(golden):
         e.close()
 
 
-def test_uniform_n1000_config():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_uniform_n1000_config(kernel):
     locs, times, p = synth.workload("uniform_n1000")
-    assert_matches_oracle(locs, times, p, name="uniform_n1000")
+    assert_matches_oracle(locs, times, p, name="uniform_n1000", kernel=kernel)
 
 
-def test_cluster_process_small():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_cluster_process_small(kernel):
     locs, times = synth.cluster_process(2800, side=np.sqrt(4000 / 100), seed=777)
-    assert_matches_oracle(locs, times, synth.P0, name="cluster_4k")
+    assert_matches_oracle(locs, times, synth.P0, name="cluster_4k", kernel=kernel)
 
 
-@pytest.mark.parametrize("D", [1, 3, 5, 8])
-def test_dimensions(D):
+@pytest.mark.parametrize("D,kernel", [(1, "rowwise"), (3, "rowwise"), (5, "rowwise"), (8, "rowwise"),
+                                      (1, "symmetric"), (3, "symmetric")])
+def test_dimensions(D, kernel):
     rng = np.random.default_rng(100 + D)
-    n = 700
+    n = 1700
     locs = rng.uniform(0, 5, (n, D))
     times = np.sort(rng.uniform(0, 500, n))
     p = np.array([0.9, 0.7, 0.05, 0.4, 0.8, 0.6])
-    assert_matches_oracle(locs, times, p, name="D%d" % D)
+    assert_matches_oracle(locs, times, p, name="D%d" % D, kernel=kernel)
 
 
-def test_unsorted_times_and_ties():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_unsorted_times_and_ties(kernel):
     rng = np.random.default_rng(21)
-    n = 900
+    n = 1900
     locs = rng.uniform(0, 4, (n, 2))
     times = np.round(rng.uniform(0, 300, n))  # many ties, not sorted; T = last element
     p = np.array([1.1, 0.8, 0.02, 0.3, 0.9, 0.7])
-    assert_matches_oracle(locs, times, p, name="unsorted_ties")
+    assert_matches_oracle(locs, times, p, name="unsorted_ties", kernel=kernel)
 
 
-def test_row_counts_around_block_sizes():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_row_counts_around_block_sizes(kernel):
     rng = np.random.default_rng(4)
-    for n in (1, 2, 63, 64, 65, 255, 256, 257, 513):
+    for n in (1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1025, 1537):
         locs = rng.uniform(0, 3, (n, 2))
         times = np.sort(rng.uniform(0, 100, n))
-        assert_matches_oracle(locs, times, [1.0, 0.9, 0.1, 0.5, 0.6, 0.9], name="n%d" % n)
+        assert_matches_oracle(locs, times, [1.0, 0.9, 0.1, 0.5, 0.6, 0.9], name="n%d" % n, kernel=kernel)
 
 
-def test_batch_equals_single_calls_bitwise():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_batch_equals_single_calls_bitwise(kernel):
     locs, times = synth.cluster_process(1500, seed=31)
     P = synth.jitter_params(8)
-    e = make_engine(locs, times, P[0])
+    e = make_engine(locs, times, P[0], kernel=kernel)
     e.set_parameters_batch(P)
     ll_b, g_b = e.log_likelihood_batch()
     for k in range(8):
@@ -124,16 +138,14 @@ def test_batch_equals_single_calls_bitwise():
     ("small_omega_large_tauT", [1.0, 1 / 1.6, 1 / 3.0, 0.05, 1.0, 1.0]),
     ("large_omega", [1.0, 1 / 1.6, 1 / 336.0, 40.0, 1.0, 1.0]),
 ])
-def test_zero_tile_skip_is_bitwise_exact(name, params):
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_zero_tile_skip_is_bitwise_exact(name, params, kernel):
     """Skipping tiles whose terms all underflow to +0.0 gives bitwise the same results as
     evaluating every pair (HPH_FLAG_ALL_PAIRS)."""
     locs, times = synth.cluster_process(6000, side=20.0, horizon=43_800.0, seed=41)
     out = []
     for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
-        e = Engine(2, len(times), flags=flags)
-        e.update_locations(locs)
-        e.set_times(times)
-        e.set_parameters(params)
+        e = make_engine(locs, times, params, kernel=kernel, flags=flags)
         ll, g = e.log_likelihood_and_gradient()
         out.append((ll, g, e.probs_self_excite()))
         e.close()
@@ -154,14 +166,20 @@ def test_repeat_calls_are_bitwise_identical():
         assert np.array_equal(pa, e.probs_self_excite())
 
 
-def test_config2_full_parity_and_finite_differences():
+@pytest.fixture(scope="module")
+def config2_reference():
+    locs, times, p = synth.workload("cluster_b35k")
+    return locs, times, p, oracle.total(locs, times, p, precision="double")
+
+
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_config2_full_parity_and_finite_differences(config2_reference, kernel):
     """Cluster process, 35,000 background events (N ~ 49.5k): FP64 ll + 6-gradient against the
     compensated double oracle, and the gradient against central finite differences of the GPU
     log-likelihood (h = 1e-6 relative)."""
-    locs, times, p = synth.workload("cluster_b35k")
-    print("config2 N =", len(times), flush=True)
-    ref = oracle.total(locs, times, p, precision="double")
-    e = make_engine(locs, times, p)
+    locs, times, p, ref = config2_reference
+    print("config2 N =", len(times), kernel, flush=True)
+    e = make_engine(locs, times, p, kernel=kernel)
     ll, g = e.log_likelihood_and_gradient()
     assert rel_err(ll, ref["ll"]) <= LL_TOL
     assert grad_err(g, ref).max() <= GRAD_TOL
@@ -177,21 +195,34 @@ def test_config2_full_parity_and_finite_differences():
         assert abs(fd - g[k]) <= 1e-5 * max(abs(g[k]), 1e-3 * ref["scale"][1 + k]), (k, fd, g[k])
 
 
-def test_full_size_rows_against_oracle():
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_full_size_rows_against_oracle(kernel):
     """N ~ 1e5 (70,000 background events): per-event probabilities of sampled rows against
     the oracle's row sums over all N columns, and translation invariance of the total."""
     locs, times, p = synth.workload("cluster_b70k")
     n = len(times)
-    e = make_engine(locs, times, p)
+    e = make_engine(locs, times, p, kernel=kernel)
     probs = e.probs_self_excite()
     ll, g = e.log_likelihood_and_gradient()
-    for lo in (0, n // 3, n - 64):
+    for lo in (0, 511, n // 3, n - 600, n - 64):
         rows = oracle.rows(locs, times, p, lo, lo + 64, precision="double")
         np.testing.assert_allclose(probs[lo:lo + 64], rows[:, 8], rtol=PROB_TOL, atol=0)
     assert probs[0] == 0.0 and np.all((probs >= 0) & (probs <= 1))
     e.update_locations(locs + np.array([1000.0, -250.0]))
     ll2 = e.log_likelihood()
     assert rel_err(ll2, ll) <= 1e-12
+    e.close()
+
+
+def test_kernels_agree_at_full_size():
+    locs, times, p = synth.workload("cluster_b70k")
+    res = []
+    for kernel in ("rowwise", "symmetric"):
+        e = make_engine(locs, times, p, kernel=kernel)
+        res.append(e.log_likelihood_and_gradient())
+        e.close()
+    assert rel_err(res[1][0], res[0][0]) <= 1e-13
+    assert np.all(np.abs(res[1][1] - res[0][1]) <= 1e-12 * np.maximum(np.abs(res[0][1]), 1.0))
 
 
 def test_single_location_update_and_state():
