@@ -17,7 +17,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -81,6 +86,77 @@ struct HostBuf {
     }
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
+
+// Pair-pass configuration chosen by the autotuner.
+struct TuneConfig {
+    int kind = HPH_PAIR_KERNEL_ROWWISE; // HPH_PAIR_KERNEL_ROWWISE or HPH_PAIR_KERNEL_SYMMETRIC
+    int rows_per_lane = 2;              // symmetric pass: event rows per lane
+    int target_blocks = 4096;           // grid size the work decomposition aims for
+};
+
+// Shape of a tuned problem: log2 of N, dimension, batch size, gradient, precision mode.
+struct TuneKey {
+    int log2n = 0, dim = 0, K = 0, grad = 0, fp32 = 0;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(log2n, dim, K, grad, fp32) < std::tie(o.log2n, o.dim, o.K, o.grad, o.fp32);
+    }
+};
+
+// Process-wide tuning results, shared by all engines. When HPH_TUNE_CACHE names a file, results
+// are read from it at first use and appended to it after each tuning run (one line per shape:
+// "log2n dim K grad fp32 kind rows_per_lane target_blocks").
+class TuneCache {
+  public:
+    bool lookup(const TuneKey &k, TuneConfig *out) {
+        std::lock_guard<std::mutex> lock(mu_);
+        load();
+        auto it = map_.find(k);
+        if (it == map_.end()) return false;
+        *out = it->second;
+        return true;
+    }
+    void store(const TuneKey &k, const TuneConfig &c) {
+        std::lock_guard<std::mutex> lock(mu_);
+        load();
+        map_[k] = c;
+        const char *path = std::getenv("HPH_TUNE_CACHE");
+        if (path && *path) {
+            if (FILE *f = std::fopen(path, "a")) {
+                std::fprintf(f, "%d %d %d %d %d %d %d %d\n", k.log2n, k.dim, k.K, k.grad, k.fp32, c.kind,
+                             c.rows_per_lane, c.target_blocks);
+                std::fclose(f);
+            }
+        }
+    }
+
+  private:
+    void load() {
+        if (loaded_) return;
+        loaded_ = true;
+        const char *path = std::getenv("HPH_TUNE_CACHE");
+        if (!path || !*path) return;
+        FILE *f = std::fopen(path, "r");
+        if (!f) return;
+        TuneKey k;
+        TuneConfig c;
+        while (std::fscanf(f, "%d %d %d %d %d %d %d %d", &k.log2n, &k.dim, &k.K, &k.grad, &k.fp32, &c.kind,
+                           &c.rows_per_lane, &c.target_blocks) == 8) {
+            const bool ok = (c.kind == HPH_PAIR_KERNEL_ROWWISE || c.kind == HPH_PAIR_KERNEL_SYMMETRIC) &&
+                            (c.rows_per_lane == 1 || c.rows_per_lane == 2) && c.target_blocks >= 64 &&
+                            c.target_blocks <= (1 << 20) && (c.kind == HPH_PAIR_KERNEL_ROWWISE || k.dim <= 3);
+            if (ok) map_[k] = c;
+        }
+        std::fclose(f);
+    }
+    std::mutex mu_;
+    bool loaded_ = false;
+    std::map<TuneKey, TuneConfig> map_;
+};
+
+static TuneCache &tune_cache() {
+    static TuneCache cache;
+    return cache;
+}
 
 class HipHawkes : public AbstractHawkes {
   public:
@@ -250,12 +326,81 @@ class HipHawkes : public AbstractHawkes {
         timing_ = enabled;
     }
 
-    void setPairKernel(int kind) override {
+    // Times every candidate configuration on the current data and parameters (LL + gradient for
+    // K = batch size, or 1) and keeps the fastest for this shape in the process-wide cache.
+    void autotune(int repeats, double *best_ms) override {
+        if (!have_params_ && batch_.empty()) throw NotInitialized("parameters are not set (setParameters)");
+        if (repeats < 1) repeats = 1;
+        const bool use_batch = !batch_.empty();
+        const int K = use_batch ? (int)(batch_.size() / 6) : 1;
+        const double *params = use_batch ? batch_.data() : params_;
+        std::vector<TuneConfig> cands;
+        for (int target : {1024, 2048, 4096, 8192}) {
+            TuneConfig c;
+            c.kind = HPH_PAIR_KERNEL_ROWWISE;
+            c.target_blocks = target;
+            cands.push_back(c);
+            if (embeddingDimension <= 3 && kernel_ != HPH_PAIR_KERNEL_ROWWISE) {
+                for (int A : {1, 2}) {
+                    TuneConfig d;
+                    d.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+                    d.rows_per_lane = A;
+                    d.target_blocks = target;
+                    cands.push_back(d);
+                }
+            }
+        }
+        if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC) {
+            std::vector<TuneConfig> only;
+            for (auto &c : cands)
+                if (c.kind == HPH_PAIR_KERNEL_SYMMETRIC) only.push_back(c);
+            cands.swap(only);
+        }
+        const bool was_timing = timing_;
+        setTiming(true);
+        TuneConfig best;
+        double best_t = 1e300;
+        for (const TuneConfig &c : cands) {
+            run_with(params, K, true, nullptr, c); // warm: plans, buffers, code objects
+            double t = 1e300;
+            for (int r = 0; r < repeats; ++r) {
+                run_with(params, K, true, nullptr, c);
+                t = std::min(t, last_ms_[0] + last_ms_[1]);
+            }
+            if (t < best_t) {
+                best_t = t;
+                best = c;
+            }
+        }
+        timing_ = was_timing;
+        tune_cache().store(tune_key(K, true), best);
+        tune_cache().store(tune_key(K, false), best);
+        if (best_ms) *best_ms = best_t;
+        tuned_ = best;
+    }
+
+    void getTunedConfig(int *out, size_t length) override {
+        if (out == nullptr || length < 3) throw std::invalid_argument("tuned-config output needs 3 values");
+        const TuneConfig c = config_for(1, true);
+        out[0] = c.kind;
+        out[1] = c.rows_per_lane;
+        out[2] = c.target_blocks;
+    }
+
+    void setPairKernel(int kind) override { setPairConfig(kind, 2, 4096); }
+
+    void setPairConfig(int kind, int rows_per_lane, int target_blocks) override {
         if (kind < HPH_PAIR_KERNEL_AUTO || kind > HPH_PAIR_KERNEL_SYMMETRIC)
             throw std::invalid_argument("unknown pair kernel kind");
         if (kind == HPH_PAIR_KERNEL_SYMMETRIC && embeddingDimension > 3)
             throw std::invalid_argument("the symmetric pair kernel is built for dimensions 1..3");
+        if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
+        if (target_blocks < 64 || target_blocks > (1 << 20))
+            throw std::invalid_argument("target grid size must be in 64..2^20");
         kernel_ = kind;
+        forced_.kind = kind == HPH_PAIR_KERNEL_AUTO ? HPH_PAIR_KERNEL_ROWWISE : kind;
+        forced_.rows_per_lane = rows_per_lane;
+        forced_.target_blocks = target_blocks;
     }
 
     void getTiming(double *out, size_t length) override {
@@ -312,17 +457,41 @@ class HipHawkes : public AbstractHawkes {
         return ev;
     }
 
-    // Symmetric pass for D <= 3 once the data spans a few row blocks; the row-wise pass otherwise.
-    bool use_symmetric() const {
-        if (kernel_ == HPH_PAIR_KERNEL_ROWWISE) return false;
-        if (embeddingDimension > 3) return false;
-        if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC) return true;
-        return locationCount >= kSymMinEvents;
+    // Pair-pass configuration for K parameter vectors: the autotuned one when this engine or an
+    // earlier one in the process has tuned this shape, else the built-in choice (symmetric pass
+    // with 2 rows per lane for D <= 3 and at least 4096 events, row-wise otherwise).
+    TuneConfig config_for(int K, bool grad) const {
+        TuneConfig c;
+        const bool sym_ok = embeddingDimension <= 3;
+        if (kernel_ != HPH_PAIR_KERNEL_AUTO) {
+            c = forced_;
+        } else if (!sym_ok) {
+            c.kind = HPH_PAIR_KERNEL_ROWWISE;
+        } else {
+            TuneConfig t;
+            if (tune_cache().lookup(tune_key(K, grad), &t)) return t;
+            c.kind = locationCount >= kSymMinEvents ? HPH_PAIR_KERNEL_SYMMETRIC : HPH_PAIR_KERNEL_ROWWISE;
+        }
+        return c;
+    }
+
+    TuneKey tune_key(int K, bool grad) const {
+        TuneKey key;
+        int b = 0;
+        while ((1 << (b + 1)) <= locationCount) ++b;
+        key.log2n = b;
+        key.dim = embeddingDimension;
+        key.K = K;
+        key.grad = grad ? 1 : 0;
+        key.fp32 = fp32_ ? 1 : 0;
+        return key;
     }
 
     // Runs the pair pass, finalize and reduction kernels for K parameter vectors; results land in
     // h_result_ ([K][7]).
-    void run(const double *params, int K, bool grad, double *probs_host) {
+    void run(const double *params, int K, bool grad, double *probs_host) { run_with(params, K, grad, probs_host, config_for(K, grad)); }
+
+    void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg) {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
         if (data_dirty_) upload();
@@ -339,15 +508,17 @@ class HipHawkes : public AbstractHawkes {
         const kern::Events ev = events();
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         double *probs_dev = probs_host ? d_probs_.as<double>() : nullptr;
-        if (use_symmetric()) {
-            if (!sym_valid_ || sym_K_ != K) {
-                sym_ = kern::plan_sym_grid(locationCount, embeddingDimension, K, kTargetBlocks);
+        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
+            if (!sym_valid_ || sym_K_ != K || sym_A_ != cfg.rows_per_lane || sym_target_ != cfg.target_blocks) {
+                sym_ = kern::plan_sym_grid(locationCount, embeddingDimension, K, cfg.target_blocks, cfg.rows_per_lane);
                 d_items_.ensure(sizeof(int2) * sym_.items.size());
                 check(hipMemcpy(d_items_.ptr, sym_.items.data(), sizeof(int2) * sym_.items.size(),
                                 hipMemcpyHostToDevice),
                       "upload work items");
                 sym_valid_ = true;
                 sym_K_ = K;
+                sym_A_ = cfg.rows_per_lane;
+                sym_target_ = cfg.target_blocks;
             }
             kern::SymGrid g = sym_.grid;
             g.item = d_items_.as<int2>();
@@ -363,7 +534,7 @@ class HipHawkes : public AbstractHawkes {
                                       d_colpart_.as<double>(), d_block_.as<double>(), probs_dev, stream_);
             check(hipGetLastError(), "finalize kernel launch");
         } else {
-            kern::PairGrid g = kern::plan_pair_grid(locationCount, K, kTargetBlocks);
+            kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
             const int nacc = grad ? 6 : 2;
             d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
@@ -407,12 +578,11 @@ class HipHawkes : public AbstractHawkes {
         }
     }
 
-    static constexpr int kTargetBlocks = 4096;
     static constexpr int kSymMinEvents = 4096;
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     kern::SymPlan sym_;
     bool sym_valid_ = false;
-    int sym_K_ = 0;
+    int sym_K_ = 0, sym_A_ = 0, sym_target_ = 0;
 
     int device_;
     bool fp32_;
@@ -434,6 +604,7 @@ class HipHawkes : public AbstractHawkes {
     std::vector<double> stored_locs_;
     double stored_ll_ = 0.0;
 
+    TuneConfig tuned_, forced_;
     bool timing_ = false;
     hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
     double last_ms_[2] = {0.0, 0.0};
@@ -595,6 +766,21 @@ int hph_get_log_likelihood_batch(hph_engine *e, double *loglik, double *grad, si
 int hph_set_timing(hph_engine *e, int enabled) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->setTiming(enabled != 0); });
+}
+
+int hph_autotune(hph_engine *e, int repeats, double *best_ms) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->autotune(repeats, best_ms); });
+}
+
+int hph_get_tuned_config(hph_engine *e, int *out, size_t length) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->getTunedConfig(out, length); });
+}
+
+int hph_set_pair_config(hph_engine *e, int kind, int rows_per_lane, int target_blocks) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->setPairConfig(kind, rows_per_lane, target_blocks); });
 }
 
 int hph_set_pair_kernel(hph_engine *e, int kind) {

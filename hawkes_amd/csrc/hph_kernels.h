@@ -75,7 +75,7 @@ struct SymPlan {
     std::vector<int2> items; // host copy of the work items, heaviest first
 };
 
-SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks);
+SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lane);
 size_t sym_rowpart_doubles(const SymGrid &g, int K, bool grad);
 size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad);
 void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, double *rowpart,

@@ -362,10 +362,11 @@ static long long sym_items_for(int nb, int R, int L) {
     return items;
 }
 
-SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks) {
+SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lane) {
+    if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
     SymPlan plan;
     SymGrid &g = plan.grid;
-    g.A = 2;
+    g.A = rows_per_lane;
     g.R = kSymWaves * 64 * g.A;
     g.nb = (n + g.R - 1) / g.R;
     const int max_len = ((g.nb > 1 ? (g.nb - 1) * g.R : g.R) + kTile - 1) / kTile * kTile;
@@ -401,12 +402,17 @@ template <int D, bool GRAD>
 static void launch_sym_d(const Events &ev, const double *params, int K, const SymGrid &g, double *rowpart,
                          double *colpart, hipStream_t s) {
     dim3 grid((unsigned)g.items, (unsigned)K);
-    hipLaunchKernelGGL((sym_kernel<D, GRAD, 2>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart, colpart);
+    if (g.A == 2)
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
+                           colpart);
+    else
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
+                           colpart);
 }
 
 void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, double *rowpart,
                 double *colpart, hipStream_t s) {
-    if (g.A != 2) throw std::invalid_argument("symmetric pass is built for 2 rows per lane");
+    if (g.A != 1 && g.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
     switch (ev.dim * 2 + (grad ? 1 : 0)) {
     case 2: launch_sym_d<1, false>(ev, params, K, g, rowpart, colpart, s); break;
     case 3: launch_sym_d<1, true>(ev, params, K, g, rowpart, colpart, s); break;

@@ -106,6 +106,22 @@ class Engine:
         """_lib.PAIR_KERNEL_AUTO / PAIR_KERNEL_ROWWISE / PAIR_KERNEL_SYMMETRIC."""
         check(_lib.lib().hph_set_pair_kernel(self._h, int(kind)))
 
+    def set_pair_config(self, kind, rows_per_lane=2, target_blocks=4096):
+        check(_lib.lib().hph_set_pair_config(self._h, int(kind), int(rows_per_lane), int(target_blocks)))
+
+    def autotune(self, repeats=2):
+        """Times the pair-pass configurations on the current data and parameters, keeps the
+        fastest (process-wide, and in the file named by HPH_TUNE_CACHE); returns its ms."""
+        best = np.zeros(1)
+        check(_lib.lib().hph_autotune(self._h, int(repeats), dptr(best)))
+        return float(best[0])
+
+    def tuned_config(self):
+        """(pair kernel kind, rows per lane, target grid size) used for a single evaluation."""
+        out = (ctypes.c_int * 3)()
+        check(_lib.lib().hph_get_tuned_config(self._h, out, 3))
+        return tuple(out)
+
     # -- kernel timing (HIP events on the engine's stream) --
     def set_timing(self, enabled=True):
         check(_lib.lib().hph_set_timing(self._h, 1 if enabled else 0))

@@ -118,6 +118,21 @@ int hph_get_timing(hph_engine *engine, double *out, size_t length);
 #define HPH_PAIR_KERNEL_ROWWISE 1
 #define HPH_PAIR_KERNEL_SYMMETRIC 2
 int hph_set_pair_kernel(hph_engine *engine, int kind);
+/* Full form: rows_per_lane (1 or 2) applies to the symmetric pass; target_blocks (64..2^20) is
+ * the grid size the work decomposition aims for. hph_set_pair_kernel(e, k) is
+ * hph_set_pair_config(e, k, 2, 4096). */
+int hph_set_pair_config(hph_engine *engine, int kind, int rows_per_lane, int target_blocks);
+
+/* ---- Autotuner (extension) ----
+ * Times every pair-pass configuration (row-wise; symmetric with 1 or 2 rows per lane; four
+ * grid sizes) on the engine's current data and parameters (the batch when one is set), keeps
+ * the fastest for this shape (log2 N, dimension, batch size, precision) for every engine of the
+ * process, and appends it to the file named by the HPH_TUNE_CACHE environment variable, if set,
+ * which later processes read. best_ms (may be NULL) receives the winning kernel time.
+ * hph_get_tuned_config: out[0] = pair kernel kind, out[1] = rows per lane, out[2] = target grid
+ * size of the configuration a single-vector evaluation would use now; length must be >= 3. */
+int hph_autotune(hph_engine *engine, int repeats, double *best_ms);
+int hph_get_tuned_config(hph_engine *engine, int *out, size_t length);
 
 #ifdef __cplusplus
 }

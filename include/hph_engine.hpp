@@ -46,6 +46,9 @@ class AbstractHawkes {
     virtual void getLogLikelihoodBatch(double *loglik, double *gradient, size_t K) = 0;
     virtual void setTiming(bool enabled) = 0;
     virtual void setPairKernel(int kind) = 0;
+    virtual void setPairConfig(int kind, int rows_per_lane, int target_blocks) = 0;
+    virtual void autotune(int repeats, double *best_ms) = 0;
+    virtual void getTunedConfig(int *out, size_t length) = 0;
     virtual void getTiming(double *out, size_t length) = 0;
 
   protected:

@@ -23,8 +23,10 @@ GRAD_TOL = 1e-12
 PROB_TOL = 1e-12
 
 
-KERNELS = {"auto": _lib.PAIR_KERNEL_AUTO, "rowwise": _lib.PAIR_KERNEL_ROWWISE,
-           "symmetric": _lib.PAIR_KERNEL_SYMMETRIC}
+KERNELS = {"auto": (_lib.PAIR_KERNEL_AUTO, 2, 4096), "rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096),
+           "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
+           "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
+           "rowwise_small_grid": (_lib.PAIR_KERNEL_ROWWISE, 2, 64)}
 
 
 def make_engine(locs, times, params, kernel="auto", flags=_lib.OPENCL):
@@ -32,7 +34,7 @@ def make_engine(locs, times, params, kernel="auto", flags=_lib.OPENCL):
     if locs.ndim == 1:
         locs = locs[:, None]
     e = Engine(locs.shape[1], locs.shape[0], flags=flags)
-    e.set_pair_kernel(KERNELS[kernel])
+    e.set_pair_config(*KERNELS[kernel])
     e.update_locations(locs)
     e.set_times(times)
     e.set_parameters(params)
@@ -81,7 +83,7 @@ def test_uniform_n1000_config(kernel):
     assert_matches_oracle(locs, times, p, name="uniform_n1000", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_small_grid"])
 def test_cluster_process_small(kernel):
     locs, times = synth.cluster_process(2800, side=np.sqrt(4000 / 100), seed=777)
     assert_matches_oracle(locs, times, synth.P0, name="cluster_4k", kernel=kernel)
@@ -108,7 +110,7 @@ def test_unsorted_times_and_ties(kernel):
     assert_matches_oracle(locs, times, p, name="unsorted_ties", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
 def test_row_counts_around_block_sizes(kernel):
     rng = np.random.default_rng(4)
     for n in (1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1025, 1537):
@@ -195,7 +197,7 @@ def test_config2_full_parity_and_finite_differences(config2_reference, kernel):
         assert abs(fd - g[k]) <= 1e-5 * max(abs(g[k]), 1e-3 * ref["scale"][1 + k]), (k, fd, g[k])
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
 def test_full_size_rows_against_oracle(kernel):
     """N ~ 1e5 (70,000 background events): per-event probabilities of sampled rows against
     the oracle's row sums over all N columns, and translation invariance of the total."""
@@ -223,6 +225,24 @@ def test_kernels_agree_at_full_size():
         e.close()
     assert rel_err(res[1][0], res[0][0]) <= 1e-13
     assert np.all(np.abs(res[1][1] - res[0][1]) <= 1e-12 * np.maximum(np.abs(res[0][1]), 1.0))
+
+
+def test_autotune_picks_a_candidate_and_keeps_parity(tmp_path, monkeypatch):
+    locs, times = synth.cluster_process(6000, seed=77)
+    p = synth.P0
+    ref = oracle.total(locs, times, p, precision="double")
+    e = make_engine(locs, times, p)
+    ms = e.autotune(repeats=1)
+    assert ms > 0
+    kind, a, target = e.tuned_config()
+    assert kind in (_lib.PAIR_KERNEL_ROWWISE, _lib.PAIR_KERNEL_SYMMETRIC) and a in (1, 2) and target >= 64
+    e.set_parameters(p)
+    ll, g = e.log_likelihood_and_gradient()
+    assert rel_err(ll, ref["ll"]) <= LL_TOL
+    assert grad_err(g, ref).max() <= GRAD_TOL
+    # a second engine of the same shape starts from the tuned configuration
+    e2 = make_engine(locs, times, p)
+    assert e2.tuned_config() == (kind, a, target)
 
 
 def test_single_location_update_and_state():
