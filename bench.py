@@ -169,15 +169,16 @@ def main():
         for s in range(args.warmup):
             e.set_parameters(params[s])
             e.log_likelihood_and_gradient()
-        pair_ms, fin_ms, formed = [], [], []
+        pair_ms, fin_ms, formed, results = [], [], [], []
         t0 = time.perf_counter()
         for s in range(steps):
             e.set_parameters(params[args.warmup + s])
-            e.log_likelihood_and_gradient()
+            results.append(e.log_likelihood_and_gradient())
             a, b = e.timing()
             pair_ms.append(a)
             fin_ms.append(b)
         wall = time.perf_counter() - t0
+        run.results = results
         kind, rows_per_lane, _ = e.tuned_config()
         for s in range(steps):
             skip = not (flags & _lib.ALL_PAIRS)
@@ -188,6 +189,7 @@ def main():
         return e, wall, pair_ms, fin_ms, formed
 
     e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps)
+    fp64_results = run.results
     ms_per_step = 1e3 * wall / args.steps
     value = n * n / (wall / args.steps)
 
@@ -220,6 +222,21 @@ def main():
         "roofline_frac": flop_a / (float(np.mean(pair_a)) * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
     }
     ea.close()
+
+    # FP32-bulk mode: FP32 pair arithmetic, FP64 accumulation; error against the FP64 results above.
+    ef, wall_f, pair_f, _, _ = run(_lib.OPENCL | _lib.FLOAT, args.steps)
+    ll_err = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(run.results, fp64_results))
+    g_err = max(float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-3 * np.max(np.abs(b[1])))))
+                for a, b in zip(run.results, fp64_results))
+    fp32 = {
+        "value": n * n / (wall_f / args.steps),
+        "ms_per_step": 1e3 * wall_f / args.steps,
+        "kernel_ms": float(np.mean(pair_f)),
+        "kernel": kernel_name(ef.tuned_config()),
+        "rel_err_ll_vs_fp64": ll_err,
+        "rel_err_grad_vs_fp64": g_err,
+    }
+    ef.close()
 
     batched = None
     if args.batch > 0:
@@ -256,6 +273,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "all_pairs": all_pairs,
+        "fp32_bulk": fp32,
     }
     print(json.dumps(out))
 

@@ -10,6 +10,24 @@ namespace kern {
 // Excitation-term mode of a tile: absent, unmasked (every column strictly earlier), masked per pair.
 enum { kSNone = 0, kSFull = 1, kSMixed = 2 };
 
+// FP32-bulk mode: exp arguments pre-scaled by log2(e) so one v_exp_f32 (2^x) forms each term.
+struct F32Consts {
+    float bx, bt; // -tauX^2/2 * log2 e, -tauT^2/2 * log2 e
+    float sx, nw; // -sigma^2/2 * log2 e, -omega * log2 e
+};
+
+__device__ __forceinline__ F32Consts make_f32_consts(double cBx, double cBt, double cSx, double nomega) {
+    const double l2e = 1.4426950408889634;
+    F32Consts c;
+    c.bx = (float)(cBx * l2e);
+    c.bt = (float)(cBt * l2e);
+    c.sx = (float)(cSx * l2e);
+    c.nw = (float)(nomega * l2e);
+    return c;
+}
+
+__device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp2f(x); }
+
 // exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
 // Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
 // fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of

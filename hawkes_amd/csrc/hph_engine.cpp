@@ -164,7 +164,6 @@ class HipHawkes : public AbstractHawkes {
         : AbstractHawkes(D, N, flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
         if (D < 1 || D > kern::kMaxDim) throw std::invalid_argument("embeddingDimension must be in 1..8");
         if (N < 1) throw std::invalid_argument("locationCount must be >= 1");
-        if (fp32_) throw std::invalid_argument("FLOAT (FP32-bulk) mode is not built in this library yet");
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device_ >= count) throw std::invalid_argument("device index out of range");
@@ -523,11 +522,12 @@ class HipHawkes : public AbstractHawkes {
             kern::SymGrid g = sym_.grid;
             g.item = d_items_.as<int2>();
             g.skip_zero = skip_zero;
+            g.zero_arg = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
             d_part_.ensure(sizeof(double) * kern::sym_rowpart_doubles(g, K, grad));
             d_colpart_.ensure(sizeof(double) * kern::sym_colpart_doubles(g, K, grad));
             if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-            kern::launch_sym(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(), d_colpart_.as<double>(),
-                             stream_);
+            kern::launch_sym(ev, d_params_.as<double>(), K, g, grad, fp32_, d_part_.as<double>(),
+                             d_colpart_.as<double>(), stream_);
             check(hipGetLastError(), "symmetric pair kernel launch");
             if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
             kern::launch_sym_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
@@ -536,6 +536,7 @@ class HipHawkes : public AbstractHawkes {
         } else {
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
+            g.zero_arg = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
             const int nacc = grad ? 6 : 2;
             d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
             if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");

@@ -30,11 +30,14 @@ struct PairGrid {
     int chunks;        // column chunks per row block
     int chunk_len;     // events per chunk (multiple of kTile)
     int skip_zero;     // 1: skip tiles whose every term underflows to exactly +0.0
+    double zero_arg;   // exp arguments <= -zero_arg give exactly +0.0 (kZeroExpArg / kZeroExpArg32)
 };
 
 // exp(x) is exactly +0.0 in FP64 for x <= -745.14 (both here and in the reference's exp);
 // a tile is skipped only when every one of its arguments is <= -kZeroExpArg.
 constexpr double kZeroExpArg = 746.0;
+// FP32-bulk mode: v_exp_f32 (2^y) is exactly +0.0 for y <= -150, i.e. x <= -103.98.
+constexpr double kZeroExpArg32 = 105.0;
 
 PairGrid plan_pair_grid(int n, int K, int target_blocks);
 
@@ -68,6 +71,7 @@ struct SymGrid {
     const int2 *item;    // device array of work items
     long long col_stride;// column-partial doubles per parameter vector
     int skip_zero;       // as PairGrid::skip_zero
+    double zero_arg;     // as PairGrid::zero_arg
 };
 
 struct SymPlan {
@@ -78,8 +82,8 @@ struct SymPlan {
 SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lane);
 size_t sym_rowpart_doubles(const SymGrid &g, int K, bool grad);
 size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad);
-void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, double *rowpart,
-                double *colpart, hipStream_t s);
+void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, bool fp32,
+                double *rowpart, double *colpart, hipStream_t s);
 void launch_sym_finalize(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
                          const double *rowpart, const double *colpart, double *block_out, double *probs,
                          hipStream_t s);

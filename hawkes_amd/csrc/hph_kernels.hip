@@ -65,9 +65,68 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
     }
 }
 
+// FP32-bulk variant of tile_pass: differences formed in FP64 and rounded to FP32, FP32
+// Gaussian/exponential arithmetic with v_exp_f32, FP32 sums over runs of 64 columns added to the
+// FP64 accumulators.
+template <int D, bool GRAD, bool DOB, int SMODE>
+__device__ __forceinline__ void tile_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
+                                              double ti, const F32Consts &fc, double (&acc)[6]) {
+    constexpr int ST = TileStride<D>::value;
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+        const int jn = min(cnt, j0 + 64);
+        float a32[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+        for (int j = j0; j < jn; ++j) {
+            const double *e = tile + j * ST;
+            float r2;
+            {
+                const float d0 = (float)(xi[0] - e[0]);
+                r2 = d0 * d0;
+            }
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const float dd = (float)(xi[d] - e[d]);
+                r2 = __builtin_fmaf(dd, dd, r2);
+            }
+            const double dt64 = ti - e[D];
+            const float dt = (float)dt64;
+            if (DOB) {
+                const float dt2 = dt * dt;
+                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
+                a32[0] += eb;
+                if (GRAD) {
+                    a32[1] = __builtin_fmaf(eb, r2, a32[1]);
+                    a32[2] = __builtin_fmaf(eb, dt2, a32[2]);
+                }
+            }
+            if (SMODE != kSNone) {
+                float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
+                if (SMODE == kSMixed) es = dt64 > 0.0 ? es : 0.0f;
+                a32[3] += es;
+                if (GRAD) {
+                    a32[4] = __builtin_fmaf(es, r2, a32[4]);
+                    a32[5] = __builtin_fmaf(es, dt, a32[5]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += (double)a32[q];
+    }
+}
+
+template <int D, bool GRAD, bool F32, bool DOB, int SMODE>
+__device__ __forceinline__ void tile_run(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
+                                         double cBx, double cBt, double cSx, double nomega, const F32Consts &fc,
+                                         double (&acc)[6]) {
+    if (F32)
+        tile_pass_f32<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, fc, acc);
+    else
+        tile_pass<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
+}
+
 // Grid: x = row_blocks * chunks (later row blocks first: they carry more excitation work),
 // y = parameter vector k. Block: 256 lanes, one row each.
-template <int D, bool GRAD>
+template <int D, bool GRAD, bool F32>
 __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const double *__restrict__ params,
                                                              PairGrid g, double *__restrict__ part) {
     constexpr int ST = TileStride<D>::value;
@@ -81,6 +140,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
     const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
     const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
+    const F32Consts fc = make_f32_consts(cBx, cBt, cSx, nomega);
 
     const int tid = threadIdx.x;
     const int n = ev.n, npad = ev.npad;
@@ -117,16 +177,16 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
                 const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
                 // Background: arg <= -tauT^2 dt^2 / 2; excitation (tile before the rows only):
                 // arg <= -omega dt. Terms whose argument is <= -kZeroExpArg are exactly +0.0.
-                if (-cBt * gap * gap >= kZeroExpArg) dob = 0;
-                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= kZeroExpArg) smode = kSNone;
+                if (-cBt * gap * gap >= g.zero_arg) dob = 0;
+                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg) smode = kSNone;
             }
             const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
             switch (mode) {
-            case 4 + kSNone: tile_pass<D, GRAD, true, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
-            case 4 + kSFull: tile_pass<D, GRAD, true, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
-            case 4 + kSMixed: tile_pass<D, GRAD, true, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
-            case kSFull: tile_pass<D, GRAD, false, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
-            case kSMixed: tile_pass<D, GRAD, false, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc); break;
+            case 4 + kSNone: tile_run<D, GRAD, F32, true, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
+            case 4 + kSFull: tile_run<D, GRAD, F32, true, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
+            case 4 + kSMixed: tile_run<D, GRAD, F32, true, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
+            case kSFull: tile_run<D, GRAD, F32, false, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
+            case kSMixed: tile_run<D, GRAD, F32, false, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
             default: break; // every term of the tile is exactly zero
             }
         }
@@ -217,33 +277,42 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
     g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
     g.skip_zero = 1;
+    g.zero_arg = kZeroExpArg;
     return g;
 }
 
-template <int D>
+template <int D, bool F32>
 static void launch_pair_d(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
                           double *part, hipStream_t s) {
     dim3 grid((unsigned)(g.row_blocks * g.chunks), (unsigned)K);
     if (grad)
-        hipLaunchKernelGGL((pair_kernel<D, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
+        hipLaunchKernelGGL((pair_kernel<D, true, F32>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
     else
-        hipLaunchKernelGGL((pair_kernel<D, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
+        hipLaunchKernelGGL((pair_kernel<D, false, F32>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
+}
+
+template <bool F32>
+static void launch_pair_p(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
+                          double *part, hipStream_t s) {
+    switch (ev.dim) {
+    case 1: launch_pair_d<1, F32>(ev, params, K, g, grad, part, s); break;
+    case 2: launch_pair_d<2, F32>(ev, params, K, g, grad, part, s); break;
+    case 3: launch_pair_d<3, F32>(ev, params, K, g, grad, part, s); break;
+    case 4: launch_pair_d<4, F32>(ev, params, K, g, grad, part, s); break;
+    case 5: launch_pair_d<5, F32>(ev, params, K, g, grad, part, s); break;
+    case 6: launch_pair_d<6, F32>(ev, params, K, g, grad, part, s); break;
+    case 7: launch_pair_d<7, F32>(ev, params, K, g, grad, part, s); break;
+    case 8: launch_pair_d<8, F32>(ev, params, K, g, grad, part, s); break;
+    default: throw std::invalid_argument("embedding dimension must be 1..8");
+    }
 }
 
 void launch_pair(const Events &ev, const double *params, int K, const PairGrid &g, bool grad, bool fp32,
                  double *part, hipStream_t s) {
-    if (fp32) throw std::invalid_argument("FP32-bulk pair pass is not built in this library");
-    switch (ev.dim) {
-    case 1: launch_pair_d<1>(ev, params, K, g, grad, part, s); break;
-    case 2: launch_pair_d<2>(ev, params, K, g, grad, part, s); break;
-    case 3: launch_pair_d<3>(ev, params, K, g, grad, part, s); break;
-    case 4: launch_pair_d<4>(ev, params, K, g, grad, part, s); break;
-    case 5: launch_pair_d<5>(ev, params, K, g, grad, part, s); break;
-    case 6: launch_pair_d<6>(ev, params, K, g, grad, part, s); break;
-    case 7: launch_pair_d<7>(ev, params, K, g, grad, part, s); break;
-    case 8: launch_pair_d<8>(ev, params, K, g, grad, part, s); break;
-    default: throw std::invalid_argument("embedding dimension must be 1..8");
-    }
+    if (fp32)
+        launch_pair_p<true>(ev, params, K, g, grad, part, s);
+    else
+        launch_pair_p<false>(ev, params, K, g, grad, part, s);
 }
 
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,

@@ -130,33 +130,170 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     }
 }
 
-template <int D, bool GRAD, int A, bool TAIL>
+__device__ __forceinline__ float lane_shift_down_f32(float v, int byte_addr) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(byte_addr, __builtin_bit_cast(int, v)));
+}
+
+// FP32-bulk variant of sym_subtile: differences are formed in FP64 and rounded to FP32, the
+// Gaussian/exponential arithmetic and exp (v_exp_f32) are FP32, the subtile's 64-step sums are
+// FP32 and are added to the FP64 row accumulators (and returned as FP64 column sums) at its end.
+template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL>
+__device__ __forceinline__ void sym_subtile_f32(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
+                                                const double (&ti)[A], const double (&vm)[A], const F32Consts &fc,
+                                                double (&acc)[A][6], double (&ca64)[GRAD ? 3 : 1]) {
+    constexpr int ST = TileStride<D>::value;
+    constexpr int NC = GRAD ? 3 : 1;
+    const int src = ((lane + 1) & 63) << 2;
+    float a32[A][6];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a32[a][q] = 0.0f;
+    float ca[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ca[c] = 0.0f;
+    float fvm[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) fvm[a] = (float)vm[a];
+#pragma unroll 2
+    for (int s = 0; s < 64; ++s) {
+        const double *e = sub + (lane + s) * ST;
+        double q[D + 1];
+#pragma unroll
+        for (int d = 0; d <= D; ++d) q[d] = e[d];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            float r2;
+            {
+                const float d0 = (float)(xi[a][0] - q[0]);
+                r2 = d0 * d0;
+            }
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const float dd = (float)(xi[a][d] - q[d]);
+                r2 = __builtin_fmaf(dd, dd, r2);
+            }
+            const double dt64 = ti[a] - q[D];
+            const float dt = (float)dt64;
+            if (DOB) {
+                const float dt2 = dt * dt;
+                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
+                a32[a][0] += eb;
+                if (GRAD) {
+                    a32[a][1] = __builtin_fmaf(eb, r2, a32[a][1]);
+                    a32[a][2] = __builtin_fmaf(eb, dt2, a32[a][2]);
+                }
+                const float ec = TAIL ? eb * fvm[a] : eb;
+                ca[0] += ec;
+                if (GRAD) {
+                    ca[1] = __builtin_fmaf(ec, r2, ca[1]);
+                    ca[2] = __builtin_fmaf(ec, dt2, ca[2]);
+                }
+            }
+            if (SMODE != kSNone) {
+                float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
+                if (SMODE == kSMixed) es = dt64 > 0.0 ? es : 0.0f;
+                a32[a][3] += es;
+                if (GRAD) {
+                    a32[a][4] = __builtin_fmaf(es, r2, a32[a][4]);
+                    a32[a][5] = __builtin_fmaf(es, dt, a32[a][5]);
+                }
+            }
+        }
+        if (DOB) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) ca[c] = lane_shift_down_f32(ca[c], src);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[a][q] += (double)a32[a][q];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ca64[c] = (double)ca[c];
+}
+
+// FP32-bulk variant of diag_pass: FP32 sums over runs of 64 columns, added to FP64 accumulators.
+template <int D, bool GRAD>
+__device__ __forceinline__ void diag_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
+                                              double ti, const F32Consts &fc, double (&acc)[6]) {
+    constexpr int ST = TileStride<D>::value;
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+        const int jn = min(cnt, j0 + 64);
+        float a32[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+        for (int j = j0; j < jn; ++j) {
+            const double *e = tile + j * ST;
+            float r2;
+            {
+                const float d0 = (float)(xi[0] - e[0]);
+                r2 = d0 * d0;
+            }
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const float dd = (float)(xi[d] - e[d]);
+                r2 = __builtin_fmaf(dd, dd, r2);
+            }
+            const double dt64 = ti - e[D];
+            const float dt = (float)dt64;
+            const float dt2 = dt * dt;
+            const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
+            a32[0] += eb;
+            if (GRAD) {
+                a32[1] = __builtin_fmaf(eb, r2, a32[1]);
+                a32[2] = __builtin_fmaf(eb, dt2, a32[2]);
+            }
+            float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
+            es = dt64 > 0.0 ? es : 0.0f;
+            a32[3] += es;
+            if (GRAD) {
+                a32[4] = __builtin_fmaf(es, r2, a32[4]);
+                a32[5] = __builtin_fmaf(es, dt, a32[5]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += (double)a32[q];
+    }
+}
+
+template <int D, bool GRAD, int A, bool TAIL, bool F32, bool DOB, int SMODE>
+__device__ __forceinline__ void sym_run(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
+                                        const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
+                                        double cSx, double nomega, const F32Consts &fc, double (&acc)[A][6],
+                                        double (&ca)[GRAD ? 3 : 1]) {
+    if (F32)
+        sym_subtile_f32<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, fc, acc, ca);
+    else
+        sym_subtile<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+}
+
+template <int D, bool GRAD, int A, bool TAIL, bool F32>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
-                                             double cBx, double cBt, double cSx, double nomega, double (&acc)[A][6],
-                                             double (&ca)[GRAD ? 3 : 1]) {
+                                             double cBx, double cBt, double cSx, double nomega, const F32Consts &fc,
+                                             double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_run<D, GRAD, A, TAIL, F32, true, kSNone>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_run<D, GRAD, A, TAIL, F32, true, kSFull>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_run<D, GRAD, A, TAIL, F32, true, kSMixed>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_run<D, GRAD, A, TAIL, F32, false, kSFull>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_run<D, GRAD, A, TAIL, F32, false, kSMixed>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
         break;
     default: break; // every term of the subtile is exactly zero
     }
 }
 
 // Grid: x = work item (row block, chunk of earlier columns | diagonal), y = parameter vector k.
-template <int D, bool GRAD, int A>
+template <int D, bool GRAD, int A, bool F32>
 __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const double *__restrict__ params,
                                                              SymGrid g, double *__restrict__ rowpart,
                                                              double *__restrict__ colpart) {
@@ -173,6 +310,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
     const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
     const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
+    const F32Consts fc = make_f32_consts(cBx, cBt, cSx, nomega);
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = ev.n, npad = ev.npad;
@@ -215,7 +353,12 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
             __syncthreads();
             if (wave_live) {
 #pragma unroll
-                for (int a = 0; a < A; ++a) diag_pass<D, GRAD>(tile, cnt, xi[a], ti[a], cBx, cBt, cSx, nomega, acc[a]);
+                for (int a = 0; a < A; ++a) {
+                    if (F32)
+                        diag_pass_f32<D, GRAD>(tile, cnt, xi[a], ti[a], fc, acc[a]);
+                    else
+                        diag_pass<D, GRAD>(tile, cnt, xi[a], ti[a], cBx, cBt, cSx, nomega, acc[a]);
+                }
             }
         }
     } else {
@@ -247,8 +390,8 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
                 int dob = 1;
                 if (g.skip_zero) {
                     const double gap = tl < t_lo ? t_lo - tl : 0.0;
-                    if (-cBt * gap * gap >= kZeroExpArg) dob = 0;
-                    if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= kZeroExpArg) smode = kSNone;
+                    if (-cBt * gap * gap >= g.zero_arg) dob = 0;
+                    if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg) smode = kSNone;
                 }
                 const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                 for (int sb = 0; sb < 4; ++sb) {
@@ -257,9 +400,11 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
                     for (int c = 0; c < NC; ++c) ca[c] = 0.0;
                     const double *sub = tile + sb * 128 * ST;
                     if (tail)
-                        sym_dispatch<D, GRAD, A, true>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+                        sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc,
+                                                            ca);
                     else
-                        sym_dispatch<D, GRAD, A, false>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+                        sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc,
+                                                             acc, ca);
 #pragma unroll
                     for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
                 }
@@ -377,6 +522,7 @@ SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lan
     const int NC = 3;
     g.col_stride = (long long)NC * g.R * ((long long)g.nb * (g.nb - 1) / 2);
     g.skip_zero = 1;
+    g.zero_arg = kZeroExpArg;
     plan.items.clear();
     for (int P = g.nb - 1; P >= 1; --P) {
         const int nch = (P * g.R + L - 1) / L;
@@ -398,30 +544,39 @@ size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad) {
     return (size_t)K * (size_t)g.col_stride;
 }
 
-template <int D, bool GRAD>
+template <int D, bool GRAD, bool F32>
 static void launch_sym_d(const Events &ev, const double *params, int K, const SymGrid &g, double *rowpart,
                          double *colpart, hipStream_t s) {
     dim3 grid((unsigned)g.items, (unsigned)K);
     if (g.A == 2)
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
                            colpart);
     else
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
                            colpart);
 }
 
-void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, double *rowpart,
-                double *colpart, hipStream_t s) {
-    if (g.A != 1 && g.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
+template <bool F32>
+static void launch_sym_p(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
+                         double *rowpart, double *colpart, hipStream_t s) {
     switch (ev.dim * 2 + (grad ? 1 : 0)) {
-    case 2: launch_sym_d<1, false>(ev, params, K, g, rowpart, colpart, s); break;
-    case 3: launch_sym_d<1, true>(ev, params, K, g, rowpart, colpart, s); break;
-    case 4: launch_sym_d<2, false>(ev, params, K, g, rowpart, colpart, s); break;
-    case 5: launch_sym_d<2, true>(ev, params, K, g, rowpart, colpart, s); break;
-    case 6: launch_sym_d<3, false>(ev, params, K, g, rowpart, colpart, s); break;
-    case 7: launch_sym_d<3, true>(ev, params, K, g, rowpart, colpart, s); break;
+    case 2: launch_sym_d<1, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
+    case 3: launch_sym_d<1, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
+    case 4: launch_sym_d<2, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
+    case 5: launch_sym_d<2, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
+    case 6: launch_sym_d<3, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
+    case 7: launch_sym_d<3, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
     default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
+}
+
+void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, bool fp32,
+                double *rowpart, double *colpart, hipStream_t s) {
+    if (g.A != 1 && g.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
+    if (fp32)
+        launch_sym_p<true>(ev, params, K, g, grad, rowpart, colpart, s);
+    else
+        launch_sym_p<false>(ev, params, K, g, grad, rowpart, colpart, s);
 }
 
 void launch_sym_finalize(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
