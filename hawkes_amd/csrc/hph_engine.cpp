@@ -522,7 +522,8 @@ class HipHawkes : public AbstractHawkes {
             kern::SymGrid g = sym_.grid;
             g.item = d_items_.as<int2>();
             g.skip_zero = skip_zero;
-            g.zero_arg = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+            g.zero_arg_b = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+            g.zero_arg_s = kern::kZeroExpArg;
             d_part_.ensure(sizeof(double) * kern::sym_rowpart_doubles(g, K, grad));
             d_colpart_.ensure(sizeof(double) * kern::sym_colpart_doubles(g, K, grad));
             if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
@@ -536,7 +537,8 @@ class HipHawkes : public AbstractHawkes {
         } else {
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
-            g.zero_arg = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+            g.zero_arg_b = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+            g.zero_arg_s = kern::kZeroExpArg;
             const int nacc = grad ? 6 : 2;
             d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
             if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");

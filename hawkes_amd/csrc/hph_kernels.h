@@ -30,13 +30,14 @@ struct PairGrid {
     int chunks;        // column chunks per row block
     int chunk_len;     // events per chunk (multiple of kTile)
     int skip_zero;     // 1: skip tiles whose every term underflows to exactly +0.0
-    double zero_arg;   // exp arguments <= -zero_arg give exactly +0.0 (kZeroExpArg / kZeroExpArg32)
+    double zero_arg_b; // background exp arguments <= -zero_arg_b give exactly +0.0
+    double zero_arg_s; // excitation exp arguments <= -zero_arg_s give exactly +0.0
 };
 
 // exp(x) is exactly +0.0 in FP64 for x <= -745.14 (both here and in the reference's exp);
 // a tile is skipped only when every one of its arguments is <= -kZeroExpArg.
 constexpr double kZeroExpArg = 746.0;
-// FP32-bulk mode: v_exp_f32 (2^y) is exactly +0.0 for y <= -150, i.e. x <= -103.98.
+// FP32-bulk mode (background term): v_exp_f32 (2^y) is exactly +0.0 for y <= -150, i.e. x <= -103.98.
 constexpr double kZeroExpArg32 = 105.0;
 
 PairGrid plan_pair_grid(int n, int K, int target_blocks);
@@ -71,7 +72,8 @@ struct SymGrid {
     const int2 *item;    // device array of work items
     long long col_stride;// column-partial doubles per parameter vector
     int skip_zero;       // as PairGrid::skip_zero
-    double zero_arg;     // as PairGrid::zero_arg
+    double zero_arg_b;   // as PairGrid::zero_arg_b
+    double zero_arg_s;   // as PairGrid::zero_arg_s
 };
 
 struct SymPlan {

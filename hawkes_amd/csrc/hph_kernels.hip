@@ -65,52 +65,55 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
     }
 }
 
-// FP32-bulk variant of tile_pass: differences formed in FP64 and rounded to FP32, FP32
-// Gaussian/exponential arithmetic with v_exp_f32, FP32 sums over runs of 64 columns added to the
-// FP64 accumulators.
+// FP32-bulk variant of tile_pass: distances and time differences in FP64; the background term in
+// FP32 (v_exp_f32), summed in FP32 over runs of 64 columns and added to the FP64 accumulators; the
+// excitation term in FP64 (its exponent reaches tens, where FP32 rounding of the exponent would
+// cost relative accuracy |x| * 6e-8).
 template <int D, bool GRAD, bool DOB, int SMODE>
 __device__ __forceinline__ void tile_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
-                                              double ti, const F32Consts &fc, double (&acc)[6]) {
+                                              double ti, double cSx, double nomega, const F32Consts &fc,
+                                              double (&acc)[6]) {
     constexpr int ST = TileStride<D>::value;
     for (int j0 = 0; j0 < cnt; j0 += 64) {
         const int jn = min(cnt, j0 + 64);
-        float a32[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float b32[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 2
         for (int j = j0; j < jn; ++j) {
             const double *e = tile + j * ST;
-            float r2;
+            double r2;
             {
-                const float d0 = (float)(xi[0] - e[0]);
+                const double d0 = xi[0] - e[0];
                 r2 = d0 * d0;
             }
 #pragma unroll
             for (int d = 1; d < D; ++d) {
-                const float dd = (float)(xi[d] - e[d]);
-                r2 = __builtin_fmaf(dd, dd, r2);
+                const double dd = xi[d] - e[d];
+                r2 = __builtin_fma(dd, dd, r2);
             }
-            const double dt64 = ti - e[D];
-            const float dt = (float)dt64;
+            const double dt = ti - e[D];
             if (DOB) {
-                const float dt2 = dt * dt;
-                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
-                a32[0] += eb;
+                const float r2f = (float)r2;
+                const float dtf = (float)dt;
+                const float dt2 = dtf * dtf;
+                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
+                b32[0] += eb;
                 if (GRAD) {
-                    a32[1] = __builtin_fmaf(eb, r2, a32[1]);
-                    a32[2] = __builtin_fmaf(eb, dt2, a32[2]);
+                    b32[1] = __builtin_fmaf(eb, r2f, b32[1]);
+                    b32[2] = __builtin_fmaf(eb, dt2, b32[2]);
                 }
             }
             if (SMODE != kSNone) {
-                float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
-                if (SMODE == kSMixed) es = dt64 > 0.0 ? es : 0.0f;
-                a32[3] += es;
+                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
+                acc[3] += es;
                 if (GRAD) {
-                    a32[4] = __builtin_fmaf(es, r2, a32[4]);
-                    a32[5] = __builtin_fmaf(es, dt, a32[5]);
+                    acc[4] = __builtin_fma(es, r2, acc[4]);
+                    acc[5] = __builtin_fma(es, dt, acc[5]);
                 }
             }
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) acc[q] += (double)a32[q];
+        for (int q = 0; q < 3; ++q) acc[q] += (double)b32[q];
     }
 }
 
@@ -119,7 +122,7 @@ __device__ __forceinline__ void tile_run(const double *__restrict__ tile, int cn
                                          double cBx, double cBt, double cSx, double nomega, const F32Consts &fc,
                                          double (&acc)[6]) {
     if (F32)
-        tile_pass_f32<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, fc, acc);
+        tile_pass_f32<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, cSx, nomega, fc, acc);
     else
         tile_pass<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
 }
@@ -177,8 +180,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
                 const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
                 // Background: arg <= -tauT^2 dt^2 / 2; excitation (tile before the rows only):
                 // arg <= -omega dt. Terms whose argument is <= -kZeroExpArg are exactly +0.0.
-                if (-cBt * gap * gap >= g.zero_arg) dob = 0;
-                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg) smode = kSNone;
+                if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
+                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
             }
             const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
             switch (mode) {
@@ -277,7 +280,8 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
     g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
     g.skip_zero = 1;
-    g.zero_arg = kZeroExpArg;
+    g.zero_arg_b = kZeroExpArg;
+    g.zero_arg_s = kZeroExpArg;
     return g;
 }
 

@@ -134,21 +134,25 @@ __device__ __forceinline__ float lane_shift_down_f32(float v, int byte_addr) {
     return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(byte_addr, __builtin_bit_cast(int, v)));
 }
 
-// FP32-bulk variant of sym_subtile: differences are formed in FP64 and rounded to FP32, the
-// Gaussian/exponential arithmetic and exp (v_exp_f32) are FP32, the subtile's 64-step sums are
-// FP32 and are added to the FP64 row accumulators (and returned as FP64 column sums) at its end.
+// FP32-bulk variant of sym_subtile. Distances and time differences are formed in FP64; the
+// background term (the bulk of the pairs) is evaluated in FP32 with v_exp_f32 and summed in FP32
+// over the subtile's 64 steps, then added to the FP64 row accumulators (and returned as FP64
+// column sums). The excitation term stays FP64: its exponent -omega dt - sigma^2 r^2/2 reaches
+// tens, and FP32 rounding of the exponent would give relative errors |x| * 6e-8 in the
+// per-event probabilities; after exact-zero skipping it is a small fraction of the pairs.
 template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL>
 __device__ __forceinline__ void sym_subtile_f32(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
-                                                const double (&ti)[A], const double (&vm)[A], const F32Consts &fc,
-                                                double (&acc)[A][6], double (&ca64)[GRAD ? 3 : 1]) {
+                                                const double (&ti)[A], const double (&vm)[A], double cSx,
+                                                double nomega, const F32Consts &fc, double (&acc)[A][6],
+                                                double (&ca64)[GRAD ? 3 : 1]) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NC = GRAD ? 3 : 1;
     const int src = ((lane + 1) & 63) << 2;
-    float a32[A][6];
+    float b32[A][3];
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) a32[a][q] = 0.0f;
+        for (int q = 0; q < 3; ++q) b32[a][q] = 0.0f;
     float ca[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) ca[c] = 0.0f;
@@ -163,40 +167,41 @@ __device__ __forceinline__ void sym_subtile_f32(const double *__restrict__ sub, 
         for (int d = 0; d <= D; ++d) q[d] = e[d];
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-            float r2;
+            double r2;
             {
-                const float d0 = (float)(xi[a][0] - q[0]);
+                const double d0 = xi[a][0] - q[0];
                 r2 = d0 * d0;
             }
 #pragma unroll
             for (int d = 1; d < D; ++d) {
-                const float dd = (float)(xi[a][d] - q[d]);
-                r2 = __builtin_fmaf(dd, dd, r2);
+                const double dd = xi[a][d] - q[d];
+                r2 = __builtin_fma(dd, dd, r2);
             }
-            const double dt64 = ti[a] - q[D];
-            const float dt = (float)dt64;
+            const double dt = ti[a] - q[D];
             if (DOB) {
-                const float dt2 = dt * dt;
-                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
-                a32[a][0] += eb;
+                const float r2f = (float)r2;
+                const float dtf = (float)dt;
+                const float dt2 = dtf * dtf;
+                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
+                b32[a][0] += eb;
                 if (GRAD) {
-                    a32[a][1] = __builtin_fmaf(eb, r2, a32[a][1]);
-                    a32[a][2] = __builtin_fmaf(eb, dt2, a32[a][2]);
+                    b32[a][1] = __builtin_fmaf(eb, r2f, b32[a][1]);
+                    b32[a][2] = __builtin_fmaf(eb, dt2, b32[a][2]);
                 }
                 const float ec = TAIL ? eb * fvm[a] : eb;
                 ca[0] += ec;
                 if (GRAD) {
-                    ca[1] = __builtin_fmaf(ec, r2, ca[1]);
+                    ca[1] = __builtin_fmaf(ec, r2f, ca[1]);
                     ca[2] = __builtin_fmaf(ec, dt2, ca[2]);
                 }
             }
             if (SMODE != kSNone) {
-                float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
-                if (SMODE == kSMixed) es = dt64 > 0.0 ? es : 0.0f;
-                a32[a][3] += es;
+                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
+                acc[a][3] += es;
                 if (GRAD) {
-                    a32[a][4] = __builtin_fmaf(es, r2, a32[a][4]);
-                    a32[a][5] = __builtin_fmaf(es, dt, a32[a][5]);
+                    acc[a][4] = __builtin_fma(es, r2, acc[a][4]);
+                    acc[a][5] = __builtin_fma(es, dt, acc[a][5]);
                 }
             }
         }
@@ -208,51 +213,54 @@ __device__ __forceinline__ void sym_subtile_f32(const double *__restrict__ sub, 
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
-        for (int q = 0; q < 6; ++q) acc[a][q] += (double)a32[a][q];
+        for (int q = 0; q < 3; ++q) acc[a][q] += (double)b32[a][q];
 #pragma unroll
     for (int c = 0; c < NC; ++c) ca64[c] = (double)ca[c];
 }
 
-// FP32-bulk variant of diag_pass: FP32 sums over runs of 64 columns, added to FP64 accumulators.
+// FP32-bulk variant of diag_pass: background in FP32 summed over runs of 64 columns, excitation
+// in FP64.
 template <int D, bool GRAD>
 __device__ __forceinline__ void diag_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
-                                              double ti, const F32Consts &fc, double (&acc)[6]) {
+                                              double ti, double cSx, double nomega, const F32Consts &fc,
+                                              double (&acc)[6]) {
     constexpr int ST = TileStride<D>::value;
     for (int j0 = 0; j0 < cnt; j0 += 64) {
         const int jn = min(cnt, j0 + 64);
-        float a32[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float b32[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 2
         for (int j = j0; j < jn; ++j) {
             const double *e = tile + j * ST;
-            float r2;
+            double r2;
             {
-                const float d0 = (float)(xi[0] - e[0]);
+                const double d0 = xi[0] - e[0];
                 r2 = d0 * d0;
             }
 #pragma unroll
             for (int d = 1; d < D; ++d) {
-                const float dd = (float)(xi[d] - e[d]);
-                r2 = __builtin_fmaf(dd, dd, r2);
+                const double dd = xi[d] - e[d];
+                r2 = __builtin_fma(dd, dd, r2);
             }
-            const double dt64 = ti - e[D];
-            const float dt = (float)dt64;
-            const float dt2 = dt * dt;
-            const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2));
-            a32[0] += eb;
+            const double dt = ti - e[D];
+            const float r2f = (float)r2;
+            const float dtf = (float)dt;
+            const float dt2 = dtf * dtf;
+            const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
+            b32[0] += eb;
             if (GRAD) {
-                a32[1] = __builtin_fmaf(eb, r2, a32[1]);
-                a32[2] = __builtin_fmaf(eb, dt2, a32[2]);
+                b32[1] = __builtin_fmaf(eb, r2f, b32[1]);
+                b32[2] = __builtin_fmaf(eb, dt2, b32[2]);
             }
-            float es = exp2_f32(__builtin_fmaf(fc.sx, r2, fc.nw * dt));
-            es = dt64 > 0.0 ? es : 0.0f;
-            a32[3] += es;
+            double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+            es = dt > 0.0 ? es : 0.0;
+            acc[3] += es;
             if (GRAD) {
-                a32[4] = __builtin_fmaf(es, r2, a32[4]);
-                a32[5] = __builtin_fmaf(es, dt, a32[5]);
+                acc[4] = __builtin_fma(es, r2, acc[4]);
+                acc[5] = __builtin_fma(es, dt, acc[5]);
             }
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) acc[q] += (double)a32[q];
+        for (int q = 0; q < 3; ++q) acc[q] += (double)b32[q];
     }
 }
 
@@ -262,7 +270,7 @@ __device__ __forceinline__ void sym_run(const double *__restrict__ sub, int lane
                                         double cSx, double nomega, const F32Consts &fc, double (&acc)[A][6],
                                         double (&ca)[GRAD ? 3 : 1]) {
     if (F32)
-        sym_subtile_f32<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, fc, acc, ca);
+        sym_subtile_f32<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, cSx, nomega, fc, acc, ca);
     else
         sym_subtile<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
 }
@@ -355,7 +363,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
 #pragma unroll
                 for (int a = 0; a < A; ++a) {
                     if (F32)
-                        diag_pass_f32<D, GRAD>(tile, cnt, xi[a], ti[a], fc, acc[a]);
+                        diag_pass_f32<D, GRAD>(tile, cnt, xi[a], ti[a], cSx, nomega, fc, acc[a]);
                     else
                         diag_pass<D, GRAD>(tile, cnt, xi[a], ti[a], cBx, cBt, cSx, nomega, acc[a]);
                 }
@@ -390,8 +398,8 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
                 int dob = 1;
                 if (g.skip_zero) {
                     const double gap = tl < t_lo ? t_lo - tl : 0.0;
-                    if (-cBt * gap * gap >= g.zero_arg) dob = 0;
-                    if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg) smode = kSNone;
+                    if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
+                    if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
                 }
                 const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                 for (int sb = 0; sb < 4; ++sb) {
@@ -522,7 +530,8 @@ SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lan
     const int NC = 3;
     g.col_stride = (long long)NC * g.R * ((long long)g.nb * (g.nb - 1) / 2);
     g.skip_zero = 1;
-    g.zero_arg = kZeroExpArg;
+    g.zero_arg_b = kZeroExpArg;
+    g.zero_arg_s = kZeroExpArg;
     plan.items.clear();
     for (int P = g.nb - 1; P >= 1; --P) {
         const int nch = (P * g.R + L - 1) / L;
