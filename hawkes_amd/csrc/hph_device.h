@@ -10,23 +10,19 @@ namespace kern {
 // Excitation-term mode of a tile: absent, unmasked (every column strictly earlier), masked per pair.
 enum { kSNone = 0, kSFull = 1, kSMixed = 2 };
 
-// FP32-bulk mode: exp arguments pre-scaled by log2(e) so one v_exp_f32 (2^x) forms each term.
-struct F32Consts {
-    float bx, bt; // -tauX^2/2 * log2 e, -tauT^2/2 * log2 e
-    float sx, nw; // -sigma^2/2 * log2 e, -omega * log2 e
-};
-
-__device__ __forceinline__ F32Consts make_f32_consts(double cBx, double cBt, double cSx, double nomega) {
-    const double l2e = 1.4426950408889634;
-    F32Consts c;
-    c.bx = (float)(cBx * l2e);
-    c.bt = (float)(cBt * l2e);
-    c.sx = (float)(cSx * l2e);
-    c.nw = (float)(nomega * l2e);
-    return c;
-}
-
+// FP32-bulk mode: the background exponential is evaluated by one v_exp_f32 (2^y) on the
+// FP64-formed exponent scaled by log2(e) and rounded to FP32; the result is widened back to FP64.
+constexpr double kLog2e = 1.4426950408889634;
 __device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
+// constants are then pre-scaled by log2 e).
+template <bool F32>
+__device__ __forceinline__ double background_exp(double cBx, double cBt, double r2, double dt2) {
+    const double arg = __builtin_fma(cBt, dt2, cBx * r2);
+    if (F32) return (double)exp2_f32((float)arg);
+    return exp_fp64(arg);
+}
 
 // exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
 // Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of

@@ -25,7 +25,7 @@ namespace kern {
 // excitation term is absent (every column at or after the row block, or every term exactly
 // zero), unmasked (every column strictly before it) or masked per pair (the diagonal band, and
 // ties).
-template <int D, bool GRAD, bool DOB, int SMODE>
+template <int D, bool GRAD, bool DOB, int SMODE, bool F32>
 __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D],
                                           double ti, double cBx, double cBt, double cSx, double nomega,
                                           double (&acc)[6]) {
@@ -46,7 +46,7 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
         const double dt = ti - e[D];
         if (DOB) {
             const double dt2 = dt * dt;
-            const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
+            const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
             acc[0] += eb;
             if (GRAD) {
                 acc[1] = __builtin_fma(eb, r2, acc[1]);
@@ -65,68 +65,6 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
     }
 }
 
-// FP32-bulk variant of tile_pass: distances and time differences in FP64; the background term in
-// FP32 (v_exp_f32), summed in FP32 over runs of 64 columns and added to the FP64 accumulators; the
-// excitation term in FP64 (its exponent reaches tens, where FP32 rounding of the exponent would
-// cost relative accuracy |x| * 6e-8).
-template <int D, bool GRAD, bool DOB, int SMODE>
-__device__ __forceinline__ void tile_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
-                                              double ti, double cSx, double nomega, const F32Consts &fc,
-                                              double (&acc)[6]) {
-    constexpr int ST = TileStride<D>::value;
-    for (int j0 = 0; j0 < cnt; j0 += 64) {
-        const int jn = min(cnt, j0 + 64);
-        float b32[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll 2
-        for (int j = j0; j < jn; ++j) {
-            const double *e = tile + j * ST;
-            double r2;
-            {
-                const double d0 = xi[0] - e[0];
-                r2 = d0 * d0;
-            }
-#pragma unroll
-            for (int d = 1; d < D; ++d) {
-                const double dd = xi[d] - e[d];
-                r2 = __builtin_fma(dd, dd, r2);
-            }
-            const double dt = ti - e[D];
-            if (DOB) {
-                const float r2f = (float)r2;
-                const float dtf = (float)dt;
-                const float dt2 = dtf * dtf;
-                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
-                b32[0] += eb;
-                if (GRAD) {
-                    b32[1] = __builtin_fmaf(eb, r2f, b32[1]);
-                    b32[2] = __builtin_fmaf(eb, dt2, b32[2]);
-                }
-            }
-            if (SMODE != kSNone) {
-                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
-                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
-                acc[3] += es;
-                if (GRAD) {
-                    acc[4] = __builtin_fma(es, r2, acc[4]);
-                    acc[5] = __builtin_fma(es, dt, acc[5]);
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) acc[q] += (double)b32[q];
-    }
-}
-
-template <int D, bool GRAD, bool F32, bool DOB, int SMODE>
-__device__ __forceinline__ void tile_run(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
-                                         double cBx, double cBt, double cSx, double nomega, const F32Consts &fc,
-                                         double (&acc)[6]) {
-    if (F32)
-        tile_pass_f32<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, cSx, nomega, fc, acc);
-    else
-        tile_pass<D, GRAD, DOB, SMODE>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, acc);
-}
-
 // Grid: x = row_blocks * chunks (later row blocks first: they carry more excitation work),
 // y = parameter vector k. Block: 256 lanes, one row each.
 template <int D, bool GRAD, bool F32>
@@ -143,7 +81,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
     const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
     const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
-    const F32Consts fc = make_f32_consts(cBx, cBt, cSx, nomega);
+    // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
+    const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
 
     const int tid = threadIdx.x;
     const int n = ev.n, npad = ev.npad;
@@ -185,11 +124,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
             }
             const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
             switch (mode) {
-            case 4 + kSNone: tile_run<D, GRAD, F32, true, kSNone>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
-            case 4 + kSFull: tile_run<D, GRAD, F32, true, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
-            case 4 + kSMixed: tile_run<D, GRAD, F32, true, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
-            case kSFull: tile_run<D, GRAD, F32, false, kSFull>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
-            case kSMixed: tile_run<D, GRAD, F32, false, kSMixed>(tile, cnt, xi, ti, cBx, cBt, cSx, nomega, fc, acc); break;
+            case 4 + kSNone: tile_pass<D, GRAD, true, kSNone, F32>(tile, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+            case 4 + kSFull: tile_pass<D, GRAD, true, kSFull, F32>(tile, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+            case 4 + kSMixed: tile_pass<D, GRAD, true, kSMixed, F32>(tile, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+            case kSFull: tile_pass<D, GRAD, false, kSFull, F32>(tile, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+            case kSMixed: tile_pass<D, GRAD, false, kSMixed, F32>(tile, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
             default: break; // every term of the tile is exactly zero
             }
         }

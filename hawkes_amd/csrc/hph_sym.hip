@@ -33,8 +33,9 @@ __device__ __forceinline__ double lane_shift_down(double v, int byte_addr) {
 
 // One 64-column subtile against this lane's A rows, rotating the column accumulators ca.
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
-// has rows past the last event, whose column contributions are masked by vm.
-template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL>
+// has rows past the last event, whose column contributions are masked by vm; F32: FP32-bulk
+// background exponential (cBx, cBt then carry the log2 e scale).
+template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
                                             double cSx, double nomega, double (&acc)[A][6],
@@ -63,7 +64,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             const double dt = ti[a] - q[D];
             if (DOB) {
                 const double dt2 = dt * dt;
-                const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
+                const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
                 acc[a][0] += eb;
                 if (GRAD) {
                     acc[a][1] = __builtin_fma(eb, r2, acc[a][1]);
@@ -95,7 +96,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
 // the dt > 0 mask.
-template <int D, bool GRAD>
+template <int D, bool GRAD, bool F32>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
                                           double cBx, double cBt, double cSx, double nomega, double (&acc)[6]) {
     constexpr int ST = TileStride<D>::value;
@@ -114,7 +115,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
         }
         const double dt = ti - e[D];
         const double dt2 = dt * dt;
-        const double eb = exp_fp64(__builtin_fma(cBt, dt2, cBx * r2));
+        const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
         acc[0] += eb;
         if (GRAD) {
             acc[1] = __builtin_fma(eb, r2, acc[1]);
@@ -130,171 +131,26 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     }
 }
 
-__device__ __forceinline__ float lane_shift_down_f32(float v, int byte_addr) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(byte_addr, __builtin_bit_cast(int, v)));
-}
-
-// FP32-bulk variant of sym_subtile. Distances and time differences are formed in FP64; the
-// background term (the bulk of the pairs) is evaluated in FP32 with v_exp_f32 and summed in FP32
-// over the subtile's 64 steps, then added to the FP64 row accumulators (and returned as FP64
-// column sums). The excitation term stays FP64: its exponent -omega dt - sigma^2 r^2/2 reaches
-// tens, and FP32 rounding of the exponent would give relative errors |x| * 6e-8 in the
-// per-event probabilities; after exact-zero skipping it is a small fraction of the pairs.
-template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL>
-__device__ __forceinline__ void sym_subtile_f32(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
-                                                const double (&ti)[A], const double (&vm)[A], double cSx,
-                                                double nomega, const F32Consts &fc, double (&acc)[A][6],
-                                                double (&ca64)[GRAD ? 3 : 1]) {
-    constexpr int ST = TileStride<D>::value;
-    constexpr int NC = GRAD ? 3 : 1;
-    const int src = ((lane + 1) & 63) << 2;
-    float b32[A][3];
-#pragma unroll
-    for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) b32[a][q] = 0.0f;
-    float ca[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) ca[c] = 0.0f;
-    float fvm[A];
-#pragma unroll
-    for (int a = 0; a < A; ++a) fvm[a] = (float)vm[a];
-#pragma unroll 2
-    for (int s = 0; s < 64; ++s) {
-        const double *e = sub + (lane + s) * ST;
-        double q[D + 1];
-#pragma unroll
-        for (int d = 0; d <= D; ++d) q[d] = e[d];
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-            double r2;
-            {
-                const double d0 = xi[a][0] - q[0];
-                r2 = d0 * d0;
-            }
-#pragma unroll
-            for (int d = 1; d < D; ++d) {
-                const double dd = xi[a][d] - q[d];
-                r2 = __builtin_fma(dd, dd, r2);
-            }
-            const double dt = ti[a] - q[D];
-            if (DOB) {
-                const float r2f = (float)r2;
-                const float dtf = (float)dt;
-                const float dt2 = dtf * dtf;
-                const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
-                b32[a][0] += eb;
-                if (GRAD) {
-                    b32[a][1] = __builtin_fmaf(eb, r2f, b32[a][1]);
-                    b32[a][2] = __builtin_fmaf(eb, dt2, b32[a][2]);
-                }
-                const float ec = TAIL ? eb * fvm[a] : eb;
-                ca[0] += ec;
-                if (GRAD) {
-                    ca[1] = __builtin_fmaf(ec, r2f, ca[1]);
-                    ca[2] = __builtin_fmaf(ec, dt2, ca[2]);
-                }
-            }
-            if (SMODE != kSNone) {
-                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
-                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
-                acc[a][3] += es;
-                if (GRAD) {
-                    acc[a][4] = __builtin_fma(es, r2, acc[a][4]);
-                    acc[a][5] = __builtin_fma(es, dt, acc[a][5]);
-                }
-            }
-        }
-        if (DOB) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) ca[c] = lane_shift_down_f32(ca[c], src);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) acc[a][q] += (double)b32[a][q];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) ca64[c] = (double)ca[c];
-}
-
-// FP32-bulk variant of diag_pass: background in FP32 summed over runs of 64 columns, excitation
-// in FP64.
-template <int D, bool GRAD>
-__device__ __forceinline__ void diag_pass_f32(const double *__restrict__ tile, int cnt, const double (&xi)[D],
-                                              double ti, double cSx, double nomega, const F32Consts &fc,
-                                              double (&acc)[6]) {
-    constexpr int ST = TileStride<D>::value;
-    for (int j0 = 0; j0 < cnt; j0 += 64) {
-        const int jn = min(cnt, j0 + 64);
-        float b32[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll 2
-        for (int j = j0; j < jn; ++j) {
-            const double *e = tile + j * ST;
-            double r2;
-            {
-                const double d0 = xi[0] - e[0];
-                r2 = d0 * d0;
-            }
-#pragma unroll
-            for (int d = 1; d < D; ++d) {
-                const double dd = xi[d] - e[d];
-                r2 = __builtin_fma(dd, dd, r2);
-            }
-            const double dt = ti - e[D];
-            const float r2f = (float)r2;
-            const float dtf = (float)dt;
-            const float dt2 = dtf * dtf;
-            const float eb = exp2_f32(__builtin_fmaf(fc.bt, dt2, fc.bx * r2f));
-            b32[0] += eb;
-            if (GRAD) {
-                b32[1] = __builtin_fmaf(eb, r2f, b32[1]);
-                b32[2] = __builtin_fmaf(eb, dt2, b32[2]);
-            }
-            double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
-            es = dt > 0.0 ? es : 0.0;
-            acc[3] += es;
-            if (GRAD) {
-                acc[4] = __builtin_fma(es, r2, acc[4]);
-                acc[5] = __builtin_fma(es, dt, acc[5]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) acc[q] += (double)b32[q];
-    }
-}
-
-template <int D, bool GRAD, int A, bool TAIL, bool F32, bool DOB, int SMODE>
-__device__ __forceinline__ void sym_run(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
-                                        const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
-                                        double cSx, double nomega, const F32Consts &fc, double (&acc)[A][6],
-                                        double (&ca)[GRAD ? 3 : 1]) {
-    if (F32)
-        sym_subtile_f32<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, cSx, nomega, fc, acc, ca);
-    else
-        sym_subtile<D, GRAD, A, DOB, SMODE, TAIL>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
-}
-
 template <int D, bool GRAD, int A, bool TAIL, bool F32>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
-                                             double cBx, double cBt, double cSx, double nomega, const F32Consts &fc,
-                                             double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
+                                             double cBx, double cBt, double cSx, double nomega, double (&acc)[A][6],
+                                             double (&ca)[GRAD ? 3 : 1]) {
     switch (mode) {
     case 4 + kSNone:
-        sym_run<D, GRAD, A, TAIL, F32, true, kSNone>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case 4 + kSFull:
-        sym_run<D, GRAD, A, TAIL, F32, true, kSFull>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case 4 + kSMixed:
-        sym_run<D, GRAD, A, TAIL, F32, true, kSMixed>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case kSFull:
-        sym_run<D, GRAD, A, TAIL, F32, false, kSFull>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case kSMixed:
-        sym_run<D, GRAD, A, TAIL, F32, false, kSMixed>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     default: break; // every term of the subtile is exactly zero
     }
@@ -318,7 +174,8 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
     const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
     const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
-    const F32Consts fc = make_f32_consts(cBx, cBt, cSx, nomega);
+    // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
+    const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = ev.n, npad = ev.npad;
@@ -361,12 +218,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
             __syncthreads();
             if (wave_live) {
 #pragma unroll
-                for (int a = 0; a < A; ++a) {
-                    if (F32)
-                        diag_pass_f32<D, GRAD>(tile, cnt, xi[a], ti[a], cSx, nomega, fc, acc[a]);
-                    else
-                        diag_pass<D, GRAD>(tile, cnt, xi[a], ti[a], cBx, cBt, cSx, nomega, acc[a]);
-                }
+                for (int a = 0; a < A; ++a) diag_pass<D, GRAD, F32>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
             }
         }
     } else {
@@ -408,11 +260,9 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
                     for (int c = 0; c < NC; ++c) ca[c] = 0.0;
                     const double *sub = tile + sb * 128 * ST;
                     if (tail)
-                        sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc, acc,
-                                                            ca);
+                        sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc, ca);
                     else
-                        sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, fc,
-                                                             acc, ca);
+                        sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc, ca);
 #pragma unroll
                     for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
                 }

@@ -1,9 +1,9 @@
 """FP32-bulk mode (HPH_FLAG_FLOAT) against the FP64 oracle.
 
-The pair arithmetic (Gaussians, exponentials via v_exp_f32) is FP32; differences are formed in
-FP64 first; every 64 columns the FP32 sums are added to FP64 row accumulators; log, Phi and the
-exp integral terms, the gradient algebra and all outer sums are FP64. Tolerances (stated per
-result, as measured errors are printed):
+The background exponential (the bulk of the pair work) is one v_exp_f32 on the FP64-formed
+exponent rounded to FP32; distances, time differences, the excitation term, every accumulation,
+log, Phi and the exp integral terms and the gradient algebra stay FP64. Tolerances (stated per
+result; measured errors are printed):
   * log-likelihood: relative error <= 1e-6;
   * gradient component k: |g - g_oracle| <= 1e-6 * max(|g_oracle|, 1e-3 * S_k), S_k = sum of
     |per-row terms|;
