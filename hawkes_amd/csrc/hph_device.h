@@ -15,15 +15,6 @@ enum { kSNone = 0, kSFull = 1, kSMixed = 2 };
 constexpr double kLog2e = 1.4426950408889634;
 __device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp2f(x); }
 
-// Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
-// constants are then pre-scaled by log2 e).
-template <bool F32>
-__device__ __forceinline__ double background_exp(double cBx, double cBt, double r2, double dt2) {
-    const double arg = __builtin_fma(cBt, dt2, cBx * r2);
-    if (F32) return (double)exp2_f32((float)arg);
-    return exp_fp64(arg);
-}
-
 // exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
 // Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
 // fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of
@@ -49,6 +40,15 @@ __device__ __forceinline__ double exp_fp64(double x) {
     p = __builtin_fma(p, r, HPH_EXP_C1);
     p = __builtin_fma(p, r, HPH_EXP_C0);
     return __builtin_amdgcn_ldexp(p, ki);
+}
+
+// Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
+// constants are then pre-scaled by log2 e).
+template <bool F32>
+__device__ __forceinline__ double background_exp(double cBx, double cBt, double r2, double dt2) {
+    const double arg = __builtin_fma(cBt, dt2, cBx * r2);
+    if (F32) return (double)exp2_f32((float)arg);
+    return exp_fp64(arg);
 }
 
 // Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.
