@@ -13,7 +13,8 @@
 // Column sums inside a wavefront use a rotation: over 64 steps lane l pairs its rows with column
 // (l + s) mod 64 of a 64-column subtile and adds into a 3-double accumulator that moves one lane
 // down after every step (ds_bpermute), so after 64 steps lane l holds the sums of column l.
-// The subtile is stored twice in LDS so the per-lane address (l + s) needs no wrap.
+// The subtile is stored twice in LDS, as structure of arrays, so the per-lane address (l + s)
+// needs no wrap and consecutive lanes read consecutive doubles (no bank conflicts).
 #include "hph_device.h"
 
 #include <stdexcept>
@@ -40,15 +41,14 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                                             const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
                                             double cSx, double nomega, double (&acc)[A][6],
                                             double (&ca)[GRAD ? 3 : 1]) {
-    constexpr int ST = TileStride<D>::value;
     constexpr int NC = GRAD ? 3 : 1;
     const int src = ((lane + 1) & 63) << 2;
 #pragma unroll 2
     for (int s = 0; s < 64; ++s) {
-        const double *e = sub + (lane + s) * ST;
+        // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]
         double q[D + 1];
 #pragma unroll
-        for (int d = 0; d <= D; ++d) q[d] = e[d];
+        for (int d = 0; d <= D; ++d) q[d] = sub[d * 128 + lane + s];
 #pragma unroll
         for (int a = 0; a < A; ++a) {
             double r2;
@@ -164,7 +164,8 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
     constexpr int ST = TileStride<D>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
-    __shared__ double tile[2 * kTile * ST];
+    // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x (D + 1) x 128 doubles
+    __shared__ double tile[(2 * kTile * (D + 1) > kTile * ST) ? 2 * kTile * (D + 1) : kTile * ST];
     __shared__ double colbuf[kSymWaves * NC * kTile];
 
     const int2 it = g.item[blockIdx.x];
@@ -231,21 +232,23 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
             // c1 - c0 and P0 are multiples of kTile: every tile here is full.
             __syncthreads();
             {
-                const int e0 = ((tid >> 6) * 128 + (tid & 63)) * ST;
+                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
+                // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
+                const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double v = ev.x[(size_t)d * npad + j0 + tid];
-                    tile[e0 + d] = v;
-                    tile[e0 + 64 * ST + d] = v;
+                    tile[e0 + d * 128] = v;
+                    tile[e0 + d * 128 + 64] = v;
                 }
                 const double tv = ev.t[j0 + tid];
-                tile[e0 + D] = tv;
-                tile[e0 + 64 * ST + D] = tv;
+                tile[e0 + D * 128] = tv;
+                tile[e0 + D * 128 + 64] = tv;
             }
             __syncthreads();
             if (wave_live) {
                 // columns are earlier than the rows: tl <= t_lo; equal times need the mask
-                const double tl = tile[(3 * 128 + 63) * ST + D];
+                const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
                 int smode = tl < t_lo ? kSFull : kSMixed;
                 int dob = 1;
                 if (g.skip_zero) {
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const do
                     double ca[NC];
 #pragma unroll
                     for (int c = 0; c < NC; ++c) ca[c] = 0.0;
-                    const double *sub = tile + sb * 128 * ST;
+                    const double *sub = tile + sb * (D + 1) * 128;
                     if (tail)
                         sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc, ca);
                     else
