@@ -54,6 +54,15 @@ SIGNATURES = {
     "hph_set_pair_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hph_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "hph_get_tuned_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _sz]),
+    "hph_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_long,
+                                        ctypes.c_int, ctypes.POINTER(_vp)]),
+    "hph_batch_destroy": (ctypes.c_int, [_vp]),
+    "hph_batch_set_data": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _sz, _dp, _sz]),
+    "hph_batch_set_parameters": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _dp, _sz]),
+    "hph_batch_get_log_likelihood": (ctypes.c_int, [_vp, _dp, _dp, _sz]),
+    "hph_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "hph_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "hph_batch_get_timing": (ctypes.c_int, [_vp, _dp]),
 }
 
 _lib = None

@@ -87,6 +87,122 @@ struct HostBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Drives the symmetric pass over a set of device-resident datasets: plans work items, keeps the
+// plan and its device tables while the shape is unchanged, owns the partial buffers.
+class SymRunner {
+  public:
+    struct DataRef {
+        const double *x, *t;
+        const int *perm;
+        int n, npad;
+        double t_last;
+    };
+
+    ~SymRunner() {
+        for (DevBuf *b : {&d_data_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_}) b->release();
+    }
+
+    // Plans (if the shape changed) and enqueues the pass, finalize and reduction on stream s.
+    // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev: dataset 0, vector 0.
+    // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
+    void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
+                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, double *probs_dev,
+                 double *result, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                 hipEvent_t ev2 = nullptr) {
+        std::vector<int> n(refs.size());
+        for (size_t d = 0; d < refs.size(); ++d) n[d] = refs[d].n;
+        const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
+        if (!same) {
+            plan_ = kern::plan_sym(n, K, rows_per_lane, target_items, grad);
+            n_ = n;
+            K_ = K;
+            A_ = rows_per_lane;
+            target_ = target_items;
+            grad_ = grad;
+            d_items_.ensure(sizeof(int4) * plan_.items.size());
+            check(hipMemcpy(d_items_.ptr, plan_.items.data(), sizeof(int4) * plan_.items.size(), hipMemcpyHostToDevice),
+                  "upload work items");
+            d_fin_.ensure(sizeof(int4) * plan_.fin_items.size());
+            check(hipMemcpy(d_fin_.ptr, plan_.fin_items.data(), sizeof(int4) * plan_.fin_items.size(),
+                            hipMemcpyHostToDevice),
+                  "upload finalize items");
+            d_res_.ensure(sizeof(int2) * plan_.res_items.size());
+            check(hipMemcpy(d_res_.ptr, plan_.res_items.data(), sizeof(int2) * plan_.res_items.size(),
+                            hipMemcpyHostToDevice),
+                  "upload result items");
+            if (!d_counter_.ptr) {
+                d_counter_.ensure(sizeof(int));
+                check(hipMemset(d_counter_.ptr, 0, sizeof(int)), "zero queue counter");
+            }
+            d_row_.ensure(sizeof(double) * std::max<size_t>(plan_.rowpart_doubles, 1));
+            d_col_.ensure(sizeof(double) * std::max<size_t>(plan_.colpart_doubles, 1));
+            d_block_.ensure(sizeof(double) * std::max<size_t>(plan_.block_partials, 1) * kern::kResultWidth * 2);
+            refs_.clear();
+            valid_ = true;
+        }
+        bool refs_same = refs_.size() == refs.size();
+        for (size_t d = 0; refs_same && d < refs.size(); ++d)
+            refs_same = refs_[d].x == refs[d].x && refs_[d].t == refs[d].t && refs_[d].perm == refs[d].perm &&
+                        refs_[d].npad == refs[d].npad && refs_[d].t_last == refs[d].t_last;
+        if (!refs_same) {
+            for (size_t d = 0; d < refs.size(); ++d) {
+                kern::SymData &sd = plan_.data[d];
+                sd.x = refs[d].x;
+                sd.t = refs[d].t;
+                sd.perm = refs[d].perm;
+                sd.npad = refs[d].npad;
+                sd.t_last = refs[d].t_last;
+            }
+            d_data_.ensure(sizeof(kern::SymData) * plan_.data.size());
+            check(hipMemcpy(d_data_.ptr, plan_.data.data(), sizeof(kern::SymData) * plan_.data.size(),
+                            hipMemcpyHostToDevice),
+                  "upload dataset table");
+            refs_ = refs;
+        }
+        if (cus_ == 0) {
+            int dev = 0;
+            check(hipGetDevice(&dev), "hipGetDevice");
+            check(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev), "device CU count");
+        }
+        kern::SymLaunch L{};
+        L.data = d_data_.as<kern::SymData>();
+        L.item = d_items_.as<int4>();
+        L.items = (int)plan_.items.size();
+        L.counter = d_counter_.as<int>();
+        L.grid = std::min(L.items, std::max(cus_, 1) * 4);
+        L.dim = dim;
+        L.A = plan_.A;
+        L.R = plan_.R;
+        L.skip_zero = skip_zero;
+        L.zero_arg_b = fp32 ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+        L.zero_arg_s = kern::kZeroExpArg;
+        L.fin_item = d_fin_.as<int4>();
+        L.fin_items = (int)plan_.fin_items.size();
+        L.res_item = d_res_.as<int2>();
+        L.res_items = (int)plan_.res_items.size();
+        if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
+        kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
+        check(hipGetLastError(), "symmetric pair kernel launch");
+        if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), d_block_.as<double>(),
+                                  probs_dev, s);
+        check(hipGetLastError(), "finalize kernel launch");
+        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
+        check(hipGetLastError(), "reduce kernel launch");
+        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+    }
+
+    const kern::SymPlan &plan() const { return plan_; }
+
+  private:
+    kern::SymPlan plan_;
+    bool valid_ = false, grad_ = false;
+    std::vector<int> n_, K_;
+    int A_ = 0, target_ = 0, cus_ = 0;
+    std::vector<DataRef> refs_;
+    DevBuf d_data_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_;
+};
+
 // Pair-pass configuration chosen by the autotuner.
 struct TuneConfig {
     int kind = HPH_PAIR_KERNEL_ROWWISE; // HPH_PAIR_KERNEL_ROWWISE or HPH_PAIR_KERNEL_SYMMETRIC
@@ -181,7 +297,7 @@ class HipHawkes : public AbstractHawkes {
     ~HipHawkes() override {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
-        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_colpart_, &d_items_})
+        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -508,32 +624,10 @@ class HipHawkes : public AbstractHawkes {
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         double *probs_dev = probs_host ? d_probs_.as<double>() : nullptr;
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
-            if (!sym_valid_ || sym_K_ != K || sym_A_ != cfg.rows_per_lane || sym_target_ != cfg.target_blocks) {
-                sym_ = kern::plan_sym_grid(locationCount, embeddingDimension, K, cfg.target_blocks, cfg.rows_per_lane);
-                d_items_.ensure(sizeof(int2) * sym_.items.size());
-                check(hipMemcpy(d_items_.ptr, sym_.items.data(), sizeof(int2) * sym_.items.size(),
-                                hipMemcpyHostToDevice),
-                      "upload work items");
-                sym_valid_ = true;
-                sym_K_ = K;
-                sym_A_ = cfg.rows_per_lane;
-                sym_target_ = cfg.target_blocks;
-            }
-            kern::SymGrid g = sym_.grid;
-            g.item = d_items_.as<int2>();
-            g.skip_zero = skip_zero;
-            g.zero_arg_b = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
-            g.zero_arg_s = kern::kZeroExpArg;
-            d_part_.ensure(sizeof(double) * kern::sym_rowpart_doubles(g, K, grad));
-            d_colpart_.ensure(sizeof(double) * kern::sym_colpart_doubles(g, K, grad));
-            if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-            kern::launch_sym(ev, d_params_.as<double>(), K, g, grad, fp32_, d_part_.as<double>(),
-                             d_colpart_.as<double>(), stream_);
-            check(hipGetLastError(), "symmetric pair kernel launch");
-            if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
-            kern::launch_sym_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
-                                      d_colpart_.as<double>(), d_block_.as<double>(), probs_dev, stream_);
-            check(hipGetLastError(), "finalize kernel launch");
+            SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+            sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
+                         cfg.target_blocks, skip_zero, probs_dev, d_result_.as<double>(), stream_,
+                         timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
@@ -548,10 +642,10 @@ class HipHawkes : public AbstractHawkes {
             kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
                                   d_block_.as<double>(), probs_dev, stream_);
             check(hipGetLastError(), "finalize kernel launch");
+            kern::launch_reduce(ev, K, fin_blocks, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
+            check(hipGetLastError(), "reduce kernel launch");
+            if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         }
-        kern::launch_reduce(ev, K, fin_blocks, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
-        check(hipGetLastError(), "reduce kernel launch");
-        if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * (size_t)K * kern::kResultWidth,
                              hipMemcpyDeviceToHost, stream_),
               "download result");
@@ -583,9 +677,7 @@ class HipHawkes : public AbstractHawkes {
 
     static constexpr int kSymMinEvents = 4096;
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
-    kern::SymPlan sym_;
-    bool sym_valid_ = false;
-    int sym_K_ = 0, sym_A_ = 0, sym_target_ = 0;
+    SymRunner sym_;
 
     int device_;
     bool fp32_;
@@ -612,8 +704,216 @@ class HipHawkes : public AbstractHawkes {
     hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
     double last_ms_[2] = {0.0, 0.0};
 
-    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_, d_colpart_, d_items_;
+    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_;
     HostBuf h_stage_, h_result_, h_params_;
+};
+
+
+// Many datasets of uneven size, each with its own parameter vectors, evaluated by one symmetric
+// pass (one persistent launch over a cost-ordered work queue). With graph mode on, the launch
+// sequence (parameter upload, queue reset, pass, finalize, reduction, result download) is
+// captured once per shape and replayed.
+class HipHawkesBatch {
+  public:
+    HipHawkesBatch(int D, const std::vector<int> &counts, long flags, int device)
+        : D_(D), counts_(counts), flags_(flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
+        if (D < 1 || D > 3) throw std::invalid_argument("batched datasets support dimensions 1..3");
+        if (counts.empty()) throw std::invalid_argument("dataset_count must be >= 1");
+        for (int c : counts)
+            if (c < 1) throw std::invalid_argument("every dataset needs at least one event");
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device_ >= count) throw std::invalid_argument("device index out of range");
+        check(hipSetDevice(device_), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        const int nd = (int)counts.size();
+        off_.resize(nd + 1, 0);
+        npad_.resize(nd);
+        for (int d = 0; d < nd; ++d) {
+            npad_[d] = ((counts[d] + kern::kRowsPerBlock - 1) / kern::kRowsPerBlock) * kern::kRowsPerBlock;
+            off_[d + 1] = off_[d] + npad_[d];
+        }
+        locs_.resize(nd);
+        times_.resize(nd);
+        have_.assign(nd, false);
+        d_x_.ensure(sizeof(double) * (size_t)D * off_[nd]);
+        d_t_.ensure(sizeof(double) * (size_t)off_[nd]);
+        d_perm_.ensure(sizeof(int) * (size_t)off_[nd]);
+        for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+    }
+
+    ~HipHawkesBatch() {
+        (void)hipSetDevice(device_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        drop_graph();
+        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_params_, &d_result_}) b->release();
+        h_params_.release();
+        h_result_.release();
+        for (auto &e : ev_)
+            if (e) (void)hipEventDestroy(e);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    void setData(int d, const double *locs, size_t locs_len, const double *times, size_t times_len) {
+        if (d < 0 || d >= (int)counts_.size()) throw std::invalid_argument("dataset index out of range");
+        const int n = counts_[d];
+        if (locs == nullptr || times == nullptr) throw std::invalid_argument("data pointer is null");
+        if (locs_len != (size_t)n * D_ || times_len != (size_t)n)
+            throw std::invalid_argument("dataset lengths must be N*D locations and N times");
+        for (size_t q = 0; q < locs_len; ++q)
+            if (!std::isfinite(locs[q])) throw std::invalid_argument("locations must be finite");
+        for (size_t q = 0; q < times_len; ++q)
+            if (!std::isfinite(times[q])) throw std::invalid_argument("times must be finite");
+        check(hipSetDevice(device_), "hipSetDevice");
+        locs_[d].assign(locs, locs + locs_len);
+        times_[d].assign(times, times + times_len);
+        std::vector<int> perm(n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return times[a] < times[b]; });
+        const int np = npad_[d];
+        std::vector<double> hx((size_t)D_ * np), ht(np);
+        std::vector<int> hp(np);
+        for (int s = 0; s < np; ++s) {
+            const int u = perm[s < n ? s : n - 1];
+            for (int q = 0; q < D_; ++q) hx[(size_t)q * np + s] = locs[(size_t)u * D_ + q];
+            ht[s] = times[u];
+            hp[s] = u;
+        }
+        // x planes of dataset d live at d_x_ + D * off_[d]: plane q at + q * npad_[d]
+        check(hipMemcpy(d_x_.as<double>() + (size_t)D_ * off_[d], hx.data(), sizeof(double) * hx.size(),
+                        hipMemcpyHostToDevice),
+              "upload x");
+        check(hipMemcpy(d_t_.as<double>() + off_[d], ht.data(), sizeof(double) * np, hipMemcpyHostToDevice),
+              "upload t");
+        check(hipMemcpy(d_perm_.as<int>() + off_[d], hp.data(), sizeof(int) * np, hipMemcpyHostToDevice),
+              "upload perm");
+        have_[d] = true;
+        drop_graph();
+    }
+
+    void setParameters(const int *vectors, const double *params, size_t count) {
+        if (vectors == nullptr || params == nullptr) throw std::invalid_argument("parameter pointer is null");
+        const int nd = (int)counts_.size();
+        std::vector<int> K(vectors, vectors + nd);
+        size_t total = 0;
+        for (int k : K) {
+            if (k < 1) throw std::invalid_argument("every dataset needs at least one parameter vector");
+            total += (size_t)k;
+        }
+        if (count != total * 6) throw std::invalid_argument("count must be 6 x the total number of vectors");
+        for (size_t q = 0; q < count; ++q)
+            if (!std::isfinite(params[q])) throw std::invalid_argument("parameters must be finite");
+        if (K != K_) drop_graph();
+        K_ = K;
+        total_ = (int)total;
+        h_params_.ensure(sizeof(double) * count);
+        std::memcpy(h_params_.ptr, params, sizeof(double) * count);
+        have_params_ = true;
+    }
+
+    void setGraph(bool enabled) {
+        if (!enabled) drop_graph();
+        graph_mode_ = enabled;
+    }
+
+    void setConfig(int rows_per_lane, int target_items) {
+        if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
+        if (target_items < 1 || target_items > (1 << 24)) throw std::invalid_argument("target items out of range");
+        A_ = rows_per_lane;
+        target_ = target_items;
+        drop_graph();
+    }
+
+    void evaluate(double *loglik, double *grad, size_t total) {
+        if (loglik == nullptr) throw std::invalid_argument("loglik pointer is null");
+        for (size_t d = 0; d < have_.size(); ++d)
+            if (!have_[d]) throw NotInitialized("dataset " + std::to_string(d) + " has no data (set_data)");
+        if (!have_params_) throw NotInitialized("parameters are not set");
+        if (total != (size_t)total_) throw std::invalid_argument("total does not match the parameter vectors");
+        check(hipSetDevice(device_), "hipSetDevice");
+        const bool want_grad = grad != nullptr;
+        if (want_grad != grad_) drop_graph();
+        grad_ = want_grad;
+        d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
+        d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
+        h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
+        if (graph_mode_) {
+            if (!graph_exec_) {
+                enqueue(false); // plans, allocates and uploads the tables outside the capture
+                check(hipStreamSynchronize(stream_), "stream sync");
+                check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                enqueue(false);
+                hipGraph_t g = nullptr;
+                check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+                graph_ = g;
+                check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+            }
+            check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+            check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+            check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
+        } else {
+            enqueue(true);
+        }
+        check(hipStreamSynchronize(stream_), "stream sync");
+        float a = 0.f;
+        check(hipEventElapsedTime(&a, ev_[0], ev_[2]), "hipEventElapsedTime");
+        last_ms_ = a;
+        const double *r = h_result_.as<double>();
+        for (int k = 0; k < total_; ++k) {
+            loglik[k] = r[(size_t)k * kern::kResultWidth];
+            if (want_grad)
+                for (int q = 0; q < 6; ++q) grad[(size_t)k * 6 + q] = r[(size_t)k * kern::kResultWidth + 1 + q];
+        }
+    }
+
+    double lastMs() const { return last_ms_; }
+
+  private:
+    // Parameter upload, pass, finalize, reduction and result download on the batch's stream.
+    void enqueue(bool events) {
+        check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)total_, hipMemcpyHostToDevice,
+                             stream_),
+              "upload params");
+        std::vector<SymRunner::DataRef> refs(counts_.size());
+        for (size_t d = 0; d < counts_.size(); ++d)
+            refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
+                                         d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
+        const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
+        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, nullptr,
+                        d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
+                        events ? ev_[2] : nullptr);
+        check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,
+                             hipMemcpyDeviceToHost, stream_),
+              "download result");
+    }
+
+    void drop_graph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+    int D_;
+    std::vector<int> counts_;
+    long flags_;
+    int device_;
+    bool fp32_;
+    hipStream_t stream_ = nullptr;
+    std::vector<int> off_, npad_;
+    std::vector<std::vector<double>> locs_, times_;
+    std::vector<bool> have_;
+    std::vector<int> K_;
+    int total_ = 0;
+    bool have_params_ = false, grad_ = true, graph_mode_ = false;
+    int A_ = 2, target_ = 8192;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+    double last_ms_ = 0.0;
+    SymRunner runner_;
+    DevBuf d_x_, d_t_, d_perm_, d_params_, d_result_;
+    HostBuf h_params_, h_result_;
 };
 
 SharedPtr factory(int dim1, int dim2, long flags, int device, int /*threads*/) {
@@ -794,6 +1094,77 @@ int hph_set_pair_kernel(hph_engine *e, int kind) {
 int hph_get_timing(hph_engine *e, double *out, size_t length) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->getTiming(out, length); });
+}
+
+struct hph_batch {
+    hph::HipHawkesBatch *impl;
+};
+
+int hph_batch_create(int embedding_dimension, int dataset_count, const int *location_counts, long flags, int device,
+                     hph_batch **out) {
+    if (out == nullptr) {
+        g_last_error = "output handle pointer is null";
+        return HPH_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (dataset_count < 1 || location_counts == nullptr) {
+        g_last_error = "dataset_count must be >= 1 and location_counts non-null";
+        return HPH_ERR_INVALID_ARGUMENT;
+    }
+    return guarded([&] {
+        std::vector<int> counts(location_counts, location_counts + dataset_count);
+        auto *impl = new hph::HipHawkesBatch(embedding_dimension, counts, flags, device);
+        *out = new hph_batch{impl};
+    });
+}
+
+int hph_batch_destroy(hph_batch *b) {
+    if (b == nullptr) return HPH_OK;
+    return guarded([&] {
+        delete b->impl;
+        delete b;
+    });
+}
+
+#define HPH_REQUIRE_BATCH(b)                                                                                   \
+    if ((b) == nullptr || (b)->impl == nullptr) {                                                              \
+        g_last_error = "batch handle is null";                                                                 \
+        return HPH_ERR_INVALID_ARGUMENT;                                                                       \
+    }
+
+int hph_batch_set_data(hph_batch *b, int dataset, const double *locations, size_t locations_length,
+                       const double *times, size_t times_length) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->setData(dataset, locations, locations_length, times, times_length); });
+}
+
+int hph_batch_set_parameters(hph_batch *b, const int *vectors_per_dataset, const double *params, size_t count) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->setParameters(vectors_per_dataset, params, count); });
+}
+
+int hph_batch_get_log_likelihood(hph_batch *b, double *loglik, double *grad, size_t total_vectors) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->evaluate(loglik, grad, total_vectors); });
+}
+
+int hph_batch_set_graph(hph_batch *b, int enabled) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->setGraph(enabled != 0); });
+}
+
+int hph_batch_set_config(hph_batch *b, int rows_per_lane, int target_items) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->setConfig(rows_per_lane, target_items); });
+}
+
+int hph_batch_get_timing(hph_batch *b, double *out) {
+    HPH_REQUIRE_BATCH(b);
+    if (out == nullptr) {
+        g_last_error = "output pointer is null";
+        return HPH_ERR_INVALID_ARGUMENT;
+    }
+    return guarded([&] { *out = b->impl->lastMs(); });
 }
 
 } // extern "C"

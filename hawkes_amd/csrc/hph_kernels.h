@@ -62,33 +62,66 @@ void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double
                    hipStream_t s);
 
 // ---- Symmetric pass (hph_sym.hip): background exps once per unordered pair ----
-struct SymGrid {
-    int A;               // event rows per lane
-    int R;               // rows per block (4 wavefronts x 64 lanes x A)
-    int nb;              // row blocks = ceil(n / R)
-    int L;               // earlier columns per work item (multiple of kTile)
-    int max_chunks;      // chunk slots of the row partials; the diagonal block uses slot max_chunks
-    int items;           // work items (block, chunk) and (block, -1) for the diagonal block
-    const int2 *item;    // device array of work items
-    long long col_stride;// column-partial doubles per parameter vector
-    int skip_zero;       // as PairGrid::skip_zero
-    double zero_arg_b;   // as PairGrid::zero_arg_b
-    double zero_arg_s;   // as PairGrid::zero_arg_s
+//
+// The pass runs over a table of datasets (one for an engine, many for a batch); every dataset
+// has its own events, parameter vectors and slices of the partial buffers. Work items
+// (dataset, row block, chunk of earlier columns | -1 for the diagonal block, parameter vector)
+// are sorted heaviest first and taken from a queue by a persistent grid.
+
+// One dataset of a symmetric pass (device table entry).
+struct SymData {
+    const double *x;        // D planes of npad doubles (time-sorted)
+    const double *t;        // npad doubles (sorted)
+    const int *perm;        // sorted position -> caller's index
+    int n, npad;            // events; padded length of x and t
+    int nb;                 // row blocks of R rows
+    int L;                  // earlier columns per chunk item (multiple of kTile)
+    int max_chunks;         // row-partial slots; the diagonal block uses slot max_chunks
+    int K;                  // parameter vectors of this dataset
+    int param0;             // index of its first parameter vector
+    int fin_blocks;         // ceil(n / 256)
+    double t_last;          // the caller's last time (T of the model)
+    long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
+    long long col0, col_kstride; // column partials: the same
+    long long blk0;         // index of its (k = 0, block 0) block partial
+    int res0;               // index of its (k = 0) result
 };
 
+struct SymLaunch {
+    const SymData *data;    // device table
+    const int4 *item;       // (dataset, block, chunk | -1, k), heaviest first
+    int items;
+    int *counter;           // work-queue counter, zeroed by launch_sym before the pass
+    int grid;               // persistent workgroups launched
+    int dim, A, R;
+    int skip_zero;          // as PairGrid::skip_zero
+    double zero_arg_b;      // as PairGrid::zero_arg_b
+    double zero_arg_s;      // as PairGrid::zero_arg_s
+    const int4 *fin_item;   // (dataset, k, finalize block, 0)
+    int fin_items;
+    const int2 *res_item;   // (dataset, k)
+    int res_items;
+};
+
+// Host-side plan of a symmetric pass over datasets of sizes n[d] with K[d] vectors each.
 struct SymPlan {
-    SymGrid grid;
-    std::vector<int2> items; // host copy of the work items, heaviest first
+    int A = 2, R = 512;
+    std::vector<SymData> data;   // pointers filled in by the caller
+    std::vector<int4> items, fin_items;
+    std::vector<int2> res_items;
+    size_t rowpart_doubles = 0, colpart_doubles = 0, block_partials = 0;
+    int results = 0;
 };
 
-SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lane);
-size_t sym_rowpart_doubles(const SymGrid &g, int K, bool grad);
-size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad);
-void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, bool fp32,
-                double *rowpart, double *colpart, hipStream_t s);
-void launch_sym_finalize(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
-                         const double *rowpart, const double *colpart, double *block_out, double *probs,
-                         hipStream_t s);
+SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_per_lane, int target_items,
+                 bool grad);
+void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart, double *colpart,
+                hipStream_t s);
+// Finalize (per row, then per 256-row block in double-double) and reduction (per dataset and
+// vector: result[(res0 + k) * 7 + q]); probs (may be null) receives dataset 0, vector 0.
+void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
+                         const double *colpart, double *block_out, double *probs, hipStream_t s);
+void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
 
 } // namespace kern
 } // namespace hph

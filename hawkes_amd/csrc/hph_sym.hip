@@ -17,6 +17,7 @@
 // needs no wrap and consecutive lanes read consecutive doubles (no bank conflicts).
 #include "hph_device.h"
 
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -156,188 +157,207 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
     }
 }
 
-// Grid: x = work item (row block, chunk of earlier columns | diagonal), y = parameter vector k.
+// Persistent pass: every workgroup takes work items from the queue until it is empty.
 template <int D, bool GRAD, int A, bool F32>
-__global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(Events ev, const double *__restrict__ params,
-                                                             SymGrid g, double *__restrict__ rowpart,
+__global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const double *__restrict__ params,
+                                                             double *__restrict__ rowpart,
                                                              double *__restrict__ colpart) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
+    constexpr int R = kSymWaves * 64 * A;
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x (D + 1) x 128 doubles
     __shared__ double tile[(2 * kTile * (D + 1) > kTile * ST) ? 2 * kTile * (D + 1) : kTile * ST];
     __shared__ double colbuf[kSymWaves * NC * kTile];
-
-    const int2 it = g.item[blockIdx.x];
-    const int P = it.x, chunk = it.y;
-    const int k = blockIdx.y;
-    const double *pk = params + 6 * k;
-    const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
-    const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
-    const double cSx = -0.5 * sigma * sigma, nomega = -omega;
-    // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
-    const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
+    __shared__ int s_item;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int n = ev.n, npad = ev.npad;
-    const int R = g.R;
-    const int P0 = P * R;
-    const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
-    double xi[A][D], ti[A], vm[A];
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-        const int r = wrow0 + a * 64 + lane;
-        const int rc = min(r, n - 1);
-#pragma unroll
-        for (int d = 0; d < D; ++d) xi[a][d] = ev.x[(size_t)d * npad + rc];
-        ti[a] = ev.t[rc];
-        vm[a] = r < n ? 1.0 : 0.0;
-    }
-    const bool wave_live = wrow0 < n;
-    const double t_lo = ev.t[min(wrow0, n - 1)];
-    const double t_hi = ev.t[min(wrow0 + 64 * A - 1, n - 1)];
-    (void)t_hi;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(L.counter, 1);
+        __syncthreads();
+        const int idx = s_item;
+        __syncthreads();
+        if (idx >= L.items) break;
 
-    double acc[A][6];
-#pragma unroll
-    for (int a = 0; a < A; ++a)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) acc[a][q] = 0.0;
+        const int4 it = L.item[idx];
+        const SymData &ds = L.data[it.x];
+        const int P = it.y, chunk = it.z, k = it.w;
+        const double *pk = params + 6 * (size_t)(ds.param0 + k);
+        const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
+        const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
+        const double cSx = -0.5 * sigma * sigma, nomega = -omega;
+        // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
+        const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
+        const double *__restrict__ ex = ds.x;
+        const double *__restrict__ et = ds.t;
+        const int n = ds.n, npad = ds.npad;
 
-    int slot;
-    if (chunk < 0) {
-        slot = g.max_chunks;
-        const int c1 = min(P0 + R, n);
-        for (int j0 = P0; j0 < c1; j0 += kTile) {
-            const int cnt = min(kTile, c1 - j0);
-            __syncthreads();
-            if (tid < cnt) {
+        const int P0 = P * R;
+        const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
+        double xi[A][D], ti[A], vm[A];
 #pragma unroll
-                for (int d = 0; d < D; ++d) tile[tid * ST + d] = ev.x[(size_t)d * npad + j0 + tid];
-                tile[tid * ST + D] = ev.t[j0 + tid];
-            }
-            __syncthreads();
-            if (wave_live) {
+        for (int a = 0; a < A; ++a) {
+            const int r = wrow0 + a * 64 + lane;
+            const int rc = min(r, n - 1);
 #pragma unroll
-                for (int a = 0; a < A; ++a) diag_pass<D, GRAD, F32>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
-            }
+            for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
+            ti[a] = et[rc];
+            vm[a] = r < n ? 1.0 : 0.0;
         }
-    } else {
-        slot = chunk;
-        const int c0 = chunk * g.L;
-        const int c1 = min(c0 + g.L, P0);
-        const bool tail = P0 + R > n;
-        double *cp = colpart + (size_t)k * g.col_stride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
-        for (int j0 = c0; j0 < c1; j0 += kTile) {
-            // c1 - c0 and P0 are multiples of kTile: every tile here is full.
-            __syncthreads();
-            {
-                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
-                // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
-                const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
+        const bool wave_live = wrow0 < n;
+        const double t_lo = et[min(wrow0, n - 1)];
+
+        double acc[A][6];
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const double v = ev.x[(size_t)d * npad + j0 + tid];
-                    tile[e0 + d * 128] = v;
-                    tile[e0 + d * 128 + 64] = v;
+        for (int a = 0; a < A; ++a)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc[a][q] = 0.0;
+
+        int slot;
+        if (chunk < 0) {
+            slot = ds.max_chunks;
+            const int c1 = min(P0 + R, n);
+            for (int j0 = P0; j0 < c1; j0 += kTile) {
+                const int cnt = min(kTile, c1 - j0);
+                __syncthreads();
+                if (tid < cnt) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) tile[tid * ST + d] = ex[(size_t)d * npad + j0 + tid];
+                    tile[tid * ST + D] = et[j0 + tid];
                 }
-                const double tv = ev.t[j0 + tid];
-                tile[e0 + D * 128] = tv;
-                tile[e0 + D * 128 + 64] = tv;
-            }
-            __syncthreads();
-            if (wave_live) {
-                // columns are earlier than the rows: tl <= t_lo; equal times need the mask
-                const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
-                int smode = tl < t_lo ? kSFull : kSMixed;
-                int dob = 1;
-                if (g.skip_zero) {
-                    const double gap = tl < t_lo ? t_lo - tl : 0.0;
-                    if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
-                    if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
+                __syncthreads();
+                if (wave_live) {
+#pragma unroll
+                    for (int a = 0; a < A; ++a)
+                        diag_pass<D, GRAD, F32>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
                 }
-                const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
-                for (int sb = 0; sb < 4; ++sb) {
-                    double ca[NC];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) ca[c] = 0.0;
-                    const double *sub = tile + sb * (D + 1) * 128;
-                    if (tail)
-                        sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc, ca);
-                    else
-                        sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc, ca);
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
-                }
-            } else {
-#pragma unroll
-                for (int sb = 0; sb < 4; ++sb)
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = 0.0;
             }
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                double v = colbuf[(0 * NC + c) * kTile + tid];
-#pragma unroll
-                for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
-                cp[(size_t)c * P0 + j0 + tid] = v;
-            }
-        }
-    }
-    const size_t rpad = (size_t)g.nb * R;
-    double *rp = rowpart + ((size_t)k * (g.max_chunks + 1) + slot) * NR * rpad;
-#pragma unroll
-    for (int a = 0; a < A; ++a) {
-        const int r = wrow0 + a * 64 + lane;
-        if (GRAD) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
         } else {
-            rp[r] = acc[a][0];
-            rp[rpad + r] = acc[a][3];
+            slot = chunk;
+            const int c0 = chunk * ds.L;
+            const int c1 = min(c0 + ds.L, P0);
+            const bool tail = P0 + R > n;
+            double *cp = colpart + ds.col0 + (size_t)k * ds.col_kstride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
+            for (int j0 = c0; j0 < c1; j0 += kTile) {
+                // c1 - c0 and P0 are multiples of kTile: every tile here is full.
+                __syncthreads();
+                {
+                    // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
+                    // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
+                    const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const double v = ex[(size_t)d * npad + j0 + tid];
+                        tile[e0 + d * 128] = v;
+                        tile[e0 + d * 128 + 64] = v;
+                    }
+                    const double tv = et[j0 + tid];
+                    tile[e0 + D * 128] = tv;
+                    tile[e0 + D * 128 + 64] = tv;
+                }
+                __syncthreads();
+                if (wave_live) {
+                    // columns are earlier than the rows: tl <= t_lo; equal times need the mask
+                    const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
+                    int smode = tl < t_lo ? kSFull : kSMixed;
+                    int dob = 1;
+                    if (L.skip_zero) {
+                        const double gap = tl < t_lo ? t_lo - tl : 0.0;
+                        if (-cBt * gap * gap >= L.zero_arg_b) dob = 0;
+                        if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= L.zero_arg_s) smode = kSNone;
+                    }
+                    const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
+                    for (int sb = 0; sb < 4; ++sb) {
+                        double ca[NC];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) ca[c] = 0.0;
+                        const double *sub = tile + sb * (D + 1) * 128;
+                        if (tail)
+                            sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc,
+                                                                ca);
+                        else
+                            sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega,
+                                                                 acc, ca);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
+                    }
+                } else {
+#pragma unroll
+                    for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = 0.0;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    double v = colbuf[(0 * NC + c) * kTile + tid];
+#pragma unroll
+                    for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
+                    cp[(size_t)c * P0 + j0 + tid] = v;
+                }
+            }
+        }
+        const size_t rpad = (size_t)ds.nb * R;
+        double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const int r = wrow0 + a * 64 + lane;
+            if (GRAD) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
+            } else {
+                rp[r] = acc[a][0];
+                rp[rpad + r] = acc[a][3];
+            }
         }
     }
 }
 
-// Row sums = row-side chunks + diagonal block + column side of every later block.
+// Zeroes the work-queue counter before a pass (a kernel, so the pass can be captured in a graph).
+__global__ void queue_reset_kernel(int *counter) { *counter = 0; }
+
+// Row sums = row-side chunks + diagonal block + column side of every later block. One block of
+// 256 rows of one (dataset, vector) per workgroup.
 template <bool GRAD>
-__global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(Events ev, const double *__restrict__ params,
-                                                                     SymGrid g, const double *__restrict__ rowpart,
+__global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L, const double *__restrict__ params,
+                                                                     const double *__restrict__ rowpart,
                                                                      const double *__restrict__ colpart,
                                                                      double *__restrict__ block_out,
                                                                      double *__restrict__ probs) {
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     __shared__ double2 sh[kRowsPerBlock];
-    const int k = blockIdx.y;
-    const int fb = blockIdx.x;
+    const int4 it = L.fin_item[blockIdx.x];
+    const SymData &ds = L.data[it.x];
+    const int k = it.y, fb = it.z;
     const int i = fb * kRowsPerBlock + threadIdx.x;
-    const int n = ev.n;
-    const int R = g.R;
+    const int n = ds.n;
+    const int R = L.R;
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < n) {
         const int P = i / R;
-        const int nch = (P * R + g.L - 1) / g.L;
-        const size_t rpad = (size_t)g.nb * R;
+        const int nch = (P * R + ds.L - 1) / ds.L;
+        const size_t rpad = (size_t)ds.nb * R;
+        const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
         double a6[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) a6[q] = 0.0;
         for (int c = 0; c < nch; ++c) {
-            const double *rp = rowpart + ((size_t)k * (g.max_chunks + 1) + c) * NR * rpad;
+            const double *rp = rbase + (size_t)c * NR * rpad;
 #pragma unroll
             for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
         }
         {
-            const double *rp = rowpart + ((size_t)k * (g.max_chunks + 1) + g.max_chunks) * NR * rpad;
+            const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
 #pragma unroll
             for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
         }
         double cs[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) cs[c] = 0.0;
-        for (int Q = P + 1; Q < g.nb; ++Q) {
-            const double *cp = colpart + (size_t)k * g.col_stride + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
+        const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+        for (int Q = P + 1; Q < ds.nb; ++Q) {
+            const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
             const size_t Q0 = (size_t)Q * R;
 #pragma unroll
             for (int c = 0; c < NC; ++c) cs[c] += cp[(size_t)c * Q0 + i];
@@ -354,10 +374,45 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(Events ev, 
             s[0] = a6[0] + cs[0];
             s[3] = a6[NR - 1];
         }
-        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v);
-        if (probs != nullptr && k == 0) probs[ev.perm[i]] = p;
+        const double p = row_terms<GRAD>(s, params + 6 * (size_t)(ds.param0 + k), L.dim, ds.t[i], ds.t_last, v);
+        if (probs != nullptr && it.x == 0 && k == 0) probs[ds.perm[i]] = p;
     }
-    store_block_sums<GRAD>(v, sh, block_out, k, (n + kRowsPerBlock - 1) / kRowsPerBlock, fb);
+    const size_t blk = (size_t)ds.blk0 + (size_t)k * ds.fin_blocks + fb;
+    for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
+        const double2 r = block_reduce_dd(make_double2(v[q], 0.0), sh);
+        if (threadIdx.x == 0) {
+            double *o = block_out + (blk * kResultWidth + q) * 2;
+            o[0] = r.x;
+            o[1] = r.y;
+        }
+    }
+}
+
+// One workgroup per (dataset, vector): fixed-order double-double sum of its block partials plus
+// the N (D-1) log(1/sqrt(2 pi)) constant (src/NewHawkes.hpp:659).
+__global__ __launch_bounds__(kRowsPerBlock) void sym_reduce_kernel(SymLaunch L, int width,
+                                                                   const double *__restrict__ block_out,
+                                                                   double *__restrict__ result) {
+    __shared__ double2 sh[kRowsPerBlock];
+    const int2 it = L.res_item[blockIdx.x];
+    const SymData &ds = L.data[it.x];
+    const int k = it.y;
+    const size_t blk0 = (size_t)ds.blk0 + (size_t)k * ds.fin_blocks;
+    for (int q = 0; q < width; ++q) {
+        double2 a = make_double2(0.0, 0.0);
+        for (int b = threadIdx.x; b < ds.fin_blocks; b += kRowsPerBlock) {
+            const double *o = block_out + ((blk0 + b) * kResultWidth + q) * 2;
+            a = dd_add(a, make_double2(o[0], o[1]));
+        }
+        double2 r = block_reduce_dd(a, sh);
+        if (threadIdx.x == 0) {
+            if (q == 0) {
+                const double c = -0.91893853320467274178 * (double)ds.n * (double)(L.dim - 1);
+                r = dd_add(r, make_double2(c, 0.0));
+            }
+            result[(size_t)(ds.res0 + k) * kResultWidth + q] = r.x + r.y;
+        }
+    }
 }
 
 // ---------------- host side ----------------
@@ -368,89 +423,136 @@ static long long sym_items_for(int nb, int R, int L) {
     return items;
 }
 
-SymPlan plan_sym_grid(int n, int dim, int K, int target_blocks, int rows_per_lane) {
+// Chunk length per dataset: the longest (multiple of kTile, at most the earlier-column span) that
+// still gives at least target_items work items over all datasets and vectors.
+SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_per_lane, int target_items,
+                 bool grad) {
     if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
+    if (n.size() != K.size() || n.empty()) throw std::invalid_argument("dataset sizes and vector counts differ");
     SymPlan plan;
-    SymGrid &g = plan.grid;
-    g.A = rows_per_lane;
-    g.R = kSymWaves * 64 * g.A;
-    g.nb = (n + g.R - 1) / g.R;
-    const int max_len = ((g.nb > 1 ? (g.nb - 1) * g.R : g.R) + kTile - 1) / kTile * kTile;
-    int L = max_len;
-    while (L > kTile && (long long)K * sym_items_for(g.nb, g.R, L) < target_blocks) L = ((L / 2 + kTile - 1) / kTile) * kTile;
-    g.L = L;
-    g.max_chunks = g.nb > 1 ? ((g.nb - 1) * g.R + L - 1) / L : 1;
-    const int NC = 3;
-    g.col_stride = (long long)NC * g.R * ((long long)g.nb * (g.nb - 1) / 2);
-    g.skip_zero = 1;
-    g.zero_arg_b = kZeroExpArg;
-    g.zero_arg_s = kZeroExpArg;
-    plan.items.clear();
-    for (int P = g.nb - 1; P >= 1; --P) {
-        const int nch = (P * g.R + L - 1) / L;
-        for (int c = 0; c < nch; ++c) plan.items.push_back(make_int2(P, c));
+    plan.A = rows_per_lane;
+    plan.R = kSymWaves * 64 * plan.A;
+    const int R = plan.R;
+    const int nd = (int)n.size();
+    std::vector<int> nb(nd), maxlen(nd);
+    for (int d = 0; d < nd; ++d) {
+        if (n[d] < 1 || K[d] < 1) throw std::invalid_argument("every dataset needs events and a vector");
+        nb[d] = (n[d] + R - 1) / R;
+        maxlen[d] = ((nb[d] > 1 ? (nb[d] - 1) * R : R) + kTile - 1) / kTile * kTile;
     }
-    for (int P = g.nb - 1; P >= 0; --P) plan.items.push_back(make_int2(P, -1));
-    g.items = (int)plan.items.size();
-    g.item = nullptr;
-    (void)dim;
+    int L = 0;
+    for (int d = 0; d < nd; ++d) L = std::max(L, maxlen[d]);
+    auto total_items = [&](int len) {
+        long long t = 0;
+        for (int d = 0; d < nd; ++d) t += (long long)K[d] * sym_items_for(nb[d], R, std::min(len, maxlen[d]));
+        return t;
+    };
+    while (L > kTile && total_items(L) < target_items) L = ((L / 2 + kTile - 1) / kTile) * kTile;
+
+    const int NR = grad ? 6 : 2, NC = grad ? 3 : 1;
+    size_t row0 = 0, col0 = 0, blk0 = 0;
+    int res0 = 0, param0 = 0;
+    struct Cost {
+        long long cost;
+        int4 item;
+    };
+    std::vector<Cost> items;
+    for (int d = 0; d < nd; ++d) {
+        SymData s{};
+        s.n = n[d];
+        s.nb = nb[d];
+        s.L = std::min(L, maxlen[d]);
+        s.max_chunks = nb[d] > 1 ? ((nb[d] - 1) * R + s.L - 1) / s.L : 1;
+        s.K = K[d];
+        s.param0 = param0;
+        s.fin_blocks = (n[d] + kRowsPerBlock - 1) / kRowsPerBlock;
+        const size_t rpad = (size_t)nb[d] * R;
+        s.row0 = (long long)row0;
+        s.row_kstride = (long long)((size_t)(s.max_chunks + 1) * NR * rpad);
+        s.col0 = (long long)col0;
+        s.col_kstride = (long long)((size_t)NC * R * ((size_t)nb[d] * (nb[d] - 1) / 2));
+        s.blk0 = (long long)blk0;
+        s.res0 = res0;
+        row0 += (size_t)K[d] * s.row_kstride;
+        col0 += (size_t)K[d] * s.col_kstride;
+        blk0 += (size_t)K[d] * s.fin_blocks;
+        res0 += K[d];
+        param0 += K[d];
+        for (int k = 0; k < K[d]; ++k) {
+            for (int P = nb[d] - 1; P >= 0; --P) {
+                const int P0 = P * R;
+                const int nch = P > 0 ? (P0 + s.L - 1) / s.L : 0;
+                for (int c = 0; c < nch; ++c) {
+                    const int cols = std::min(s.L, P0 - c * s.L);
+                    items.push_back({(long long)R * cols, make_int4(d, P, c, k)});
+                }
+                const int rows = std::min(R, n[d] - P0);
+                items.push_back({(long long)rows * rows, make_int4(d, P, -1, k)});
+            }
+            for (int fb = 0; fb < s.fin_blocks; ++fb) plan.fin_items.push_back(make_int4(d, k, fb, 0));
+            plan.res_items.push_back(make_int2(d, k));
+        }
+        plan.data.push_back(s);
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Cost &a, const Cost &b) { return a.cost > b.cost; });
+    plan.items.reserve(items.size());
+    for (const Cost &c : items) plan.items.push_back(c.item);
+    plan.rowpart_doubles = row0;
+    plan.colpart_doubles = col0;
+    plan.block_partials = blk0;
+    plan.results = res0;
     return plan;
 }
 
-size_t sym_rowpart_doubles(const SymGrid &g, int K, bool grad) {
-    return (size_t)K * (g.max_chunks + 1) * (grad ? 6 : 2) * (size_t)g.nb * g.R;
-}
-
-size_t sym_colpart_doubles(const SymGrid &g, int K, bool grad) {
-    (void)grad;
-    return (size_t)K * (size_t)g.col_stride;
-}
-
 template <int D, bool GRAD, bool F32>
-static void launch_sym_d(const Events &ev, const double *params, int K, const SymGrid &g, double *rowpart,
-                         double *colpart, hipStream_t s) {
-    dim3 grid((unsigned)g.items, (unsigned)K);
-    if (g.A == 2)
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
-                           colpart);
+static void launch_sym_d(const SymLaunch &L, const double *params, double *rowpart, double *colpart, hipStream_t s) {
+    if (L.A == 2)
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+                           params, rowpart, colpart);
     else
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32>), grid, dim3(kSymWaves * 64), 0, s, ev, params, g, rowpart,
-                           colpart);
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+                           params, rowpart, colpart);
 }
 
 template <bool F32>
-static void launch_sym_p(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
-                         double *rowpart, double *colpart, hipStream_t s) {
-    switch (ev.dim * 2 + (grad ? 1 : 0)) {
-    case 2: launch_sym_d<1, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
-    case 3: launch_sym_d<1, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
-    case 4: launch_sym_d<2, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
-    case 5: launch_sym_d<2, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
-    case 6: launch_sym_d<3, false, F32>(ev, params, K, g, rowpart, colpart, s); break;
-    case 7: launch_sym_d<3, true, F32>(ev, params, K, g, rowpart, colpart, s); break;
+static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, double *rowpart, double *colpart,
+                         hipStream_t s) {
+    switch (L.dim * 2 + (grad ? 1 : 0)) {
+    case 2: launch_sym_d<1, false, F32>(L, params, rowpart, colpart, s); break;
+    case 3: launch_sym_d<1, true, F32>(L, params, rowpart, colpart, s); break;
+    case 4: launch_sym_d<2, false, F32>(L, params, rowpart, colpart, s); break;
+    case 5: launch_sym_d<2, true, F32>(L, params, rowpart, colpart, s); break;
+    case 6: launch_sym_d<3, false, F32>(L, params, rowpart, colpart, s); break;
+    case 7: launch_sym_d<3, true, F32>(L, params, rowpart, colpart, s); break;
     default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
 }
 
-void launch_sym(const Events &ev, const double *params, int K, const SymGrid &g, bool grad, bool fp32,
-                double *rowpart, double *colpart, hipStream_t s) {
-    if (g.A != 1 && g.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
+void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart, double *colpart,
+                hipStream_t s) {
+    if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
+    if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
+    if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
     if (fp32)
-        launch_sym_p<true>(ev, params, K, g, grad, rowpart, colpart, s);
+        launch_sym_p<true>(L, params, grad, rowpart, colpart, s);
     else
-        launch_sym_p<false>(ev, params, K, g, grad, rowpart, colpart, s);
+        launch_sym_p<false>(L, params, grad, rowpart, colpart, s);
 }
 
-void launch_sym_finalize(const Events &ev, const double *params, int K, const SymGrid &g, bool grad,
-                         const double *rowpart, const double *colpart, double *block_out, double *probs,
-                         hipStream_t s) {
-    dim3 grid((unsigned)((ev.n + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)K);
+void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
+                         const double *colpart, double *block_out, double *probs, hipStream_t s) {
     if (grad)
-        hipLaunchKernelGGL((sym_finalize_kernel<true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, rowpart,
-                           colpart, block_out, probs);
+        hipLaunchKernelGGL((sym_finalize_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
+                           params, rowpart, colpart, block_out, probs);
     else
-        hipLaunchKernelGGL((sym_finalize_kernel<false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, rowpart,
-                           colpart, block_out, probs);
+        hipLaunchKernelGGL((sym_finalize_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
+                           params, rowpart, colpart, block_out, probs);
+}
+
+void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s) {
+    hipLaunchKernelGGL(sym_reduce_kernel, dim3((unsigned)L.res_items), dim3(kRowsPerBlock), 0, s, L,
+                       grad ? kResultWidth : 1, block_out, result);
 }
 
 } // namespace kern

@@ -134,6 +134,29 @@ int hph_set_pair_config(hph_engine *engine, int kind, int rows_per_lane, int tar
 int hph_autotune(hph_engine *engine, int repeats, double *best_ms);
 int hph_get_tuned_config(hph_engine *engine, int *out, size_t length);
 
+/* ---- Dataset batches (extension; no counterpart in the reference) ----
+ * Many independent datasets of uneven size (dimension 1..3), each with its own parameter
+ * vectors, evaluated by one persistent launch over a cost-ordered work queue. Flags as in
+ * hph_create (FLOAT: FP32-bulk; ALL_PAIRS). */
+typedef struct hph_batch hph_batch;
+int hph_batch_create(int embedding_dimension, int dataset_count, const int *location_counts, long flags, int device,
+                     hph_batch **out);
+int hph_batch_destroy(hph_batch *batch);
+/* Dataset d: locations row-major N_d x D (length N_d*D), times length N_d (T = last element). */
+int hph_batch_set_data(hph_batch *batch, int dataset, const double *locations, size_t locations_length,
+                       const double *times, size_t times_length);
+/* vectors_per_dataset[d] >= 1; params: the vectors of dataset 0, then dataset 1, ..., each 6
+ * values; count = 6 x total vectors. */
+int hph_batch_set_parameters(hph_batch *batch, const int *vectors_per_dataset, const double *params, size_t count);
+/* loglik[total_vectors]; grad (total_vectors x 6, may be NULL), in the order of the parameters. */
+int hph_batch_get_log_likelihood(hph_batch *batch, double *loglik, double *grad, size_t total_vectors);
+/* enabled != 0: capture the launch sequence in a HIP graph once per shape and replay it. */
+int hph_batch_set_graph(hph_batch *batch, int enabled);
+/* Work decomposition: rows per lane (1 or 2) and the target number of work items. */
+int hph_batch_set_config(hph_batch *batch, int rows_per_lane, int target_items);
+/* Device time (ms) of the last evaluation's launch sequence. */
+int hph_batch_get_timing(hph_batch *batch, double *out);
+
 #ifdef __cplusplus
 }
 #endif

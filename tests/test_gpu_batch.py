@@ -1,0 +1,118 @@
+"""Dataset batches: one launch over many datasets of uneven size and their parameter vectors."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from hawkes_amd import Engine, _lib, synth
+from hawkes_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def uneven_datasets(count, lo, hi, seed=666000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for d in range(count):
+        n_target = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        b = max(1, int(round(0.7 * n_target)))
+        locs, times = synth.cluster_process(b, side=math.sqrt(n_target / 100.0), seed=seed + d)
+        out.append((locs, times))
+    return out
+
+
+def make_batch(data, params, flags=_lib.OPENCL):
+    b = Batch(2, [len(t) for _, t in data], flags=flags)
+    for d, (locs, times) in enumerate(data):
+        b.set_data(d, locs, times)
+    b.set_parameters(params)
+    return b
+
+
+def test_batch_matches_oracle_per_dataset():
+    data = uneven_datasets(6, 200, 3000, seed=11)
+    params = [synth.jitter_params(3, seed=700 + d) for d in range(len(data))]
+    b = make_batch(data, params)
+    ll, g = b.log_likelihood()
+    lls, gs = b.split(ll), b.split(g)
+    for d, (locs, times) in enumerate(data):
+        for k in range(params[d].shape[0]):
+            ref = oracle.total(locs, times, params[d][k])
+            assert rel_err(lls[d][k], ref["ll"]) <= 1e-12, (d, k)
+            err = np.abs(gs[d][k] - ref["grad"]) / np.maximum(np.abs(ref["grad"]), 1e-3 * ref["scale"][1:])
+            assert err.max() <= 1e-12, (d, k, err)
+
+
+def test_batch_matches_single_engines():
+    data = uneven_datasets(5, 2000, 20000, seed=21)
+    params = [synth.jitter_params(2, seed=800 + d) for d in range(len(data))]
+    b = make_batch(data, params)
+    ll, g = b.log_likelihood()
+    lls, gs = b.split(ll), b.split(g)
+    for d, (locs, times) in enumerate(data):
+        e = Engine(2, len(times))
+        e.set_pair_kernel(_lib.PAIR_KERNEL_SYMMETRIC)
+        e.update_locations(locs)
+        e.set_times(times)
+        for k in range(2):
+            e.set_parameters(params[d][k])
+            l1, g1 = e.log_likelihood_and_gradient()
+            assert rel_err(lls[d][k], l1) <= 1e-13
+            assert np.all(np.abs(gs[d][k] - g1) <= 1e-12 * np.maximum(np.abs(g1), 1.0))
+        e.close()
+
+
+def test_graph_replays_match_eager():
+    data = uneven_datasets(4, 500, 8000, seed=31)
+    params = [synth.jitter_params(2, seed=900 + d) for d in range(len(data))]
+    b = make_batch(data, params)
+    eager = b.log_likelihood()
+    b.set_graph(True)
+    locs, times = synth.cluster_process(560, seed=3)  # eager work between replays on another engine
+    e_other = Engine(2, len(times))
+    e_other.update_locations(locs)
+    e_other.set_times(times)
+    e_other.set_parameters(synth.P0)
+    for rep in range(4):
+        got = b.log_likelihood()
+        assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1]), rep
+        e_other.log_likelihood_and_gradient()
+        e_other.set_parameters(synth.P0)
+    # new parameter values replay the same graph
+    params2 = [p * 1.01 for p in params]
+    b.set_parameters(params2)
+    g2 = b.log_likelihood()
+    b.set_graph(False)
+    eager2 = b.log_likelihood()
+    assert np.array_equal(g2[0], eager2[0]) and np.array_equal(g2[1], eager2[1])
+    assert not np.array_equal(g2[0], eager[0])
+
+
+def test_batch_fp32_bulk_error():
+    data = uneven_datasets(4, 1000, 5000, seed=41)
+    params = [synth.jitter_params(2, seed=950 + d) for d in range(len(data))]
+    b = make_batch(data, params, flags=_lib.OPENCL | _lib.FLOAT)
+    ll, g = b.log_likelihood()
+    lls = b.split(ll)
+    for d, (locs, times) in enumerate(data):
+        for k in range(2):
+            ref = oracle.total(locs, times, params[d][k], precision="double")
+            assert rel_err(lls[d][k], ref["ll"]) <= 1e-6
+
+
+def test_batch_errors():
+    b = Batch(2, [10, 20])
+    with pytest.raises(_lib.HphError) as ex:
+        b.set_data(2, np.zeros((10, 2)), np.zeros(10))
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    with pytest.raises(_lib.HphError) as ex:
+        b.set_data(0, np.zeros((9, 2)), np.zeros(9))
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    b.set_parameters([synth.jitter_params(1), synth.jitter_params(1)])
+    with pytest.raises(_lib.HphError) as ex:
+        b.log_likelihood()
+    assert ex.value.code == _lib.HPH_ERR_NOT_INITIALIZED
+    with pytest.raises(_lib.HphError):
+        Batch(4, [10])
