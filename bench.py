@@ -137,6 +137,85 @@ def cpu_baseline(locs, times, params, seconds):
     }
 
 
+def dataset_batch(count):
+    """Uneven multi-dataset batch: `count` cluster-process datasets (N log-uniform in 2e3..2e5),
+    8 jittered parameter vectors each, LL + gradient for all in one graph-replayed launch, FP64
+    and FP32-bulk; FP32 error stated per dataset against the FP64 result."""
+    from hawkes_amd import _lib, synth
+    from hawkes_amd.batch import Batch
+    data = synth.uneven_datasets(count)
+    params = [synth.jitter_params(8, seed=669 + d) for d in range(count)]
+    pairs = sum(8 * len(t) ** 2 for _, t in data)
+    out = {"datasets": count, "vectors_per_dataset": 8, "N_min": min(len(t) for _, t in data),
+           "N_max": max(len(t) for _, t in data), "pairs_per_launch": float(pairs)}
+    res = {}
+    for name, flags in (("fp64", _lib.OPENCL), ("fp32_bulk", _lib.OPENCL | _lib.FLOAT)):
+        b = Batch(2, [len(t) for _, t in data], flags=flags)
+        for d, (l, t) in enumerate(data):
+            b.set_data(d, l, t)
+        b.set_parameters(params)
+        b.set_graph(True)
+        b.log_likelihood()  # captures the graph
+        t0 = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            ll, g = b.log_likelihood()
+        wall = (time.perf_counter() - t0) / reps
+        res[name] = (ll, g)
+        out[name] = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing()}
+        b.close()
+    ll64, g64 = res["fp64"]
+    ll32, g32 = res["fp32_bulk"]
+    rel = np.abs(ll32 - ll64) / np.abs(ll64)
+    starts = np.cumsum([0] + [8] * count)
+    out["fp32_bulk"]["rel_err_ll_per_dataset_max"] = [float(rel[starts[d]:starts[d + 1]].max()) for d in range(count)]
+    out["fp32_bulk"]["rel_err_ll_max"] = float(rel.max())
+    return out
+
+
+def large_dataset(n_leapfrog=20):
+    """N ~ 1e6 cluster process (700k background events, 100 x 100 km, 43,800 h): FP32-bulk
+    log-likelihood + gradient + self-excitation probabilities with their error against the FP64
+    path on the same device, and one 20-step HMC leapfrog trajectory (21 gradient evaluations)
+    timed end to end."""
+    from hawkes_amd import Engine, _lib, sampler, synth
+    locs, times, p = synth.workload("cluster_b700k")
+    n = len(times)
+    out = {"N": n, "pairs_per_call": float(n) * n}
+    res = {}
+    for name, flags in (("fp64", _lib.OPENCL), ("fp32_bulk", _lib.OPENCL | _lib.FLOAT)):
+        e = Engine(2, n, flags=flags)
+        e.update_locations(locs)
+        e.set_times(times)
+        e.set_parameters(p)
+        e.set_timing(True)
+        ll, g = e.log_likelihood_and_gradient()
+        t0 = time.perf_counter()
+        e.set_parameters(p)
+        ll, g = e.log_likelihood_and_gradient()
+        t_llg = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        probs = e.probs_self_excite()
+        t_probs = time.perf_counter() - t0
+        res[name] = (ll, g, probs)
+        out[name] = {"ms_loglik_grad": 1e3 * t_llg, "ms_probs": 1e3 * t_probs,
+                     "value": float(n) * n / t_llg}
+        if name == "fp32_bulk":
+            rng = np.random.default_rng(7)
+            theta = np.log(p)
+            m0 = rng.normal(size=6)
+            t0 = time.perf_counter()
+            th, m, u, du = sampler.leapfrog(e, theta, m0, 1e-3, n_leapfrog)
+            out["hmc_leapfrog"] = {"steps": n_leapfrog, "gradient_evaluations": n_leapfrog + 1,
+                                   "seconds": time.perf_counter() - t0, "mode": "fp32_bulk"}
+        e.close()
+    (l64, g64, p64), (l32, g32, p32) = res["fp64"], res["fp32_bulk"]
+    out["fp32_bulk"]["rel_err_ll_vs_fp64"] = abs(l32 - l64) / abs(l64)
+    out["fp32_bulk"]["rel_err_grad_vs_fp64"] = float(np.max(np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-3 * np.max(np.abs(g64)))))
+    out["fp32_bulk"]["err_probs_vs_fp64"] = float(np.max(np.abs(p32 - p64) / np.maximum(p64, 1e-3)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +225,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", default="cluster_b70k")
+    ap.add_argument("--datasets", type=int, default=32, help="uneven dataset batch size (0 skips the leg)")
+    ap.add_argument("--large", type=int, default=1, help="1: N ~ 1e6 FP32-bulk + HMC leg, 0: skip")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
@@ -252,6 +333,9 @@ def main():
                    "speedup_vs_plain": bvalue / value}
     e.close()
 
+    datasets = dataset_batch(args.datasets) if args.datasets > 0 else None
+    large = large_dataset() if args.large else None
+
     cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     out = {
@@ -274,6 +358,8 @@ def main():
         "batched": batched,
         "all_pairs": all_pairs,
         "fp32_bulk": fp32,
+        "dataset_batch": datasets,
+        "large_n1e6": large,
     }
     print(json.dumps(out))
 

@@ -59,6 +59,20 @@ def causal_pairs(times):
     return int(np.searchsorted(t, t, side="left").sum())
 
 
+def uneven_datasets(count=32, lo=2_000, hi=200_000, seed=666000):
+    """Independent cluster-process datasets with N log-uniform in [lo, hi] (realised N varies),
+    square side sqrt(N/100) km, background count 0.7 N; dataset d uses seed + d."""
+    import math
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for d in range(count):
+        n_target = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        locs, times = cluster_process(max(1, int(round(0.7 * n_target))), side=math.sqrt(n_target / 100.0),
+                                      seed=seed + d)
+        out.append((locs, times))
+    return out
+
+
 # Named workloads (sizes and generators of the benchmark configurations).
 def workload(name):
     if name == "uniform_n1000":
