@@ -205,9 +205,13 @@ def large_dataset(n_leapfrog=20):
             theta = np.log(p)
             m0 = rng.normal(size=6)
             t0 = time.perf_counter()
-            th, m, u, du = sampler.leapfrog(e, theta, m0, 1e-3, n_leapfrog)
+            u0, du0 = sampler._u_and_grad(e, theta)
+            # step sized so the first momentum kick stays O(0.01): log-space gradients scale with N
+            step = 0.01 / (1.0 + float(np.linalg.norm(du0)))
+            th, m, u, du = sampler.leapfrog(e, theta, m0, step, n_leapfrog, (u0, du0))
             out["hmc_leapfrog"] = {"steps": n_leapfrog, "gradient_evaluations": n_leapfrog + 1,
-                                   "seconds": time.perf_counter() - t0, "mode": "fp32_bulk"}
+                                   "seconds": time.perf_counter() - t0, "mode": "fp32_bulk", "step_size": step,
+                                   "energy_error": float((u + 0.5 * m @ m) - (u0 + 0.5 * m0 @ m0))}
         e.close()
     (l64, g64, p64), (l32, g32, p32) = res["fp64"], res["fp32_bulk"]
     out["fp32_bulk"]["rel_err_ll_vs_fp64"] = abs(l32 - l64) / abs(l64)

@@ -95,8 +95,12 @@ def adaptive_mh(engine, n_iter, params, radius=2.0, burn_in=0, rng=None, indices
 
 
 def _u_and_grad(engine, theta):
-    """U(theta) = -loglik(e^theta) - log prior(e^theta) - sum(theta) and its gradient."""
-    p = np.exp(theta)
+    """U(theta) = -loglik(e^theta) - log prior(e^theta) - sum(theta) and its gradient; U = inf
+    (a rejected point) where the parameters are not finite."""
+    with np.errstate(over="ignore"):
+        p = np.exp(theta)
+    if not np.all(np.isfinite(p)):
+        return np.inf, np.zeros(6)
     engine.set_parameters(p)
     ll, g = engine.log_likelihood_and_gradient()
     u = -ll - log_prior(p) - float(np.sum(theta))
@@ -113,6 +117,8 @@ def leapfrog(engine, theta, momentum, step_size, n_steps, u_grad=None):
     for s in range(n_steps):
         th = th + step_size * m
         u, du = _u_and_grad(engine, th)
+        if not np.isfinite(u):
+            break
         m = m - (step_size if s < n_steps - 1 else 0.5 * step_size) * du
     return th, m, u, du
 
@@ -148,12 +154,17 @@ def hmc(engine, n_iter, params, step_size=0.01, n_leapfrog=20, burn_in=0, rng=No
 
 
 def _u_and_grad_batch(engine, thetas):
-    p = np.exp(thetas)
+    with np.errstate(over="ignore"):
+        p = np.exp(thetas)
+    bad = ~np.all(np.isfinite(p), axis=1)
+    p = np.where(bad[:, None], 1.0, p)
     engine.set_parameters_batch(p)
     ll, g = engine.log_likelihood_batch(gradient=True)
     lp = np.array([log_prior(q) for q in p])
     u = -ll - lp - thetas.sum(axis=1)
     du = -(g + grad_log_prior(p)) * p - 1.0
+    u[bad] = np.inf
+    du[bad] = 0.0
     return u, du
 
 

@@ -59,8 +59,15 @@ def test_mh_and_hmc_on_engine(data):
     e = engine_for(locs, times)
     mh = sampler.adaptive_mh(e, 200, synth.P0, rng=np.random.default_rng(1))
     assert 0.0 < mh["acceptance"] < 1.0
-    h = sampler.hmc(e, 10, synth.P0, step_size=0.005, n_leapfrog=10, rng=np.random.default_rng(2), adapt=False)
-    assert h["acceptance"] > 0.3 and np.all(np.isfinite(h["target"]))
+    start = mh["samples"][-1]
+    # leapfrog with a small step conserves H = U + |m|^2 / 2 (second-order integrator)
+    theta = np.log(start)
+    m0 = np.random.default_rng(9).normal(size=6)
+    u0, du0 = sampler._u_and_grad(e, theta)
+    th, m, u1, _ = sampler.leapfrog(e, theta, m0, 1e-5, 10, (u0, du0))
+    assert abs((u1 + 0.5 * m @ m) - (u0 + 0.5 * m0 @ m0)) < 1e-3
+    h = sampler.hmc(e, 30, start, step_size=1e-3, n_leapfrog=10, burn_in=15, rng=np.random.default_rng(2))
+    assert h["acceptance"] > 0.0 and np.all(np.isfinite(h["target"]))
     chains = sampler.hmc_chains(e, synth.jitter_params(8), 3, step_size=0.005, n_leapfrog=5,
                                 rng=np.random.default_rng(3))
     assert chains["samples"].shape == (3, 8, 6)
