@@ -372,9 +372,18 @@ class HipHawkes : public AbstractHawkes {
         return ll_;
     }
 
+    // The finalize kernel writes the probabilities on every single-vector evaluation, so after
+    // getSumOfLikContribs / getLogLikeGradient with unchanged data and parameters this is a copy
+    // (the reference recomputes, src/NewHawkes.hpp:279 notes the missing caching).
     void getProbsSelfExcite(double *result, size_t length) override {
         if (result == nullptr) throw std::invalid_argument("result pointer is null");
         if (length != (size_t)locationCount) throw std::invalid_argument("getProbsSelfExcite: length must be locationCount");
+        if (probs_valid_) {
+            check(hipSetDevice(device_), "hipSetDevice");
+            check(hipMemcpy(result, d_probs_.ptr, sizeof(double) * locationCount, hipMemcpyDeviceToHost),
+                  "download probs");
+            return;
+        }
         evaluate_single(false, result);
     }
 
@@ -525,7 +534,7 @@ class HipHawkes : public AbstractHawkes {
     }
 
   private:
-    void invalidate() { ll_valid_ = grad_valid_ = false; }
+    void invalidate() { ll_valid_ = grad_valid_ = probs_valid_ = false; }
 
     void require_ready() const {
         if (!have_locs_) throw NotInitialized("locations are not set (updateLocations)");
@@ -604,9 +613,14 @@ class HipHawkes : public AbstractHawkes {
 
     // Runs the pair pass, finalize and reduction kernels for K parameter vectors; results land in
     // h_result_ ([K][7]).
-    void run(const double *params, int K, bool grad, double *probs_host) { run_with(params, K, grad, probs_host, config_for(K, grad)); }
+    // probs_host: also download the probabilities (implies write_probs); write_probs: the finalize
+    // kernel writes vector 0's probabilities to d_probs_.
+    void run(const double *params, int K, bool grad, double *probs_host, bool write_probs = false) {
+        run_with(params, K, grad, probs_host, config_for(K, grad), write_probs);
+    }
 
-    void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg) {
+    void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg,
+                  bool write_probs = false) {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
         if (data_dirty_) upload();
@@ -622,7 +636,7 @@ class HipHawkes : public AbstractHawkes {
               "upload params");
         const kern::Events ev = events();
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
-        double *probs_dev = probs_host ? d_probs_.as<double>() : nullptr;
+        double *probs_dev = (probs_host || write_probs) ? d_probs_.as<double>() : nullptr;
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
@@ -665,7 +679,8 @@ class HipHawkes : public AbstractHawkes {
 
     void evaluate_single(bool grad, double *probs_host) {
         if (!have_params_) throw NotInitialized("parameters are not set (setParameters)");
-        run(params_, 1, grad, probs_host);
+        run(params_, 1, grad, probs_host, true);
+        probs_valid_ = true;
         const double *r = h_result_.as<double>();
         ll_ = r[0];
         ll_valid_ = true;
@@ -692,7 +707,7 @@ class HipHawkes : public AbstractHawkes {
     std::vector<double> batch_;
 
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
-    bool ll_valid_ = false, grad_valid_ = false;
+    bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
     double stored_params_[6] = {0, 0, 0, 0, 0, 0};

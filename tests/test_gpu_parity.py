@@ -156,6 +156,22 @@ def test_zero_tile_skip_is_bitwise_exact(name, params, kernel):
     assert np.array_equal(out[0][2], out[1][2]), name
 
 
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_probs_from_likelihood_pass_equal_fresh_pass(kernel):
+    locs, times = synth.cluster_process(3000, seed=19)
+    e = make_engine(locs, times, synth.P0, kernel=kernel)
+    e.log_likelihood_and_gradient()
+    cached = e.probs_self_excite()
+    e.set_parameters(synth.P0)  # invalidates: the next call runs a likelihood-only pass
+    fresh = e.probs_self_excite()
+    assert np.array_equal(cached, fresh)
+    P = synth.jitter_params(3)
+    e.set_parameters_batch(P)
+    e.log_likelihood_batch()  # batches leave the single-vector probabilities alone
+    assert np.array_equal(e.probs_self_excite(), fresh)
+    e.close()
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
