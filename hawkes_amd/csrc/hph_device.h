@@ -18,10 +18,12 @@ __device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp
 // exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
 // Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
 // fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of
-// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp keeps k inside int range; for
-// x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for x <= 709.
-__device__ __forceinline__ double exp_fp64(double x) {
-    x = fmax(x, -1000.0);
+// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp (CLAMP) keeps k inside int range for
+// any finite x; for x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for
+// x <= 709.
+template <bool CLAMP>
+__device__ __forceinline__ double exp_fp64_t(double x) {
+    if (CLAMP) x = fmax(x, -1000.0);
     const double shifter = 6755399441055744.0; // 1.5 * 2^52
     const double kf = __builtin_fma(x, HPH_LOG2E, shifter);
     const double k = kf - shifter;
@@ -42,13 +44,17 @@ __device__ __forceinline__ double exp_fp64(double x) {
     return __builtin_amdgcn_ldexp(p, ki);
 }
 
+// Without the clamp the shifter trick is exact while |x log2 e| < 2^31: the callers select the
+// clamp-free form only when the host has bounded every exponent of the pass below 1e9 in size.
+__device__ __forceinline__ double exp_fp64(double x) { return exp_fp64_t<true>(x); }
+
 // Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
 // constants are then pre-scaled by log2 e).
-template <bool F32>
+template <bool F32, bool CLAMP>
 __device__ __forceinline__ double background_exp(double cBx, double cBt, double r2, double dt2) {
     const double arg = __builtin_fma(cBt, dt2, cBx * r2);
     if (F32) return (double)exp2_f32((float)arg);
-    return exp_fp64(arg);
+    return exp_fp64_t<CLAMP>(arg);
 }
 
 // Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.

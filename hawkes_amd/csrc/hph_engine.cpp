@@ -87,6 +87,41 @@ struct HostBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Size bounds of a dataset's exponents: squared diagonal of the locations' bounding box and the
+// time span. With them the host bounds every pair's exp argument for given parameters.
+struct Extents {
+    double r2max = 0.0, span = 0.0;
+};
+
+static Extents extents_of(const std::vector<double> &locs, const std::vector<double> &times, int D) {
+    Extents e;
+    const size_t n = times.size();
+    if (n == 0) return e;
+    for (int d = 0; d < D; ++d) {
+        double lo = locs[d], hi = locs[d];
+        for (size_t i = 1; i < n; ++i) {
+            lo = std::min(lo, locs[i * D + d]);
+            hi = std::max(hi, locs[i * D + d]);
+        }
+        e.r2max += (hi - lo) * (hi - lo);
+    }
+    const auto mm = std::minmax_element(times.begin(), times.end());
+    e.span = *mm.second - *mm.first;
+    return e;
+}
+
+// True unless every background (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt +
+// sigma^2 r^2/2) exponent of the K vectors is below 1e9 in size, where the clamp-free exp is exact.
+static bool needs_clamp(const Extents &e, const double *params, int K) {
+    for (int k = 0; k < K; ++k) {
+        const double *p = params + 6 * (size_t)k;
+        const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
+        const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
+        if (!(aB < 1e9 && aS < 1e9)) return true;
+    }
+    return false;
+}
+
 // Drives the symmetric pass over a set of device-resident datasets: plans work items, keeps the
 // plan and its device tables while the shape is unchanged, owns the partial buffers.
 class SymRunner {
@@ -106,9 +141,9 @@ class SymRunner {
     // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev: dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
-                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, double *probs_dev,
-                 double *result, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                 hipEvent_t ev2 = nullptr) {
+                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, bool clamp,
+                 double *probs_dev, double *result, hipStream_t s, hipEvent_t ev0 = nullptr,
+                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
         std::vector<int> n(refs.size());
         for (size_t d = 0; d < refs.size(); ++d) n[d] = refs[d].n;
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
@@ -176,6 +211,7 @@ class SymRunner {
         L.skip_zero = skip_zero;
         L.zero_arg_b = fp32 ? kern::kZeroExpArg32 : kern::kZeroExpArg;
         L.zero_arg_s = kern::kZeroExpArg;
+        L.clamp = clamp ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
@@ -566,6 +602,7 @@ class HipHawkes : public AbstractHawkes {
         check(hipMemcpyAsync(d_t_.ptr, ht, sizeof(double) * npad_, hipMemcpyHostToDevice, stream_), "upload t");
         check(hipMemcpyAsync(d_perm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_), "upload perm");
         check(hipStreamSynchronize(stream_), "upload sync");
+        extents_ = extents_of(locs_, times_, D);
         data_dirty_ = false;
     }
 
@@ -640,7 +677,8 @@ class HipHawkes : public AbstractHawkes {
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, probs_dev, d_result_.as<double>(), stream_,
+                         cfg.target_blocks, skip_zero, needs_clamp(extents_, params, K), probs_dev,
+                         d_result_.as<double>(), stream_,
                          timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
@@ -693,6 +731,7 @@ class HipHawkes : public AbstractHawkes {
     static constexpr int kSymMinEvents = 4096;
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
+    Extents extents_;
 
     int device_;
     bool fp32_;
@@ -750,6 +789,7 @@ class HipHawkesBatch {
         }
         locs_.resize(nd);
         times_.resize(nd);
+        ext_.resize(nd);
         have_.assign(nd, false);
         d_x_.ensure(sizeof(double) * (size_t)D * off_[nd]);
         d_t_.ensure(sizeof(double) * (size_t)off_[nd]);
@@ -782,6 +822,7 @@ class HipHawkesBatch {
         check(hipSetDevice(device_), "hipSetDevice");
         locs_[d].assign(locs, locs + locs_len);
         times_[d].assign(times, times + times_len);
+        ext_[d] = extents_of(locs_[d], times_[d], D_);
         std::vector<int> perm(n);
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return times[a] < times[b]; });
@@ -852,12 +893,15 @@ class HipHawkesBatch {
         d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
+        const bool clamp = clamp_needed();
+        if (graph_exec_ && clamp != graph_clamp_) drop_graph(); // the graph holds one kernel variant
         if (graph_mode_) {
             if (!graph_exec_) {
-                enqueue(false); // plans, allocates and uploads the tables outside the capture
+                enqueue(false, clamp); // plans, allocates and uploads the tables outside the capture
                 check(hipStreamSynchronize(stream_), "stream sync");
                 check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                enqueue(false);
+                enqueue(false, clamp);
+                graph_clamp_ = clamp;
                 hipGraph_t g = nullptr;
                 check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
                 graph_ = g;
@@ -867,7 +911,7 @@ class HipHawkesBatch {
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
             check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         } else {
-            enqueue(true);
+            enqueue(true, clamp);
         }
         check(hipStreamSynchronize(stream_), "stream sync");
         float a = 0.f;
@@ -884,8 +928,17 @@ class HipHawkesBatch {
     double lastMs() const { return last_ms_; }
 
   private:
+    // True when some exponent of the current parameters may exceed the clamp-free exp's range.
+    bool clamp_needed() const {
+        bool clamp = false;
+        const double *hp = h_params_.as<double>();
+        for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
+            clamp = clamp || needs_clamp(ext_[d], hp + 6 * k0, K_[d]);
+        return clamp;
+    }
+
     // Parameter upload, pass, finalize, reduction and result download on the batch's stream.
-    void enqueue(bool events) {
+    void enqueue(bool events, bool clamp) {
         check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)total_, hipMemcpyHostToDevice,
                              stream_),
               "upload params");
@@ -894,7 +947,7 @@ class HipHawkesBatch {
             refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
                                          d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
         const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
-        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, nullptr,
+        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, clamp, nullptr,
                         d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
                         events ? ev_[2] : nullptr);
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,
@@ -917,10 +970,11 @@ class HipHawkesBatch {
     hipStream_t stream_ = nullptr;
     std::vector<int> off_, npad_;
     std::vector<std::vector<double>> locs_, times_;
+    std::vector<Extents> ext_;
     std::vector<bool> have_;
     std::vector<int> K_;
     int total_ = 0;
-    bool have_params_ = false, grad_ = true, graph_mode_ = false;
+    bool have_params_ = false, grad_ = true, graph_mode_ = false, graph_clamp_ = true;
     int A_ = 2, target_ = 8192;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

@@ -37,7 +37,7 @@ __device__ __forceinline__ double lane_shift_down(double v, int byte_addr) {
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
 // has rows past the last event, whose column contributions are masked by vm; F32: FP32-bulk
 // background exponential (cBx, cBt then carry the log2 e scale).
-template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32>
+template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool CLAMP>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
                                             double cSx, double nomega, double (&acc)[A][6],
@@ -65,7 +65,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             const double dt = ti[a] - q[D];
             if (DOB) {
                 const double dt2 = dt * dt;
-                const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
+                const double eb = background_exp<F32, CLAMP>(cBx, cBt, r2, dt2);
                 acc[a][0] += eb;
                 if (GRAD) {
                     acc[a][1] = __builtin_fma(eb, r2, acc[a][1]);
@@ -79,7 +79,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                 }
             }
             if (SMODE != kSNone) {
-                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+                double es = exp_fp64_t<CLAMP>(__builtin_fma(cSx, r2, nomega * dt));
                 if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
                 acc[a][3] += es;
                 if (GRAD) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
 // the dt > 0 mask.
-template <int D, bool GRAD, bool F32>
+template <int D, bool GRAD, bool F32, bool CLAMP>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
                                           double cBx, double cBt, double cSx, double nomega, double (&acc)[6]) {
     constexpr int ST = TileStride<D>::value;
@@ -116,13 +116,13 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
         }
         const double dt = ti - e[D];
         const double dt2 = dt * dt;
-        const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
+        const double eb = background_exp<F32, CLAMP>(cBx, cBt, r2, dt2);
         acc[0] += eb;
         if (GRAD) {
             acc[1] = __builtin_fma(eb, r2, acc[1]);
             acc[2] = __builtin_fma(eb, dt2, acc[2]);
         }
-        double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+        double es = exp_fp64_t<CLAMP>(__builtin_fma(cSx, r2, nomega * dt));
         es = dt > 0.0 ? es : 0.0;
         acc[3] += es;
         if (GRAD) {
@@ -132,33 +132,33 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     }
 }
 
-template <int D, bool GRAD, int A, bool TAIL, bool F32>
+template <int D, bool GRAD, int A, bool TAIL, bool F32, bool CLAMP>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
                                              double cBx, double cBt, double cSx, double nomega, double (&acc)[A][6],
                                              double (&ca)[GRAD ? 3 : 1]) {
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
         break;
     default: break; // every term of the subtile is exactly zero
     }
 }
 
 // Persistent pass: every workgroup takes work items from the queue until it is empty.
-template <int D, bool GRAD, int A, bool F32>
+template <int D, bool GRAD, int A, bool F32, bool CLAMP>
 __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const double *__restrict__ params,
                                                              double *__restrict__ rowpart,
                                                              double *__restrict__ colpart) {
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                 if (wave_live) {
 #pragma unroll
                     for (int a = 0; a < A; ++a)
-                        diag_pass<D, GRAD, F32>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
+                        diag_pass<D, GRAD, F32, CLAMP>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
                 }
             }
         } else {
@@ -273,10 +273,10 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                         for (int c = 0; c < NC; ++c) ca[c] = 0.0;
                         const double *sub = tile + sb * (D + 1) * 128;
                         if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc,
+                            sym_dispatch<D, GRAD, A, true, F32, CLAMP>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc,
                                                                 ca);
                         else
-                            sym_dispatch<D, GRAD, A, false, F32>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega,
+                            sym_dispatch<D, GRAD, A, false, F32, CLAMP>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega,
                                                                  acc, ca);
 #pragma unroll
                         for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
@@ -504,26 +504,26 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     return plan;
 }
 
-template <int D, bool GRAD, bool F32>
+template <int D, bool GRAD, bool F32, bool CLAMP>
 static void launch_sym_d(const SymLaunch &L, const double *params, double *rowpart, double *colpart, hipStream_t s) {
     if (L.A == 2)
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32, CLAMP>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
     else
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32, CLAMP>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
 }
 
-template <bool F32>
+template <bool F32, bool CLAMP>
 static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, double *rowpart, double *colpart,
                          hipStream_t s) {
     switch (L.dim * 2 + (grad ? 1 : 0)) {
-    case 2: launch_sym_d<1, false, F32>(L, params, rowpart, colpart, s); break;
-    case 3: launch_sym_d<1, true, F32>(L, params, rowpart, colpart, s); break;
-    case 4: launch_sym_d<2, false, F32>(L, params, rowpart, colpart, s); break;
-    case 5: launch_sym_d<2, true, F32>(L, params, rowpart, colpart, s); break;
-    case 6: launch_sym_d<3, false, F32>(L, params, rowpart, colpart, s); break;
-    case 7: launch_sym_d<3, true, F32>(L, params, rowpart, colpart, s); break;
+    case 2: launch_sym_d<1, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 3: launch_sym_d<1, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 4: launch_sym_d<2, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 5: launch_sym_d<2, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 6: launch_sym_d<3, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 7: launch_sym_d<3, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
     default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
 }
@@ -534,10 +534,17 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
-    if (fp32)
-        launch_sym_p<true>(L, params, grad, rowpart, colpart, s);
-    else
-        launch_sym_p<false>(L, params, grad, rowpart, colpart, s);
+    if (fp32) {
+        if (L.clamp)
+            launch_sym_p<true, true>(L, params, grad, rowpart, colpart, s);
+        else
+            launch_sym_p<true, false>(L, params, grad, rowpart, colpart, s);
+    } else {
+        if (L.clamp)
+            launch_sym_p<false, true>(L, params, grad, rowpart, colpart, s);
+        else
+            launch_sym_p<false, false>(L, params, grad, rowpart, colpart, s);
+    }
 }
 
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,

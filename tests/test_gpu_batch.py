@@ -90,6 +90,24 @@ def test_graph_replays_match_eager():
     assert not np.array_equal(g2[0], eager[0])
 
 
+def test_graph_recaptures_when_exponent_range_changes():
+    rng = np.random.default_rng(5)
+    data = []
+    for n in (700, 1300):
+        data.append((rng.uniform(0, 3, (n, 2)), np.sort(rng.uniform(0, 1e5, n))))
+    small = [np.array([[1.0, 0.9, 1e-4, 0.5, 0.8, 0.9]]), np.array([[1.1, 0.8, 1e-4, 0.6, 0.7, 1.0]])]
+    huge = [np.array([[1.0, 0.9, 1.0, 0.5, 0.8, 0.9]]), np.array([[1.1, 0.8, 1.0, 0.6, 0.7, 1.0]])]
+    b = make_batch(data, small)
+    b.set_graph(True)
+    b.log_likelihood()
+    for params in (huge, small, huge):
+        b.set_parameters(params)
+        got = b.log_likelihood()
+        for d, (locs, times) in enumerate(data):
+            ref = oracle.total(locs, times, params[d][0])
+            assert rel_err(got[0][d], ref["ll"]) <= 1e-12
+
+
 def test_batch_fp32_bulk_error():
     data = uneven_datasets(4, 1000, 5000, seed=41)
     params = [synth.jitter_params(2, seed=950 + d) for d in range(len(data))]

@@ -172,6 +172,18 @@ def test_probs_from_likelihood_pass_equal_fresh_pass(kernel):
     e.close()
 
 
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+def test_huge_exponents_use_the_clamped_exp(kernel):
+    """Exponents far beyond 1e9 (tauT^2 dt^2 / 2 up to 5e9 here): the engine must select the
+    clamped exp (the clamp-free shifter form is exact only below ~1.4e9)."""
+    rng = np.random.default_rng(77)
+    n = 1500
+    locs = rng.uniform(0, 3, (n, 2))
+    times = np.sort(rng.uniform(0, 1e5, n))
+    p = np.array([1.0, 0.9, 1.0, 0.5, 0.8, 0.9])
+    assert_matches_oracle(locs, times, p, name="huge", kernel=kernel)
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
