@@ -225,6 +225,46 @@ def large_dataset(n_leapfrog=20):
     return out
 
 
+def small_n():
+    """The N = 1000 workload (uniform on [0,10]^2 x [0, 8760) h): 100 repeated log-likelihood
+    evaluations on the plain path, and K = 64 jittered proposals (LL + gradient) in one batched
+    launch against 64 plain calls -- the regime where per-call overhead, not arithmetic, bounds
+    the plain path."""
+    from hawkes_amd import Engine, synth
+    locs, times, p0 = synth.workload("uniform_n1000")
+    n = len(times)
+    e = Engine(2, n)
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters(p0)
+    e.log_likelihood()
+    P = synth.jitter_params(100, seed=670)
+    t0 = time.perf_counter()
+    for k in range(100):
+        e.set_parameters(P[k])
+        e.log_likelihood()
+    t_ll = (time.perf_counter() - t0) / 100
+    Q = P[:64]
+    for k in range(64):  # warm the plain LL + gradient path
+        e.set_parameters(Q[k])
+        e.log_likelihood_and_gradient()
+    t0 = time.perf_counter()
+    for k in range(64):
+        e.set_parameters(Q[k])
+        e.log_likelihood_and_gradient()
+    t_plain = time.perf_counter() - t0
+    e.set_parameters_batch(Q)
+    e.log_likelihood_batch()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.log_likelihood_batch()
+    t_batch = (time.perf_counter() - t0) / reps
+    e.close()
+    return {"N": n, "loglik_ms_per_eval_100_evals": 1e3 * t_ll, "loglik_pair_evals_per_s": n * n / t_ll,
+            "K64_plain_ms": 1e3 * t_plain, "K64_batched_ms": 1e3 * t_batch, "K64_speedup": t_plain / t_batch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +384,7 @@ def main():
 
     datasets = dataset_batch(args.datasets) if args.datasets > 0 else None
     large = large_dataset() if args.large else None
+    small = small_n()
 
     cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
@@ -369,6 +410,7 @@ def main():
         "fp32_bulk": fp32,
         "dataset_batch": datasets,
         "large_n1e6": large,
+        "small_n1000": small,
     }
     print(json.dumps(out))
 
