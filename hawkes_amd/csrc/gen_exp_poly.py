@@ -1,14 +1,17 @@
-"""Generates hph_exp_coeffs.h: constants of the FP64 exp used in the pair kernels.
+"""Generates hph_exp_coeffs.h: constants of the FP64 exp and exp2 used in the pair kernels.
 
-Method (Cody & Waite argument reduction + polynomial, W. J. Cody and W. Waite,
+exp (Cody & Waite argument reduction + polynomial, W. J. Cody and W. Waite,
 "Software Manual for the Elementary Functions", 1980):
     x = k*ln2 + r,   k = nearest integer to x/ln2,   |r| <= ln2/2 (+ rounding slack)
     exp(x) = 2^k * p(r),   p ~ e^r on [-A, A]
 ln2 is split as LN2_HI + LN2_LO with LN2_HI holding 32 significant bits, so k*LN2_HI is exact
-for |k| < 2^21. p has p(0) = 1 exactly and the other coefficients minimise the weighted
-(relative) least-squares error on 3000 Chebyshev nodes at 40 digits, which lands close to the
-minimax polynomial. The script then measures the max relative error of the double-precision
-Horner evaluation with correctly rounded fma at 20000 points.
+for |k| < 2^21.
+exp2 (for exponents already scaled by log2 e):
+    y = k + f,   k = nearest integer to y,   f = y - k exact,   2^y = 2^k * q(f),   q ~ 2^f
+on [-1/2, 1/2]. Both polynomials have value 1 at 0 exactly and the other coefficients minimise
+the weighted (relative) least-squares error on 3000 Chebyshev nodes at 40 digits, which lands
+close to the minimax polynomial. The script then measures the max relative error of the
+double-precision Horner evaluation with correctly rounded fma at 20000 points.
 
 Usage: python hawkes_amd/csrc/gen_exp_poly.py [degree]   (writes hph_exp_coeffs.h beside itself)
 """
@@ -22,14 +25,15 @@ mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def fit(degree, A):
+def fit(degree, A, scale=1):
+    """p(r) = 1 + sum c_k r^k ~ e^(scale r) on [-A, A]."""
     nodes = [A * mp.cos(mp.pi * (j + mp.mpf(0.5)) / 3000) for j in range(3000)]
-    # unknowns c1..c_degree; p(r) = 1 + sum c_k r^k; minimise sum ((p(r) - e^r)/e^r)^2
+    # unknowns c1..c_degree; minimise sum ((p(r) - e^(scale r)) / e^(scale r))^2
     rows, rhs = [], []
     for r in nodes:
-        w = mp.exp(-r)
+        w = mp.exp(-scale * r)
         rows.append([w * r ** k for k in range(1, degree + 1)])
-        rhs.append(w * (mp.exp(r) - 1))
+        rhs.append(w * (mp.exp(scale * r) - 1))
     M = mp.matrix(rows)
     b = mp.matrix(rhs)
     sol = mp.qr_solve(M, b)[0]
@@ -48,7 +52,7 @@ def horner(coef_d, r):
 
 
 def main():
-    degree = int(sys.argv[1]) if len(sys.argv) > 1 else 11
+    degree = int(sys.argv[1]) if len(sys.argv) > 1 else 10
     ln2 = mp.log(2)
     ln2_hi = float(mp.nint(ln2 * 2 ** 32) / 2 ** 32)
     ln2_lo = float(ln2 - mp.mpf(ln2_hi))
@@ -64,10 +68,18 @@ def main():
         r = fma_d(-k, ln2_lo, fma_d(-k, ln2_hi, x))
         y = mp.ldexp(mp.mpf(horner(coef_d, r)), k)
         worst = max(worst, abs(float((y - mp.exp(x)) / mp.exp(x))))
+    coef2 = fit(degree, mp.mpf(0.5), ln2)
+    coef2_d = [float(c) for c in coef2]
+    worst2 = 0.0
+    for _ in range(20000):
+        y = random.uniform(-1070.0, 1020.0)
+        k = round(y)
+        v = mp.ldexp(mp.mpf(horner(coef2_d, y - k)), k)
+        worst2 = max(worst2, abs(float((v - mp.power(2, y)) / mp.power(2, y))))
     path = os.path.join(HERE, "hph_exp_coeffs.h")
     with open(path, "w") as f:
         f.write("// Generated by gen_exp_poly.py (degree %d); do not edit.\n" % degree)
-        f.write("// Measured max relative error of the FP64 evaluation: %.3e\n" % worst)
+        f.write("// Measured max relative error of the FP64 evaluation: exp %.3e, exp2 %.3e\n" % (worst, worst2))
         f.write("#pragma once\n")
         f.write("#define HPH_EXP_DEGREE %d\n" % degree)
         f.write("#define HPH_LOG2E %r\n" % log2e)
@@ -75,7 +87,9 @@ def main():
         f.write("#define HPH_LN2_LO %r\n" % ln2_lo)
         for k, c in enumerate(coef_d):
             f.write("#define HPH_EXP_C%d %r\n" % (k, c))
-    print("degree", degree, "max rel err %.3e" % worst, "->", path)
+        for k, c in enumerate(coef2_d):
+            f.write("#define HPH_EXP2_C%d %r\n" % (k, c))
+    print("degree", degree, "max rel err exp %.3e exp2 %.3e" % (worst, worst2), "->", path)
 
 
 if __name__ == "__main__":

@@ -18,12 +18,10 @@ __device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp
 // exp(x) in FP64 on the VALU (gfx950 has no FP64 transcendental instruction).
 // Cody-Waite reduction with a shifter constant: k = rint(x log2 e) appears in the low word of
 // fma(x, log2 e, 1.5*2^52); r = x - k ln2 in two fma; p(r) is the degree-10 polynomial of
-// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp (CLAMP) keeps k inside int range for
-// any finite x; for x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for
-// x <= 709.
-template <bool CLAMP>
-__device__ __forceinline__ double exp_fp64_t(double x) {
-    if (CLAMP) x = fmax(x, -1000.0);
+// gen_exp_poly.py; 2^k is applied by v_ldexp_f64. The clamp keeps k inside int range for any
+// finite x; for x < -1000 the result is 0 as in IEEE exp. Max relative error 3.4e-16 for x <= 709.
+__device__ __forceinline__ double exp_fp64(double x) {
+    x = fmax(x, -1000.0);
     const double shifter = 6755399441055744.0; // 1.5 * 2^52
     const double kf = __builtin_fma(x, HPH_LOG2E, shifter);
     const double k = kf - shifter;
@@ -44,17 +42,44 @@ __device__ __forceinline__ double exp_fp64_t(double x) {
     return __builtin_amdgcn_ldexp(p, ki);
 }
 
-// Without the clamp the shifter trick is exact while |x log2 e| < 2^31: the callers select the
-// clamp-free form only when the host has bounded every exponent of the pass below 1e9 in size.
-__device__ __forceinline__ double exp_fp64(double x) { return exp_fp64_t<true>(x); }
+// 2^y in FP64 for |y| < 2^31, without a clamp (callers use it only where the host has bounded
+// every exponent of the pass below 1e9 in size): k = rint(y) appears in the low word of
+// y + 1.5*2^52, f = y - k is exact (|f| <= 1/2), q(f) is the degree-10 polynomial of
+// gen_exp_poly.py and v_ldexp_f64 applies 2^k. One operation fewer than exp_fp64 (no two-part
+// ln2 reduction); max relative error 3.3e-16; exactly 0 for y < -1075.
+__device__ __forceinline__ double exp2_fp64(double y) {
+    const double shifter = 6755399441055744.0; // 1.5 * 2^52
+    const double kf = y + shifter;
+    const double k = kf - shifter;
+    const int ki = (int)(unsigned)__builtin_bit_cast(unsigned long long, kf);
+    const double f = y - k;
+    double p = HPH_EXP2_C10;
+    p = __builtin_fma(p, f, HPH_EXP2_C9);
+    p = __builtin_fma(p, f, HPH_EXP2_C8);
+    p = __builtin_fma(p, f, HPH_EXP2_C7);
+    p = __builtin_fma(p, f, HPH_EXP2_C6);
+    p = __builtin_fma(p, f, HPH_EXP2_C5);
+    p = __builtin_fma(p, f, HPH_EXP2_C4);
+    p = __builtin_fma(p, f, HPH_EXP2_C3);
+    p = __builtin_fma(p, f, HPH_EXP2_C2);
+    p = __builtin_fma(p, f, HPH_EXP2_C1);
+    p = __builtin_fma(p, f, HPH_EXP2_C0);
+    return __builtin_amdgcn_ldexp(p, ki);
+}
+
+// Scaled time of the base-2 pair form: t' = (t - t0) * st with st = |tauT| sqrt(log2(e) / 2), so
+// that the background exponent tauT^2 dt^2 / 2 becomes dt'^2 / log2(e) and 2^-(dt'^2) =
+// e^-(tauT^2 dt^2 / 2). Kernels and finalize compute st with this one expression.
+constexpr double kSqrtHalfLog2e = 0.84932180028801904272; // sqrt(log2(e) / 2)
+__device__ __forceinline__ double time_scale(double tauT) { return fabs(tauT) * kSqrtHalfLog2e; }
 
 // Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
 // constants are then pre-scaled by log2 e).
-template <bool F32, bool CLAMP>
+template <bool F32>
 __device__ __forceinline__ double background_exp(double cBx, double cBt, double r2, double dt2) {
     const double arg = __builtin_fma(cBt, dt2, cBx * r2);
     if (F32) return (double)exp2_f32((float)arg);
-    return exp_fp64_t<CLAMP>(arg);
+    return exp_fp64(arg);
 }
 
 // Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.

@@ -87,15 +87,17 @@ struct HostBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
-// Size bounds of a dataset's exponents: squared diagonal of the locations' bounding box and the
-// time span. With them the host bounds every pair's exp argument for given parameters.
+// Size bounds of a dataset's exponents (squared diagonal of the locations' bounding box, time
+// span) and whether its distinct times are far enough apart for the base-2 pair form. With them
+// the host picks the pass variant for given parameters.
 struct Extents {
     double r2max = 0.0, span = 0.0;
+    bool gaps_ok = true; // consecutive distinct times differ by at least 8 ulp(max |t|)
 };
 
-static Extents extents_of(const std::vector<double> &locs, const std::vector<double> &times, int D) {
+// locs: N x D in any order; sorted_t: the N times in ascending order.
+static Extents extents_of(const std::vector<double> &locs, const double *sorted_t, size_t n, int D) {
     Extents e;
-    const size_t n = times.size();
     if (n == 0) return e;
     for (int d = 0; d < D; ++d) {
         double lo = locs[d], hi = locs[d];
@@ -105,21 +107,34 @@ static Extents extents_of(const std::vector<double> &locs, const std::vector<dou
         }
         e.r2max += (hi - lo) * (hi - lo);
     }
-    const auto mm = std::minmax_element(times.begin(), times.end());
-    e.span = *mm.second - *mm.first;
+    e.span = sorted_t[n - 1] - sorted_t[0];
+    const double tabs = std::max(std::fabs(sorted_t[0]), std::fabs(sorted_t[n - 1]));
+    const double ulp = std::nextafter(tabs, HUGE_VAL) - tabs;
+    for (size_t i = 1; i < n && e.gaps_ok; ++i) {
+        const double g = sorted_t[i] - sorted_t[i - 1];
+        if (g > 0.0 && g < 8.0 * ulp) e.gaps_ok = false;
+    }
     return e;
 }
 
-// True unless every background (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt +
-// sigma^2 r^2/2) exponent of the K vectors is below 1e9 in size, where the clamp-free exp is exact.
-static bool needs_clamp(const Extents &e, const double *params, int K) {
+// True when the pass may use the base-2 form (scaled times, clamp-free exp2): every background
+// (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt + sigma^2 r^2/2) exponent of the K
+// vectors is below 1e9 in size, the time scale st = |tauT| sqrt(log2(e)/2) lies in [1e-100, 1e100]
+// with omega log2(e) / st finite, and distinct times are at least 8 ulp(max |t|) apart: then
+// (t - t0) st, rounded, keeps every strict order and every tie of the times, so the dt > 0 mask
+// of the excitation term is decided as on the original times.
+static bool fast_ok(const Extents &e, const double *params, int K) {
+    if (!e.gaps_ok) return false;
     for (int k = 0; k < K; ++k) {
         const double *p = params + 6 * (size_t)k;
         const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
-        if (!(aB < 1e9 && aS < 1e9)) return true;
+        if (!(aB < 1e9 && aS < 1e9)) return false;
+        const double st = std::fabs(p[2]) * 0.84932180028801904272;
+        if (!(st >= 1e-100 && st <= 1e100)) return false;
+        if (!std::isfinite(p[3] * 1.4426950408889634 / st)) return false;
     }
-    return false;
+    return true;
 }
 
 // Drives the symmetric pass over a set of device-resident datasets: plans work items, keeps the
@@ -141,7 +156,7 @@ class SymRunner {
     // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev: dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
-                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, bool clamp,
+                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, bool fast,
                  double *probs_dev, double *result, hipStream_t s, hipEvent_t ev0 = nullptr,
                  hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
         std::vector<int> n(refs.size());
@@ -211,7 +226,7 @@ class SymRunner {
         L.skip_zero = skip_zero;
         L.zero_arg_b = fp32 ? kern::kZeroExpArg32 : kern::kZeroExpArg;
         L.zero_arg_s = kern::kZeroExpArg;
-        L.clamp = clamp ? 1 : 0;
+        L.fast = fast ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
@@ -602,7 +617,7 @@ class HipHawkes : public AbstractHawkes {
         check(hipMemcpyAsync(d_t_.ptr, ht, sizeof(double) * npad_, hipMemcpyHostToDevice, stream_), "upload t");
         check(hipMemcpyAsync(d_perm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_), "upload perm");
         check(hipStreamSynchronize(stream_), "upload sync");
-        extents_ = extents_of(locs_, times_, D);
+        extents_ = extents_of(locs_, ht, (size_t)N, D);
         data_dirty_ = false;
     }
 
@@ -677,7 +692,7 @@ class HipHawkes : public AbstractHawkes {
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, needs_clamp(extents_, params, K), probs_dev,
+                         cfg.target_blocks, skip_zero, fast_ok(extents_, params, K), probs_dev,
                          d_result_.as<double>(), stream_,
                          timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
@@ -822,7 +837,6 @@ class HipHawkesBatch {
         check(hipSetDevice(device_), "hipSetDevice");
         locs_[d].assign(locs, locs + locs_len);
         times_[d].assign(times, times + times_len);
-        ext_[d] = extents_of(locs_[d], times_[d], D_);
         std::vector<int> perm(n);
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return times[a] < times[b]; });
@@ -835,6 +849,7 @@ class HipHawkesBatch {
             ht[s] = times[u];
             hp[s] = u;
         }
+        ext_[d] = extents_of(locs_[d], ht.data(), (size_t)n, D_);
         // x planes of dataset d live at d_x_ + D * off_[d]: plane q at + q * npad_[d]
         check(hipMemcpy(d_x_.as<double>() + (size_t)D_ * off_[d], hx.data(), sizeof(double) * hx.size(),
                         hipMemcpyHostToDevice),
@@ -893,15 +908,15 @@ class HipHawkesBatch {
         d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
-        const bool clamp = clamp_needed();
-        if (graph_exec_ && clamp != graph_clamp_) drop_graph(); // the graph holds one kernel variant
+        const bool fast = fast_all();
+        if (graph_exec_ && fast != graph_fast_) drop_graph(); // the graph holds one kernel variant
         if (graph_mode_) {
             if (!graph_exec_) {
-                enqueue(false, clamp); // plans, allocates and uploads the tables outside the capture
+                enqueue(false, fast); // plans, allocates and uploads the tables outside the capture
                 check(hipStreamSynchronize(stream_), "stream sync");
                 check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                enqueue(false, clamp);
-                graph_clamp_ = clamp;
+                enqueue(false, fast);
+                graph_fast_ = fast;
                 hipGraph_t g = nullptr;
                 check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
                 graph_ = g;
@@ -911,7 +926,7 @@ class HipHawkesBatch {
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
             check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         } else {
-            enqueue(true, clamp);
+            enqueue(true, fast);
         }
         check(hipStreamSynchronize(stream_), "stream sync");
         float a = 0.f;
@@ -928,17 +943,17 @@ class HipHawkesBatch {
     double lastMs() const { return last_ms_; }
 
   private:
-    // True when some exponent of the current parameters may exceed the clamp-free exp's range.
-    bool clamp_needed() const {
-        bool clamp = false;
+    // True when every dataset and vector of the current parameters allows the base-2 pass form.
+    bool fast_all() const {
+        bool fast = true;
         const double *hp = h_params_.as<double>();
         for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
-            clamp = clamp || needs_clamp(ext_[d], hp + 6 * k0, K_[d]);
-        return clamp;
+            fast = fast && fast_ok(ext_[d], hp + 6 * k0, K_[d]);
+        return fast;
     }
 
     // Parameter upload, pass, finalize, reduction and result download on the batch's stream.
-    void enqueue(bool events, bool clamp) {
+    void enqueue(bool events, bool fast) {
         check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)total_, hipMemcpyHostToDevice,
                              stream_),
               "upload params");
@@ -947,7 +962,7 @@ class HipHawkesBatch {
             refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
                                          d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
         const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
-        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, clamp, nullptr,
+        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, fast, nullptr,
                         d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
                         events ? ev_[2] : nullptr);
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,
@@ -974,7 +989,7 @@ class HipHawkesBatch {
     std::vector<bool> have_;
     std::vector<int> K_;
     int total_ = 0;
-    bool have_params_ = false, grad_ = true, graph_mode_ = false, graph_clamp_ = true;
+    bool have_params_ = false, grad_ = true, graph_mode_ = false, graph_fast_ = false;
     int A_ = 2, target_ = 8192;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

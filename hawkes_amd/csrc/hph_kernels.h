@@ -97,7 +97,8 @@ struct SymLaunch {
     int skip_zero;          // as PairGrid::skip_zero
     double zero_arg_b;      // as PairGrid::zero_arg_b
     double zero_arg_s;      // as PairGrid::zero_arg_s
-    int clamp;              // 0: every exp argument of the pass is known to be below 1e9 in size
+    int fast;               // 1: base-2 form (scaled times, clamp-free exp2); the host has checked
+                            // that exponents stay below 1e9 and scaled times keep their order
     const int4 *fin_item;   // (dataset, k, finalize block, 0)
     int fin_items;
     const int2 *res_item;   // (dataset, k)

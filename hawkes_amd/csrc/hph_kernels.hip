@@ -46,7 +46,7 @@ __device__ __forceinline__ void tile_pass(const double *__restrict__ tile, int c
         const double dt = ti - e[D];
         if (DOB) {
             const double dt2 = dt * dt;
-            const double eb = background_exp<F32, true>(cBx, cBt, r2, dt2);
+            const double eb = background_exp<F32>(cBx, cBt, r2, dt2);
             acc[0] += eb;
             if (GRAD) {
                 acc[1] = __builtin_fma(eb, r2, acc[1]);

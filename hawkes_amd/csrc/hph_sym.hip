@@ -33,15 +33,44 @@ __device__ __forceinline__ double lane_shift_down(double v, int byte_addr) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// Per-vector constants of the pair loops, in one of two forms.
+//  natural (FAST false): times as given; exponents in e-units with the clamped exp_fp64:
+//      background cB0 r^2 + cB1 dt^2 (cB0 = -tauX^2/2, cB1 = -tauT^2/2; times log2 e for the
+//      FP32-bulk v_exp_f32), excitation cS0 r^2 + cS1 dt (cS0 = -sigma^2/2, cS1 = -omega).
+//  base 2 (FAST true): times arrive as dt' = st dt (time_scale); exponents in 2-units with the
+//      clamp-free exp2_fp64: background -(cB0 r^2 + dt'^2) with cB0 = tauX^2 log2(e)/2, excitation
+//      cS0 r^2 + cS1 dt' with cS0 = -sigma^2 log2(e)/2 and cS1 = -omega log2(e)/st. The row sums
+//      of e dt^2 and e dt then carry st^2 and st, which the finalize kernel divides out.
+struct PairConst {
+    double cB0, cB1, cS0, cS1;
+};
+
+// Background term of one pair and its dt^2 (scaled by st^2 in the base-2 form).
+template <bool F32, bool FAST>
+__device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt, double &dt2) {
+    dt2 = dt * dt;
+    if (FAST) {
+        const double y = __builtin_fma(c.cB0, r2, dt2);
+        if (F32) return (double)exp2_f32((float)(-y));
+        return exp2_fp64(-y);
+    }
+    return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
+}
+
+template <bool FAST>
+__device__ __forceinline__ double excitation_term(const PairConst &c, double r2, double dt) {
+    const double y = __builtin_fma(c.cS0, r2, c.cS1 * dt);
+    return FAST ? exp2_fp64(y) : exp_fp64(y);
+}
+
 // One 64-column subtile against this lane's A rows, rotating the column accumulators ca.
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
 // has rows past the last event, whose column contributions are masked by vm; F32: FP32-bulk
-// background exponential (cBx, cBt then carry the log2 e scale).
-template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool CLAMP>
+// background exponential; FAST: the base-2 form of PairConst.
+template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
-                                            const double (&ti)[A], const double (&vm)[A], double cBx, double cBt,
-                                            double cSx, double nomega, double (&acc)[A][6],
-                                            double (&ca)[GRAD ? 3 : 1]) {
+                                            const double (&ti)[A], const double (&vm)[A], const PairConst &c,
+                                            double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
     constexpr int NC = GRAD ? 3 : 1;
     const int src = ((lane + 1) & 63) << 2;
 #pragma unroll 2
@@ -64,8 +93,8 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             }
             const double dt = ti[a] - q[D];
             if (DOB) {
-                const double dt2 = dt * dt;
-                const double eb = background_exp<F32, CLAMP>(cBx, cBt, r2, dt2);
+                double dt2;
+                const double eb = background_term<F32, FAST>(c, r2, dt, dt2);
                 acc[a][0] += eb;
                 if (GRAD) {
                     acc[a][1] = __builtin_fma(eb, r2, acc[a][1]);
@@ -79,7 +108,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                 }
             }
             if (SMODE != kSNone) {
-                double es = exp_fp64_t<CLAMP>(__builtin_fma(cSx, r2, nomega * dt));
+                double es = excitation_term<FAST>(c, r2, dt);
                 if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
                 acc[a][3] += es;
                 if (GRAD) {
@@ -97,9 +126,9 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
 // the dt > 0 mask.
-template <int D, bool GRAD, bool F32, bool CLAMP>
+template <int D, bool GRAD, bool F32, bool FAST>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
-                                          double cBx, double cBt, double cSx, double nomega, double (&acc)[6]) {
+                                          const PairConst &c, double (&acc)[6]) {
     constexpr int ST = TileStride<D>::value;
 #pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
@@ -115,14 +144,14 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
             r2 = __builtin_fma(dd, dd, r2);
         }
         const double dt = ti - e[D];
-        const double dt2 = dt * dt;
-        const double eb = background_exp<F32, CLAMP>(cBx, cBt, r2, dt2);
+        double dt2;
+        const double eb = background_term<F32, FAST>(c, r2, dt, dt2);
         acc[0] += eb;
         if (GRAD) {
             acc[1] = __builtin_fma(eb, r2, acc[1]);
             acc[2] = __builtin_fma(eb, dt2, acc[2]);
         }
-        double es = exp_fp64_t<CLAMP>(__builtin_fma(cSx, r2, nomega * dt));
+        double es = excitation_term<FAST>(c, r2, dt);
         es = dt > 0.0 ? es : 0.0;
         acc[3] += es;
         if (GRAD) {
@@ -132,33 +161,32 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     }
 }
 
-template <int D, bool GRAD, int A, bool TAIL, bool F32, bool CLAMP>
+template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
-                                             double cBx, double cBt, double cSx, double nomega, double (&acc)[A][6],
-                                             double (&ca)[GRAD ? 3 : 1]) {
+                                             const PairConst &c, double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, CLAMP>(sub, lane, xi, ti, vm, cBx, cBt, cSx, nomega, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     default: break; // every term of the subtile is exactly zero
     }
 }
 
 // Persistent pass: every workgroup takes work items from the queue until it is empty.
-template <int D, bool GRAD, int A, bool F32, bool CLAMP>
+template <int D, bool GRAD, int A, bool F32, bool FAST>
 __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const double *__restrict__ params,
                                                              double *__restrict__ rowpart,
                                                              double *__restrict__ colpart) {
@@ -184,15 +212,32 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
         const int P = it.y, chunk = it.z, k = it.w;
         const double *pk = params + 6 * (size_t)(ds.param0 + k);
         const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
-        const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
-        const double cSx = -0.5 * sigma * sigma, nomega = -omega;
-        // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
-        const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
         const double *__restrict__ ex = ds.x;
         const double *__restrict__ et = ds.t;
         const int n = ds.n, npad = ds.npad;
-
         const int P0 = P * R;
+
+        // base-2 form: times relative to the block's first event, scaled by st
+        const double st = FAST ? time_scale(tauT) : 1.0;
+        const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
+        PairConst c;
+        if (FAST) {
+            c.cB0 = tauX * tauX * (0.5 * kLog2e);
+            c.cB1 = 0.0;
+            c.cS0 = -sigma * sigma * (0.5 * kLog2e);
+            c.cS1 = -omega * kLog2e / st;
+        } else {
+            c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
+            c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);
+            c.cS0 = -0.5 * sigma * sigma;
+            c.cS1 = -omega;
+        }
+        // exact-zero tests: a tile is skipped when the smallest time gap alone drives the exponent
+        // past the underflow point (in the units of the form)
+        const double zb = FAST ? L.zero_arg_b * kLog2e : L.zero_arg_b;
+        const double zs = FAST ? L.zero_arg_s * kLog2e : L.zero_arg_s;
+        const double gb = FAST ? 1.0 : 0.5 * tauT * tauT; // background exponent per gap^2
+        const double gs = FAST ? -c.cS1 : omega;          // excitation exponent per gap
         const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
         double xi[A][D], ti[A], vm[A];
 #pragma unroll
@@ -201,11 +246,11 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
             const int rc = min(r, n - 1);
 #pragma unroll
             for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
-            ti[a] = et[rc];
+            ti[a] = FAST ? (et[rc] - t0) * st : et[rc];
             vm[a] = r < n ? 1.0 : 0.0;
         }
         const bool wave_live = wrow0 < n;
-        const double t_lo = et[min(wrow0, n - 1)];
+        const double t_lo = FAST ? (et[min(wrow0, n - 1)] - t0) * st : et[min(wrow0, n - 1)];
 
         double acc[A][6];
 #pragma unroll
@@ -223,13 +268,13 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                 if (tid < cnt) {
 #pragma unroll
                     for (int d = 0; d < D; ++d) tile[tid * ST + d] = ex[(size_t)d * npad + j0 + tid];
-                    tile[tid * ST + D] = et[j0 + tid];
+                    tile[tid * ST + D] = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
                 }
                 __syncthreads();
                 if (wave_live) {
 #pragma unroll
                     for (int a = 0; a < A; ++a)
-                        diag_pass<D, GRAD, F32, CLAMP>(tile, cnt, xi[a], ti[a], eBx, eBt, cSx, nomega, acc[a]);
+                        diag_pass<D, GRAD, F32, FAST>(tile, cnt, xi[a], ti[a], c, acc[a]);
                 }
             }
         } else {
@@ -251,7 +296,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                         tile[e0 + d * 128] = v;
                         tile[e0 + d * 128 + 64] = v;
                     }
-                    const double tv = et[j0 + tid];
+                    const double tv = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
                     tile[e0 + D * 128] = tv;
                     tile[e0 + D * 128 + 64] = tv;
                 }
@@ -263,8 +308,8 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                     int dob = 1;
                     if (L.skip_zero) {
                         const double gap = tl < t_lo ? t_lo - tl : 0.0;
-                        if (-cBt * gap * gap >= L.zero_arg_b) dob = 0;
-                        if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= L.zero_arg_s) smode = kSNone;
+                        if (gb * gap * gap >= zb) dob = 0;
+                        if (smode == kSFull && gs >= 0.0 && gs * gap >= zs) smode = kSNone;
                     }
                     const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                     for (int sb = 0; sb < 4; ++sb) {
@@ -273,11 +318,9 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_kernel(SymLaunch L, const 
                         for (int c = 0; c < NC; ++c) ca[c] = 0.0;
                         const double *sub = tile + sb * (D + 1) * 128;
                         if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32, CLAMP>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega, acc,
-                                                                ca);
+                            sym_dispatch<D, GRAD, A, true, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
                         else
-                            sym_dispatch<D, GRAD, A, false, F32, CLAMP>(mode, sub, lane, xi, ti, vm, eBx, eBt, cSx, nomega,
-                                                                 acc, ca);
+                            sym_dispatch<D, GRAD, A, false, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
 #pragma unroll
                         for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
                     }
@@ -370,6 +413,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
             s[3] = a6[3];
             s[4] = a6[4];
             s[5] = a6[5];
+            if (L.fast) { // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt
+                const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
+                s[2] /= st * st;
+                s[5] /= st;
+            }
         } else {
             s[0] = a6[0] + cs[0];
             s[3] = a6[NR - 1];
@@ -504,26 +552,26 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     return plan;
 }
 
-template <int D, bool GRAD, bool F32, bool CLAMP>
+template <int D, bool GRAD, bool F32, bool FAST>
 static void launch_sym_d(const SymLaunch &L, const double *params, double *rowpart, double *colpart, hipStream_t s) {
     if (L.A == 2)
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32, CLAMP>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32, FAST>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
     else
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32, CLAMP>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32, FAST>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
 }
 
-template <bool F32, bool CLAMP>
+template <bool F32, bool FAST>
 static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, double *rowpart, double *colpart,
                          hipStream_t s) {
     switch (L.dim * 2 + (grad ? 1 : 0)) {
-    case 2: launch_sym_d<1, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
-    case 3: launch_sym_d<1, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
-    case 4: launch_sym_d<2, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
-    case 5: launch_sym_d<2, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
-    case 6: launch_sym_d<3, false, F32, CLAMP>(L, params, rowpart, colpart, s); break;
-    case 7: launch_sym_d<3, true, F32, CLAMP>(L, params, rowpart, colpart, s); break;
+    case 2: launch_sym_d<1, false, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 3: launch_sym_d<1, true, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 4: launch_sym_d<2, false, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 5: launch_sym_d<2, true, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 6: launch_sym_d<3, false, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 7: launch_sym_d<3, true, F32, FAST>(L, params, rowpart, colpart, s); break;
     default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
 }
@@ -535,12 +583,12 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
     if (fp32) {
-        if (L.clamp)
+        if (L.fast)
             launch_sym_p<true, true>(L, params, grad, rowpart, colpart, s);
         else
             launch_sym_p<true, false>(L, params, grad, rowpart, colpart, s);
     } else {
-        if (L.clamp)
+        if (L.fast)
             launch_sym_p<false, true>(L, params, grad, rowpart, colpart, s);
         else
             launch_sym_p<false, false>(L, params, grad, rowpart, colpart, s);

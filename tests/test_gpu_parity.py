@@ -184,6 +184,24 @@ def test_huge_exponents_use_the_clamped_exp(kernel):
     assert_matches_oracle(locs, times, p, name="huge", kernel=kernel)
 
 
+@pytest.mark.parametrize("kernel", ["symmetric", "symmetric_a1", "rowwise"])
+def test_near_tied_times_keep_the_strict_mask(kernel):
+    """Pairs of events one ulp apart in time excite each other (dt > 0), exact ties do not. Scaled
+    times would merge the one-ulp pairs, so the engine must keep the original times for them; the
+    oracle decides the mask on the original times."""
+    rng = np.random.default_rng(78)
+    n = 5000
+    locs = rng.uniform(0, 4, (n, 2))
+    times = np.sort(rng.uniform(0, 8760.0, n))
+    for i in range(10, n - 1, 97):
+        times[i + 1] = np.nextafter(times[i], np.inf)
+    for i in range(50, n - 1, 211):
+        times[i + 1] = times[i]
+    times = np.sort(times)
+    assert np.sum(np.diff(times) == 0) > 0 and np.sum((np.diff(times) > 0) & (np.diff(times) < 1e-11)) > 0
+    assert_matches_oracle(locs, times, synth.P0, name="near-ties", kernel=kernel)
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
