@@ -33,13 +33,24 @@ def flop_model(n_background, n_excitation):
     return 35.0 * n_background + 28.0 * n_excitation
 
 
-def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256, zero_arg=746.0):
+ZERO_ARG = 746.0  # exp(-746) is exactly +0.0 in FP64
+
+
+def background_skip_arg(n):
+    """FP64 background skip point of the kernels: min(746, 50 + ln N) (terms below e^-(50 + ln N)
+    change every event's sums by < 5e-20 relative)."""
+    return min(ZERO_ARG, 50.0 + np.log(max(n, 1)))
+
+
+def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256):
     """Background and excitation pairs the pair kernel forms for sorted times: per (wavefront of
-    64 rows, tile of 256 columns), the tile is skipped where every term is exactly zero, the
-    excitation term is formed only for tiles not wholly after the rows (all pairs of the tile
-    in the masked diagonal case, as the kernel evaluates then masks)."""
+    64 rows, tile of 256 columns), the tile's background is skipped where every term is
+    negligible and its excitation where every term is exactly zero; the excitation term is
+    formed only for tiles not wholly after the rows (all pairs of the tile in the masked
+    diagonal case, as the kernel evaluates then masks)."""
     t = np.sort(np.asarray(times))
     n = t.size
+    zero_b = background_skip_arg(n)
     w0 = np.arange(0, n, rows_per_wave)
     w_lo, w_hi = t[w0], t[np.minimum(w0 + rows_per_wave - 1, n - 1)]
     rows = np.minimum(w0 + rows_per_wave, n) - w0
@@ -53,21 +64,23 @@ def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256, zero_arg=
     if skip_zero:
         gap = np.where(before, w_lo[:, None] - t_l[None, :],
                        np.where(t_f[None, :] > w_hi[:, None], t_f[None, :] - w_hi[:, None], 0.0))
-        b_on = 0.5 * params[2] ** 2 * gap * gap < zero_arg
+        b_on = 0.5 * params[2] ** 2 * gap * gap < zero_b
         if params[3] >= 0:
-            s_on &= ~(before & (params[3] * gap >= zero_arg))
+            s_on &= ~(before & (params[3] * gap >= ZERO_ARG))
     # the kernel runs whole 64-lane wavefronts; count lanes of real rows only
     pair_counts = rows[:, None] * cols[None, :]
     return float((pair_counts * b_on).sum()), float((pair_counts * s_on).sum())
 
 
-def pairs_formed_symmetric(times, params, skip_zero, waves=4, rows_per_lane=2, tile=256, zero_arg=746.0):
+def pairs_formed_symmetric(times, params, skip_zero, waves=4, rows_per_lane=2, tile=256):
     """Background and excitation pairs the symmetric kernel forms. Each background pair off the
     diagonal blocks is formed once for both events (the N(N+1)/2 of the cost model); per
-    (wavefront of 128 rows, earlier tile of 256 columns) a tile whose terms are all exactly zero
-    is skipped; the diagonal blocks (R x R, R = 512) are paired in both orders."""
+    (wavefront of 128 rows, earlier tile of 256 columns) the background is skipped where every
+    term is negligible and the excitation where every term is exactly zero; the diagonal blocks
+    (R x R, R = 512) are paired in both orders."""
     t = np.sort(np.asarray(times))
     n = t.size
+    zero_b = background_skip_arg(n)
     wrows = 64 * rows_per_lane
     R = waves * wrows
     nb = (n + R - 1) // R
@@ -85,9 +98,9 @@ def pairs_formed_symmetric(times, params, skip_zero, waves=4, rows_per_lane=2, t
     b_on = earlier.copy()
     s_on = earlier.copy()
     if skip_zero:
-        b_on &= 0.5 * params[2] ** 2 * gap * gap < zero_arg
+        b_on &= 0.5 * params[2] ** 2 * gap * gap < zero_b
         if params[3] >= 0:
-            s_on &= ~(before & (params[3] * gap >= zero_arg))
+            s_on &= ~(before & (params[3] * gap >= ZERO_ARG))
     nB = float((rows[:, None] * tile * b_on).sum())
     nS = float((rows[:, None] * tile * s_on).sum())
     for P in range(nb):

@@ -194,9 +194,12 @@ class SymRunner {
         for (size_t d = 0; refs_same && d < refs.size(); ++d)
             refs_same = refs_[d].x == refs[d].x && refs_[d].t == refs[d].t && refs_[d].perm == refs[d].perm &&
                         refs_[d].npad == refs[d].npad && refs_[d].t_last == refs[d].t_last;
+        for (size_t d = 0; refs_same && d < refs.size(); ++d)
+            refs_same = plan_.data[d].zero_arg_b == kern::background_skip_arg(refs[d].n, fp32);
         if (!refs_same) {
             for (size_t d = 0; d < refs.size(); ++d) {
                 kern::SymData &sd = plan_.data[d];
+                sd.zero_arg_b = kern::background_skip_arg(refs[d].n, fp32);
                 sd.x = refs[d].x;
                 sd.t = refs[d].t;
                 sd.perm = refs[d].perm;
@@ -224,7 +227,6 @@ class SymRunner {
         L.A = plan_.A;
         L.R = plan_.R;
         L.skip_zero = skip_zero;
-        L.zero_arg_b = fp32 ? kern::kZeroExpArg32 : kern::kZeroExpArg;
         L.zero_arg_s = kern::kZeroExpArg;
         L.fast = fast ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
@@ -698,7 +700,7 @@ class HipHawkes : public AbstractHawkes {
         } else {
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
-            g.zero_arg_b = fp32_ ? kern::kZeroExpArg32 : kern::kZeroExpArg;
+            g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
             g.zero_arg_s = kern::kZeroExpArg;
             const int nacc = grad ? 6 : 2;
             d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);

@@ -1,6 +1,8 @@
 // Launch interface between the engine (hph_engine.cpp) and the HIP kernels (hph_kernels.hip).
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -29,16 +31,29 @@ struct PairGrid {
     int row_blocks;    // ceil(n / kRowsPerBlock)
     int chunks;        // column chunks per row block
     int chunk_len;     // events per chunk (multiple of kTile)
-    int skip_zero;     // 1: skip tiles whose every term underflows to exactly +0.0
-    double zero_arg_b; // background exp arguments <= -zero_arg_b give exactly +0.0
+    int skip_zero;     // 1: skip tiles whose terms are all negligible (background) or zero (excitation)
+    double zero_arg_b; // background tiles whose every exponent is <= -zero_arg_b are skipped
     double zero_arg_s; // excitation exp arguments <= -zero_arg_s give exactly +0.0
 };
 
 // exp(x) is exactly +0.0 in FP64 for x <= -745.14 (both here and in the reference's exp);
-// a tile is skipped only when every one of its arguments is <= -kZeroExpArg.
+// an excitation tile is skipped only when every one of its arguments is <= -kZeroExpArg.
 constexpr double kZeroExpArg = 746.0;
 // FP32-bulk mode (background term): v_exp_f32 (2^y) is exactly +0.0 for y <= -150, i.e. x <= -103.98.
 constexpr double kZeroExpArg32 = 105.0;
+
+// Background tiles are skipped where every term is below e^-X, X = min(exact-zero point,
+// 50 + ln n). Bound: a skipped pair has b = e^-(a + c) with a = tauX^2 r^2/2 >= 0 and
+// c = tauT^2 dt^2/2 >= X, so b, b 2a and b 2c (the pair's terms in sum b, tauX^2 sum b r^2 and
+// tauT^2 sum b dt^2) are each at most 2X e^-X. Every event's own pair is in its sums (b = 1), so
+// the n pairs of one event change those sums by at most n 2X e^-X = 2X e^-50 < 5e-20 relative
+// to sum b (X <= 110): far below one FP64 rounding. The excitation term has no such floor and
+// is skipped only where it is exactly zero.
+inline double background_skip_arg(long long n, bool fp32) {
+    const double exact = fp32 ? kZeroExpArg32 : kZeroExpArg;
+    const double negl = 50.0 + std::log((double)(n > 1 ? n : 1));
+    return negl < exact ? negl : exact;
+}
 
 PairGrid plan_pair_grid(int n, int K, int target_blocks);
 
@@ -81,6 +96,7 @@ struct SymData {
     int param0;             // index of its first parameter vector
     int fin_blocks;         // ceil(n / 256)
     double t_last;          // the caller's last time (T of the model)
+    double zero_arg_b;      // background skip point of this dataset (background_skip_arg)
     long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
     long long col0, col_kstride; // column partials: the same
     long long blk0;         // index of its (k = 0, block 0) block partial
@@ -94,8 +110,7 @@ struct SymLaunch {
     int *counter;           // work-queue counter, zeroed by launch_sym before the pass
     int grid;               // persistent workgroups launched
     int dim, A, R;
-    int skip_zero;          // as PairGrid::skip_zero
-    double zero_arg_b;      // as PairGrid::zero_arg_b
+    int skip_zero;          // as PairGrid::skip_zero (background point: SymData::zero_arg_b)
     double zero_arg_s;      // as PairGrid::zero_arg_s
     int fast;               // 1: base-2 form (scaled times, clamp-free exp2); the host has checked
                             // that exponents stay below 1e9 and scaled times keep their order

@@ -117,8 +117,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
             if (g.skip_zero) {
                 // Smallest |dt| between this wavefront's rows and the tile (times are sorted).
                 const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
-                // Background: arg <= -tauT^2 dt^2 / 2; excitation (tile before the rows only):
-                // arg <= -omega dt. Terms whose argument is <= -kZeroExpArg are exactly +0.0.
+                // Background: arg <= -tauT^2 dt^2 / 2, skipped below the negligible point
+                // (background_skip_arg); excitation (tile before the rows only): arg <= -omega dt,
+                // skipped where every term is exactly +0.0.
                 if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
                 if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
             }
@@ -219,7 +220,7 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
     g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
     g.skip_zero = 1;
-    g.zero_arg_b = kZeroExpArg;
+    g.zero_arg_b = background_skip_arg(n, false);
     g.zero_arg_s = kZeroExpArg;
     return g;
 }
