@@ -9,7 +9,14 @@ vector each step (log-normal jitter, sd 0.1, around (1, 1/1.6, 1/336, 1, 1, 1)).
 resident in device memory before timing starts.
 
 Prints one JSON line. Options: --steps K --warmup W --gpus N (only 1 is supported)
---batch K64 (0 skips the batched leg) --cpu-seconds S (0 skips the CPU baseline).
+--batch K64 (0 skips the batched leg) --cpu-seconds S (0 skips the CPU baseline) --datasets D
+(0 skips the uneven dataset batch) --large 0|1 (the N ~ 1e6 leg) --legs (extra N ~ 1e5 legs)
+--config (a fixed pair-pass configuration instead of the autotuned one).
+
+value counts the N^2 ordered pairs the reference forms per call; the engine forms each
+background pair once for both events and skips tiles whose background terms are all below
+e^-(50 + ln N) or whose excitation terms are all exactly zero (roofline.pairs_formed gives the
+pairs actually formed, all_pairs the rate with no skipping).
 """
 import argparse
 import json
@@ -117,14 +124,16 @@ def kernel_name(cfg):
     return "pair_kernel<2,true> (target grid %d)" % target
 
 
-def load_traffic():
-    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary, if present."""
+def load_traffic(kind):
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary, if present and
+    measured on the pass that ran (kind: 'sym' or 'pair')."""
     path = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
     except (OSError, ValueError):
         return None
+    return rec.get("bytes_per_launch") if rec.get("kernel", "").startswith(kind + "_kernel") else None
 
 
 def cpu_baseline(locs, times, params, seconds):
@@ -289,7 +298,13 @@ def main():
     ap.add_argument("--workload", default="cluster_b70k")
     ap.add_argument("--datasets", type=int, default=32, help="uneven dataset batch size (0 skips the leg)")
     ap.add_argument("--large", type=int, default=1, help="1: N ~ 1e6 FP32-bulk + HMC leg, 0: skip")
+    ap.add_argument("--legs", default="all_pairs,fp32,small",
+                    help="comma list of the extra N ~ 1e5 legs to run (all_pairs, fp32, small) or 'none'")
+    ap.add_argument("--config", default="auto",
+                    help="pair-pass configuration: 'auto' (autotuned) or kind,rows_per_lane,target_grid with "
+                         "kind sym or rowwise, e.g. sym,2,8192 (no autotuning; for profiling one kernel)")
     args = ap.parse_args()
+    legs = set() if args.legs == "none" else set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -307,8 +322,13 @@ def main():
         e.set_times(times)
         e.set_parameters(p0)
         e.set_timing(True)
+        if args.config != "auto":
+            kind, rows_per_lane, target = args.config.split(",")
+            e.set_pair_config(_lib.PAIR_KERNEL_SYMMETRIC if kind == "sym" else _lib.PAIR_KERNEL_ROWWISE,
+                              int(rows_per_lane), int(target))
         e.log_likelihood_and_gradient()  # uploads the data; first launch
-        e.autotune(repeats=2)            # picks the pair-pass configuration for this shape
+        if args.config == "auto":
+            e.autotune(repeats=2)        # picks the pair-pass configuration for this shape
         for s in range(args.warmup):
             e.set_parameters(params[s])
             e.log_likelihood_and_gradient()
@@ -345,7 +365,7 @@ def main():
         "peak": FP64_VECTOR_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-        "traffic": load_traffic(),
+        "traffic": load_traffic("sym" if e.tuned_config()[0] == _lib.PAIR_KERNEL_SYMMETRIC else "pair"),
         "kernel": kernel_name(e.tuned_config()),
         "kernel_ms": pair_avg_ms,
         "finalize_reduce_ms": float(np.mean(fin_ms)),
@@ -354,32 +374,37 @@ def main():
                          "excitation": float(np.mean([x for _, x in formed]))},
     }
 
-    # Every pair formed (no exact-zero tile skipping): same results bitwise, more work.
-    ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4))
-    steps_a = len(pair_a)
-    flop_a = flop_model(*formed_a[0])
-    all_pairs = {
-        "value": n * n / (wall_a / steps_a),
-        "ms_per_step": 1e3 * wall_a / steps_a,
-        "kernel_ms": float(np.mean(pair_a)),
-        "roofline_frac": flop_a / (float(np.mean(pair_a)) * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
-    }
-    ea.close()
+    # Every pair formed (no tile skipping): the same results to rounding, more work.
+    all_pairs = None
+    if "all_pairs" in legs:
+        ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4))
+        steps_a = len(pair_a)
+        flop_a = flop_model(*formed_a[0])
+        all_pairs = {
+            "value": n * n / (wall_a / steps_a),
+            "ms_per_step": 1e3 * wall_a / steps_a,
+            "kernel_ms": float(np.mean(pair_a)),
+            "roofline_frac": flop_a / (float(np.mean(pair_a)) * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
+        }
+        ea.close()
 
-    # FP32-bulk mode: FP32 pair arithmetic, FP64 accumulation; error against the FP64 results above.
-    ef, wall_f, pair_f, _, _ = run(_lib.OPENCL | _lib.FLOAT, args.steps)
-    ll_err = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(run.results, fp64_results))
-    g_err = max(float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-3 * np.max(np.abs(b[1])))))
-                for a, b in zip(run.results, fp64_results))
-    fp32 = {
-        "value": n * n / (wall_f / args.steps),
-        "ms_per_step": 1e3 * wall_f / args.steps,
-        "kernel_ms": float(np.mean(pair_f)),
-        "kernel": kernel_name(ef.tuned_config()),
-        "rel_err_ll_vs_fp64": ll_err,
-        "rel_err_grad_vs_fp64": g_err,
-    }
-    ef.close()
+    # FP32-bulk mode: the background exponential in FP32, everything else FP64; error against the
+    # FP64 results above.
+    fp32 = None
+    if "fp32" in legs:
+        ef, wall_f, pair_f, _, _ = run(_lib.OPENCL | _lib.FLOAT, args.steps)
+        ll_err = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(run.results, fp64_results))
+        g_err = max(float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-3 * np.max(np.abs(b[1])))))
+                    for a, b in zip(run.results, fp64_results))
+        fp32 = {
+            "value": n * n / (wall_f / args.steps),
+            "ms_per_step": 1e3 * wall_f / args.steps,
+            "kernel_ms": float(np.mean(pair_f)),
+            "kernel": kernel_name(ef.tuned_config()),
+            "rel_err_ll_vs_fp64": ll_err,
+            "rel_err_grad_vs_fp64": g_err,
+        }
+        ef.close()
 
     batched = None
     if args.batch > 0:
@@ -397,7 +422,7 @@ def main():
 
     datasets = dataset_batch(args.datasets) if args.datasets > 0 else None
     large = large_dataset() if args.large else None
-    small = small_n()
+    small = small_n() if "small" in legs else None
 
     cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
