@@ -5,8 +5,8 @@ include/hph.h. This package holds its Python mirror (``engine``), the synthetic 
 generators (``synth``) and the torch autograd wrapper (``autograd``).
 """
 from . import _lib  # noqa: F401
-from .engine import (Engine, HphError, createEngine, getLogLikeGradient, getLogLikelihood,  # noqa: F401
-                     getProbsSelfExcite, setParameters, setTimesData, updateLocations)
+from .engine import (Engine, HphError, createEngine, getLocationGradient, getLogLikeGradient,  # noqa: F401
+                     getLogLikelihood, getProbsSelfExcite, setParameters, setTimesData, updateLocations)
 
 PARAMETER_NAMES = ("sigmaXprec", "tauXprec", "tauTprec", "omega", "theta", "mu0")
 P0 = (1.0, 1.0 / 1.6, 1.0 / 336.0, 1.0, 1.0, 1.0)

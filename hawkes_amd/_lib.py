@@ -42,6 +42,7 @@ SIGNATURES = {
     "hph_get_log_like_gradient": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_get_log_likelihood_and_gradient": (ctypes.c_int, [_vp, _dp, _dp]),
     "hph_get_probs_self_excite": (ctypes.c_int, [_vp, _dp, _sz]),
+    "hph_get_location_gradient": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_store_state": (ctypes.c_int, [_vp]),
     "hph_restore_state": (ctypes.c_int, [_vp]),
     "hph_accept_state": (ctypes.c_int, [_vp]),

@@ -350,7 +350,8 @@ class HipHawkes : public AbstractHawkes {
     ~HipHawkes() override {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
-        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_})
+        for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_w_,
+                          &d_locgrad_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -438,6 +439,42 @@ class HipHawkes : public AbstractHawkes {
             return;
         }
         evaluate_single(false, result);
+    }
+
+    // Rate pass (row-wise, no gradient) -> w = 1/lambda, then the location-gradient pass; the
+    // result is downloaded in the caller's event order.
+    void getLocationGradient(double *out, size_t length) override {
+        if (out == nullptr) throw std::invalid_argument("location gradient pointer is null");
+        const int N = locationCount, D = embeddingDimension;
+        if (length != (size_t)N * D) throw std::invalid_argument("location gradient length must be N*D");
+        require_ready();
+        check(hipSetDevice(device_), "hipSetDevice");
+        if (data_dirty_) upload();
+        d_params_.ensure(sizeof(double) * 6);
+        h_params_.ensure(sizeof(double) * 6);
+        std::memcpy(h_params_.ptr, params_, sizeof(params_));
+        check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6, hipMemcpyHostToDevice, stream_),
+              "upload params");
+        const kern::Events ev = events();
+        kern::PairGrid g = kern::plan_pair_grid(N, 1, 4096);
+        g.skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
+        d_part_.ensure(sizeof(double) * (size_t)g.chunks * std::max(2, D) * npad_);
+        d_w_.ensure(sizeof(double) * npad_);
+        d_locgrad_.ensure(sizeof(double) * (size_t)N * D);
+        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        kern::launch_pair(ev, d_params_.as<double>(), 1, g, false, false, d_part_.as<double>(), stream_);
+        check(hipGetLastError(), "rate pass launch");
+        kern::launch_inv_rate(ev, d_params_.as<double>(), g, d_part_.as<double>(), d_w_.as<double>(), stream_);
+        check(hipGetLastError(), "inverse rate launch");
+        if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+        kern::launch_locgrad(ev, d_params_.as<double>(), d_w_.as<double>(), g, d_part_.as<double>(),
+                             d_locgrad_.as<double>(), stream_);
+        check(hipGetLastError(), "location gradient launch");
+        if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
+        check(hipMemcpyAsync(out, d_locgrad_.ptr, sizeof(double) * (size_t)N * D, hipMemcpyDeviceToHost, stream_),
+              "download location gradient");
+        check(hipStreamSynchronize(stream_), "location gradient sync");
+        if (timing_) record_timing();
     }
 
     // Snapshot / restore of parameters, locations and the last log-likelihood: the intent of
@@ -723,13 +760,16 @@ class HipHawkes : public AbstractHawkes {
                                  stream_),
                   "download probs");
         check(hipStreamSynchronize(stream_), "stream sync");
-        if (timing_) {
-            float a = 0.f, b = 0.f;
-            check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "hipEventElapsedTime");
-            check(hipEventElapsedTime(&b, ev_[1], ev_[2]), "hipEventElapsedTime");
-            last_ms_[0] = a;
-            last_ms_[1] = b;
-        }
+        if (timing_) record_timing();
+    }
+
+    // Device times of the last evaluation: [0] = ev_[0] .. ev_[1], [1] = ev_[1] .. ev_[2].
+    void record_timing() {
+        float a = 0.f, b = 0.f;
+        check(hipEventElapsedTime(&a, ev_[0], ev_[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&b, ev_[1], ev_[2]), "hipEventElapsedTime");
+        last_ms_[0] = a;
+        last_ms_[1] = b;
     }
 
     void evaluate_single(bool grad, double *probs_host) {
@@ -775,7 +815,7 @@ class HipHawkes : public AbstractHawkes {
     hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
     double last_ms_[2] = {0.0, 0.0};
 
-    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_;
+    DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_, d_w_, d_locgrad_;
     HostBuf h_stage_, h_result_, h_params_;
 };
 
@@ -1116,6 +1156,11 @@ int hph_get_log_likelihood_and_gradient(hph_engine *e, double *loglik, double *g
 int hph_get_probs_self_excite(hph_engine *e, double *out, size_t length) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->getProbsSelfExcite(out, length); });
+}
+
+int hph_get_location_gradient(hph_engine *e, double *out, size_t length) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->getLocationGradient(out, length); });
 }
 
 int hph_store_state(hph_engine *e) {

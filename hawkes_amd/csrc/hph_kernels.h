@@ -76,6 +76,15 @@ void launch_finalize(const Events &ev, const double *params, int K, const PairGr
 void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,
                    hipStream_t s);
 
+// ---- Location gradient (hph_locgrad.hip) ----
+// w[i] = 1 / lambda_i (sorted order, npad entries) from a rate pass (launch_pair without
+// gradient, K = 1, grid g) whose partials are rate_part.
+void launch_inv_rate(const Events &ev, const double *params, const PairGrid &g, const double *rate_part, double *w,
+                     hipStream_t s);
+// d ll / d x into out (N x D, caller's order); part: chunk partials, g.chunks x D x npad doubles.
+void launch_locgrad(const Events &ev, const double *params, const double *w, const PairGrid &g, double *part,
+                    double *out, hipStream_t s);
+
 // ---- Symmetric pass (hph_sym.hip): background exps once per unordered pair ----
 //
 // The pass runs over a table of datasets (one for an engine, many for a batch); every dataset

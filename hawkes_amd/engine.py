@@ -90,6 +90,12 @@ class Engine:
         check(_lib.lib().hph_get_probs_self_excite(self._h, dptr(p), p.size))
         return p
 
+    def location_gradient(self):
+        """d loglik / d locations as an N x D array in the caller's event order."""
+        g = np.zeros((self.locationCount, self.embeddingDimension))
+        check(_lib.lib().hph_get_location_gradient(self._h, dptr(g), g.size))
+        return g
+
     def set_parameters_batch(self, params):
         a = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
         check(_lib.lib().hph_set_parameters_batch(self._h, dptr(a), a.size))
@@ -214,3 +220,10 @@ def getProbsSelfExcite(engine):
     """R/hph.R:41-57."""
     _require_init(engine)
     return engine.probs_self_excite()
+
+
+def getLocationGradient(engine):
+    """d loglik / d locations (N x D); the reference declares it for its Java callers
+    (src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96) without an implementation."""
+    _require_init(engine)
+    return engine.location_gradient()

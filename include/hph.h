@@ -84,6 +84,13 @@ int hph_get_log_likelihood_and_gradient(hph_engine *engine, double *loglik, doub
  * length must equal location_count; result in the caller's event order. */
 int hph_get_probs_self_excite(hph_engine *engine, double *out, size_t length);
 
+/* d loglik / d locations: out is N x D row-major in the caller's event order, length = N * D.
+ * No counterpart in the reference's engine: its Java binding declares getLocationGradient
+ * (src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96) and leaves it unimplemented
+ * (src/jni/dr_inference_hawkes_NativeHPHSingleton.cpp:80-89). FP64 in every precision mode.
+ * With timing on, hph_get_timing reports the rate pass and the gradient pass. */
+int hph_get_location_gradient(hph_engine *engine, double *out, size_t length);
+
 /* Replace storeState / restoreState / acceptState (src/AbstractHawkes.hpp:46-48): snapshot of
  * parameters, locations and the last log-likelihood; restore brings the snapshot back. */
 int hph_store_state(hph_engine *engine);

@@ -8,6 +8,9 @@ this package, as the checker or as the timed CPU baseline. The product package
 
 * ``hawkes_oracle_rows`` / ``hawkes_oracle_total``: the long-double (or double) restatement of
   src/NewHawkes.hpp:520-547, 636-660, 662-719 with compensated sums and the exact gradient.
+* ``hawkes_oracle_location_gradient``: d ll / d x (N x D), the exact derivative of the same
+  model (the reference declares but never implements a location gradient,
+  src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96).
 * ``hawkes_cpu_baseline``: the reference's serial, AVX and threaded CPU flavours
   (src/NewHawkes.hpp:899-980), timed as the CPU baseline.
 """
@@ -42,6 +45,9 @@ def lib():
         L.hawkes_cpu_baseline.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                           dp, dp, dp, ctypes.c_long, ctypes.c_long, dp]
         L.hawkes_cpu_baseline.restype = ctypes.c_int
+        L.hawkes_oracle_location_gradient.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, dp, dp, dp,
+                                                      dp, dp]
+        L.hawkes_oracle_location_gradient.restype = ctypes.c_int
         L.hawkes_oracle_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -95,6 +101,20 @@ def rows(locations, times, params, row_begin=0, row_end=None, precision="long"):
     if rc != 0:
         raise RuntimeError("oracle failed")
     return out
+
+
+def location_gradient(locations, times, params, precision="long"):
+    """Returns (grad, scale): d ll / d x as an N x D array in the caller's order, and per
+    component the sum of |terms| (the magnitude against which a cancelling sum states its error)."""
+    loc, t, p = _prep(locations, times, params)
+    N, D = loc.shape
+    g = np.zeros((N, D))
+    sc = np.zeros((N, D))
+    rc = lib().hawkes_oracle_location_gradient(0 if precision == "long" else 1, D, N, _p(loc), _p(t), _p(p),
+                                               _p(g), _p(sc))
+    if rc != 0:
+        raise RuntimeError("oracle failed")
+    return g, sc
 
 
 def cpu_baseline(flavour, locations, times, params, row_begin=0, row_end=None, grad=True):

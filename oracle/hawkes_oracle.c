@@ -95,6 +95,14 @@ int hawkes_oracle_total(int precision, int D, long N, const double *locs, const 
     return oracle_total_d(D, N, locs, times, params, ll, grad6, scale7, probs);
 }
 
+/* Location gradient d ll / d x (N x D, caller's order) and its magnitude scale (may be NULL). */
+int hawkes_oracle_location_gradient(int precision, int D, long N, const double *locs, const double *times,
+                                    const double *params, double *grad, double *scale) {
+    if (D < 1 || D > 8 || N < 1) return -1;
+    if (precision == 0) return oracle_locgrad_ld(D, N, locs, times, params, grad, scale);
+    return oracle_locgrad_d(D, N, locs, times, params, grad, scale);
+}
+
 int hawkes_oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

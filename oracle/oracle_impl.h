@@ -166,6 +166,72 @@ static int FN(oracle_total)(int D, long N, const double *locs, const double *tim
     return 0;
 }
 
+/*
+ * Location gradient out[k*D + d] = d ll / d x_kd. The reference has none (the JNI header
+ * declares getLocationGradient, src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96, and
+ * the .cpp leaves it commented out, :80-89); this is the exact derivative of ll above. The
+ * integral terms do not depend on locations, and with w_i = 1/lambda_i,
+ * d b_ij/d x_i = -tauX^2 (x_i - x_j) b_ij, d s_ij/d x_i = -sigma^2 (x_i - x_j) s_ij:
+ *   d ll/d x_k = - sum_j (x_k - x_j) [cB tauX^2 b_kj (w_k + w_j) + cS sigma^2 (s_kj w_k + s_jk w_j)].
+ * scale_out (may be NULL) receives sum_j |terms| per component (the magnitude against which
+ * errors of a cancelling sum are stated).
+ */
+static int FN(oracle_locgrad)(int D, long N, const double *locs, const double *times, const double *params,
+                              double *out, double *scale_out) {
+    const REAL sigma = params[0], tauX = params[1], tauT = params[2];
+    const REAL omega = params[3], theta = params[4], mu0 = params[5];
+    const REAL cB = mu0 * POW(tauX, (REAL)D) * tauT;
+    const REAL cS = POW(sigma, (REAL)D) * theta * omega;
+    REAL *w = (REAL *)malloc(sizeof(REAL) * (size_t)N);
+    if (!w) return -1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < N; ++i) {
+        FN(nsum_t) sb = {0, 0}, ss = {0, 0};
+        const REAL ti = times[i];
+        for (long j = 0; j < N; ++j) {
+            REAL r2 = 0;
+            for (int d = 0; d < D; ++d) {
+                const REAL dx = (REAL)locs[i * D + d] - (REAL)locs[j * D + d];
+                r2 += dx * dx;
+            }
+            const REAL r = SQRT(r2);
+            const REAL dt = ti - (REAL)times[j];
+            FN(nsum_add)(&sb, FN(pdf)(tauX * r) * FN(pdf)(tauT * dt));
+            if (dt > 0) FN(nsum_add)(&ss, EXP(-omega * dt) * FN(pdf)(sigma * r));
+        }
+        w[i] = 1 / (cB * FN(nsum_get)(&sb) + cS * FN(nsum_get)(&ss));
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long k = 0; k < N; ++k) {
+        FN(nsum_t) g[8], a[8];
+        for (int d = 0; d < D; ++d) g[d].s = g[d].c = a[d].s = a[d].c = 0;
+        const REAL tk = times[k];
+        for (long j = 0; j < N; ++j) {
+            REAL dx[8], r2 = 0;
+            for (int d = 0; d < D; ++d) {
+                dx[d] = (REAL)locs[k * D + d] - (REAL)locs[j * D + d];
+                r2 += dx[d] * dx[d];
+            }
+            const REAL r = SQRT(r2);
+            const REAL dt = tk - (REAL)times[j];
+            const REAL b = FN(pdf)(tauX * r) * FN(pdf)(tauT * dt);
+            REAL f = cB * tauX * tauX * b * (w[k] + w[j]);
+            if (dt > 0) f += cS * sigma * sigma * EXP(-omega * dt) * FN(pdf)(sigma * r) * w[k];
+            if (dt < 0) f += cS * sigma * sigma * EXP(omega * dt) * FN(pdf)(sigma * r) * w[j];
+            for (int d = 0; d < D; ++d) {
+                FN(nsum_add)(&g[d], -f * dx[d]);
+                FN(nsum_add)(&a[d], FABS(f * dx[d]));
+            }
+        }
+        for (int d = 0; d < D; ++d) {
+            out[k * D + d] = (double)FN(nsum_get)(&g[d]);
+            if (scale_out) scale_out[k * D + d] = (double)FN(nsum_get)(&a[d]);
+        }
+    }
+    free(w);
+    return 0;
+}
+
 #undef FN
 #undef CAT
 #undef CAT_

@@ -14,7 +14,8 @@ dmvnorm/dnorm/pnorm definitions), restated here from the formulas:
   prob_se(i)   = se_rate(i) / (bg_rate(i) + se_rate(i))                         (:51-55)
 
 The gradient with respect to the reference's parameters (sigmaXprec, tauXprec, tauTprec,
-omega, theta, mu0) is mpmath.diff of log_lik at 50 digits. No reference code is run.
+omega, theta, mu0) is mpmath.diff of log_lik at 50 digits, and so is the location gradient
+d log_lik / d x_kd ("loc_grad", N x D). No reference code is run.
 
 Usage: python tests/golden/make_golden.py   (rewrites the fixtures; deterministic)
 """
@@ -84,6 +85,20 @@ def gradient(locs, times, prec):
     return g
 
 
+def location_gradient(locs, times, prec):
+    out = []
+    for i in range(len(locs)):
+        row = []
+        for d in range(len(locs[i])):
+            def f(v, i=i, d=d):
+                L = [[mp.mpf(a) for a in r] for r in locs]
+                L[i][d] = v
+                return log_lik(L, times, prec)
+            row.append(mp.diff(f, mp.mpf(locs[i][d])))
+        out.append(row)
+    return out
+
+
 def case(name, locs, times, params, note):
     locs = np.asarray(locs, dtype=np.float64)
     if locs.ndim == 1:
@@ -92,6 +107,7 @@ def case(name, locs, times, params, note):
     prec = [mp.mpf(float(p)) for p in params]
     ll, rows, probs = log_lik(locs.tolist(), times.tolist(), prec, per_row=True)
     g = gradient(locs.tolist(), times.tolist(), prec)
+    lg = location_gradient(locs.tolist(), times.tolist(), prec)
     return {
         "name": name,
         "note": note,
@@ -104,6 +120,7 @@ def case(name, locs, times, params, note):
         "ll_rows": [float(r) for r in rows],
         "grad": [float(x) for x in g],
         "probs": [float(p) for p in probs],
+        "loc_grad": [[float(v) for v in row] for row in lg],
         "ll_str": mp.nstr(ll, 30),
     }
 

@@ -8,7 +8,9 @@ Each oracle function is checked against something a plausible mistake would fail
   * gradient: central finite differences, and the R analytic gradients for mu0, tau_x,
     tau_t (R/log_lik_and_grad.R:176-247) via the chain rule d/dprec = -prec^-2 d/dsd,
     theta and h at omega = 1 (where the R formulas are valid);
-  * the CPU baseline flavours (serial, AVX, threaded) against the oracle.
+  * the CPU baseline flavours (serial, AVX, threaded) against the oracle;
+  * location gradient: mpmath.diff golden vectors, translation and rotation invariance,
+    central finite differences.
 """
 import math
 
@@ -153,6 +155,40 @@ def test_gradient_central_differences():
         dn[k] -= h
         fd = (oracle.total(x, t, up)["ll"] - oracle.total(x, t, dn)["ll"]) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+def test_golden_location_gradient(golden):
+    """d ll / d x against mpmath.diff of the R statement at 50 digits, both oracle precisions."""
+    for c in golden:
+        gold = np.array(c["loc_grad"])
+        for precision, tol in (("long", 1e-14), ("double", 1e-12)):
+            g, sc = oracle.location_gradient(c["locations"], c["times"], c["params"], precision=precision)
+            err = np.abs(g - gold) / np.maximum(np.maximum(np.abs(gold), 1e-3 * sc), 1e-300)
+            assert err.max() <= tol, (c["name"], precision, err.max())
+
+
+def test_location_gradient_invariants():
+    """Translation invariance makes the location gradients sum to zero over events; rotation
+    invariance (D = 2) makes sum_k (x_k g_ky - y_k g_kx) zero; a mistake in the pair weights
+    (w_k + w_j, or the direction of the excitation term) breaks both."""
+    x, t, p = _random_data(n=60)
+    g, sc = oracle.location_gradient(x, t, p)
+    assert np.all(np.abs(g.sum(axis=0)) <= 1e-13 * sc.sum(axis=0))
+    torque = np.sum(x[:, 0] * g[:, 1] - x[:, 1] * g[:, 0])
+    assert abs(torque) <= 1e-12 * np.sum(np.abs(x) * sc)
+
+
+def test_location_gradient_central_differences():
+    x, t, p = _random_data(n=25, seed=12)
+    g, _ = oracle.location_gradient(x, t, p)
+    for k in (0, 7, 24):
+        for d in range(2):
+            h = 1e-6
+            up, dn = x.copy(), x.copy()
+            up[k, d] += h
+            dn[k, d] -= h
+            fd = (oracle.total(up, t, p)["ll"] - oracle.total(dn, t, p)["ll"]) / (2 * h)
+            assert abs(fd - g[k, d]) <= 1e-6 * max(1.0, abs(g[k, d])), (k, d, fd, g[k, d])
 
 
 def _r_rates(x, t, sd):
