@@ -298,8 +298,8 @@ def main():
     ap.add_argument("--workload", default="cluster_b70k")
     ap.add_argument("--datasets", type=int, default=32, help="uneven dataset batch size (0 skips the leg)")
     ap.add_argument("--large", type=int, default=1, help="1: N ~ 1e6 FP32-bulk + HMC leg, 0: skip")
-    ap.add_argument("--legs", default="all_pairs,fp32,small",
-                    help="comma list of the extra N ~ 1e5 legs to run (all_pairs, fp32, small) or 'none'")
+    ap.add_argument("--legs", default="all_pairs,fp32,small,locgrad",
+                    help="comma list of the extra legs to run (all_pairs, fp32, small, locgrad) or 'none'")
     ap.add_argument("--config", default="auto",
                     help="pair-pass configuration: 'auto' (autotuned) or kind,rows_per_lane,target_grid with "
                          "kind sym or rowwise, e.g. sym,2,8192 (no autotuning; for profiling one kernel)")
@@ -406,6 +406,29 @@ def main():
         }
         ef.close()
 
+    # Location gradient d ll / d x (N x D): after a parameter change it runs a natural-form rate
+    # pass (1/lambda) and the gradient pass ("cold"); at unchanged state the rates are reused.
+    locgrad = None
+    if "locgrad" in legs:
+        e.set_parameters(p0)
+        e.location_gradient()
+        reps = 3
+        cold, cold_ms = [], []
+        for r in range(reps):
+            e.set_parameters(params[r])
+            t0 = time.perf_counter()
+            e.location_gradient()
+            cold.append(time.perf_counter() - t0)
+            cold_ms.append(e.timing())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e.location_gradient()
+        warm = (time.perf_counter() - t0) / reps
+        lt = float(np.mean(cold))
+        locgrad = {"ms_per_call": 1e3 * lt, "rate_pass_ms": float(np.mean([a for a, _ in cold_ms])),
+                   "gradient_pass_ms": float(np.mean([b for _, b in cold_ms])),
+                   "ms_per_call_rates_reused": 1e3 * warm, "pair_evals_per_s": n * n / lt}
+
     batched = None
     if args.batch > 0:
         P = synth.jitter_params(args.batch, seed=668)
@@ -449,6 +472,7 @@ def main():
         "dataset_batch": datasets,
         "large_n1e6": large,
         "small_n1000": small,
+        "location_gradient": locgrad,
     }
     print(json.dumps(out))
 

@@ -87,6 +87,22 @@ template <int D> struct TileStride {
     static constexpr int value = (D + 2) & ~1;
 };
 
+// Stride of one event in a location-gradient tile: D coordinates, t, w = 1/lambda, padded.
+template <int D> struct LocStride {
+    static constexpr int value = (D + 3) & ~1;
+};
+
+// Wavefronts per workgroup of the symmetric passes.
+constexpr int kSymWaves = 4;
+
+// Value of lane (byte_addr / 4) of the wavefront (ds_bpermute through the LDS crossbar).
+__device__ __forceinline__ double lane_shift_down(double v, int byte_addr) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)u);
+    const int hi = __builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // ---- double-double helpers (Knuth TwoSum) ----
 __device__ __forceinline__ double2 dd_add(double2 a, double2 b) {
     const double s = a.x + b.x;
@@ -122,11 +138,11 @@ __device__ double2 block_reduce_dd(double2 v, double2 *sh) {
 // s holds (sum eB, sum eB r^2, sum eB dt^2, sum eS, sum eS r^2, sum eS dt) with
 // eB = exp(-(tauX^2 r^2 + tauT^2 dt^2)/2) = 2 pi pdf(tauX r) pdf(tauT dt) and
 // eS = exp(-omega dt - sigma^2 r^2 / 2) = sqrt(2 pi) e^{-omega dt} pdf(sigma r). Without GRAD only
-// s[0] and s[3] are used. v receives ll_i and d ll_i / d(sigma, tauX, tauT, omega, theta, mu0);
-// returns the self-excitation probability p_i.
+// s[0] and s[3] are used. v receives ll_i and d ll_i / d(sigma, tauX, tauT, omega, theta, mu0),
+// lam the rate lambda_i; returns the self-excitation probability p_i.
 template <bool GRAD>
 __device__ __forceinline__ double row_terms(const double (&s)[6], const double *__restrict__ pk, int D, double ti,
-                                            double t_last, double (&v)[7]) {
+                                            double t_last, double (&v)[7], double &lam_out) {
         const double sB = s[0], sS = s[3];
         const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3], theta = pk[4], mu0 = pk[5];
         const double inv_2pi = 0.15915494309189533577;
@@ -135,6 +151,7 @@ __device__ __forceinline__ double row_terms(const double (&s)[6], const double *
         const double bg = mu0 * tauXD * tauT * inv_2pi * sB;
         const double se = theta * omega * sigmaD * inv_sqrt_2pi * sS;
         const double lam = bg + se;
+        lam_out = lam;
         const double dT = t_last - ti;
         const double eT = exp(-omega * dT);
         const double phiA = 0.5 * erfc(-tauT * dT * M_SQRT1_2);

@@ -92,7 +92,8 @@ struct HostBuf {
 // the host picks the pass variant for given parameters.
 struct Extents {
     double r2max = 0.0, span = 0.0;
-    bool gaps_ok = true; // consecutive distinct times differ by at least 8 ulp(max |t|)
+    double block_span = 0.0; // largest time span of 512 consecutive sorted events
+    bool gaps_ok = true;     // consecutive distinct times differ by at least 8 ulp(max |t|)
 };
 
 // locs: N x D in any order; sorted_t: the N times in ascending order.
@@ -114,6 +115,7 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
         const double g = sorted_t[i] - sorted_t[i - 1];
         if (g > 0.0 && g < 8.0 * ulp) e.gaps_ok = false;
     }
+    for (size_t b = 0; b < n; b += 512) e.block_span = std::max(e.block_span, sorted_t[std::min(b + 511, n - 1)] - sorted_t[b]);
     return e;
 }
 
@@ -122,11 +124,15 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
 // vectors is below 1e9 in size, the time scale st = |tauT| sqrt(log2(e)/2) lies in [1e-100, 1e100]
 // with omega log2(e) / st finite, and distinct times are at least 8 ulp(max |t|) apart: then
 // (t - t0) st, rounded, keeps every strict order and every tie of the times, so the dt > 0 mask
-// of the excitation term is decided as on the original times.
+// of the excitation term is decided as on the original times. Accuracy: the scaled times carry
+// rounding of order eps |t - t0| st, t0 = the block's first event, so an excitation exponent
+// omega dt is off by up to ~2 eps log2(e) |omega| (block time span); the form is used only while
+// that stays below 1e-13 (|omega| x span <= 300), i.e. excitation terms within 1e-13 relative.
 static bool fast_ok(const Extents &e, const double *params, int K) {
     if (!e.gaps_ok) return false;
     for (int k = 0; k < K; ++k) {
         const double *p = params + 6 * (size_t)k;
+        if (!(std::fabs(p[3]) * e.block_span <= 300.0)) return false;
         const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
         if (!(aB < 1e9 && aS < 1e9)) return false;
@@ -153,12 +159,41 @@ class SymRunner {
     }
 
     // Plans (if the shape changed) and enqueues the pass, finalize and reduction on stream s.
-    // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev: dataset 0, vector 0.
+    // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev, inv_lam_dev (may be null):
+    // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
                  bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, bool fast,
-                 double *probs_dev, double *result, hipStream_t s, hipEvent_t ev0 = nullptr,
+                 double *probs_dev, double *inv_lam_dev, double *result, hipStream_t s, hipEvent_t ev0 = nullptr,
                  hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
+        const kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
+        if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
+        kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
+        check(hipGetLastError(), "symmetric pair kernel launch");
+        if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), d_block_.as<double>(),
+                                  probs_dev, inv_lam_dev, s);
+        check(hipGetLastError(), "finalize kernel launch");
+        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
+        check(hipGetLastError(), "reduce kernel launch");
+        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+    }
+
+    // Location-gradient pass over one dataset and one parameter vector (the plan of a K = 1
+    // gradient evaluation, shared with it): w = 1/lambda (device, sorted), out = N x D (device).
+    void enqueue_locgrad(const DataRef &ref, int dim, const double *params, const double *w, double *out,
+                         int rows_per_lane, int target_items, int skip_zero, hipStream_t s) {
+        const kern::SymLaunch L = prepare({ref}, {1}, dim, true, false, rows_per_lane, target_items, skip_zero, false);
+        kern::launch_sym_locgrad(L, params, w, d_row_.as<double>(), d_col_.as<double>(), out, s);
+        check(hipGetLastError(), "symmetric location gradient launch");
+    }
+
+    const kern::SymPlan &plan() const { return plan_; }
+
+  private:
+    // Plans (if the shape changed), uploads the tables (if they changed) and fills the launch.
+    kern::SymLaunch prepare(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, bool grad,
+                            bool fp32, int rows_per_lane, int target_items, int skip_zero, bool fast) {
         std::vector<int> n(refs.size());
         for (size_t d = 0; d < refs.size(); ++d) n[d] = refs[d].n;
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
@@ -233,21 +268,9 @@ class SymRunner {
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
         L.res_items = (int)plan_.res_items.size();
-        if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
-        kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
-        check(hipGetLastError(), "symmetric pair kernel launch");
-        if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
-        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), d_block_.as<double>(),
-                                  probs_dev, s);
-        check(hipGetLastError(), "finalize kernel launch");
-        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
-        check(hipGetLastError(), "reduce kernel launch");
-        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+        return L;
     }
 
-    const kern::SymPlan &plan() const { return plan_; }
-
-  private:
     kern::SymPlan plan_;
     bool valid_ = false, grad_ = false;
     std::vector<int> n_, K_;
@@ -441,14 +464,32 @@ class HipHawkes : public AbstractHawkes {
         evaluate_single(false, result);
     }
 
-    // Rate pass (row-wise, no gradient) -> w = 1/lambda, then the location-gradient pass; the
-    // result is downloaded in the caller's event order.
+    // w = 1/lambda from the last single-vector evaluation of the current state (else one is run,
+    // which also caches the log-likelihood, gradient and probabilities; FP32-bulk engines run a
+    // row-wise FP64 rate pass instead), then the location-gradient pass of the engine's pass
+    // configuration (symmetric for D <= 3, else row-wise); the result is downloaded in the
+    // caller's event order. Timing: [0] = the rate pass run by this call (0
+    // when reused), [1] = the location-gradient pass.
     void getLocationGradient(double *out, size_t length) override {
         if (out == nullptr) throw std::invalid_argument("location gradient pointer is null");
         const int N = locationCount, D = embeddingDimension;
         if (length != (size_t)N * D) throw std::invalid_argument("location gradient length must be N*D");
         require_ready();
+        if (!have_params_) throw NotInitialized("parameters are not set (setParameters)");
         check(hipSetDevice(device_), "hipSetDevice");
+        double rate_ms = 0.0;
+        const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
+        if (!w_valid_ && !fp32_) {
+            force_natural_ = true;
+            try {
+                evaluate_single(true, nullptr);
+            } catch (...) {
+                force_natural_ = false;
+                throw;
+            }
+            force_natural_ = false;
+            rate_ms = last_ms_[0] + last_ms_[1];
+        }
         if (data_dirty_) upload();
         d_params_.ensure(sizeof(double) * 6);
         h_params_.ensure(sizeof(double) * 6);
@@ -457,24 +498,38 @@ class HipHawkes : public AbstractHawkes {
               "upload params");
         const kern::Events ev = events();
         kern::PairGrid g = kern::plan_pair_grid(N, 1, 4096);
-        g.skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
+        g.skip_zero = skip_zero;
         d_part_.ensure(sizeof(double) * (size_t)g.chunks * std::max(2, D) * npad_);
         d_w_.ensure(sizeof(double) * npad_);
         d_locgrad_.ensure(sizeof(double) * (size_t)N * D);
-        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-        kern::launch_pair(ev, d_params_.as<double>(), 1, g, false, false, d_part_.as<double>(), stream_);
-        check(hipGetLastError(), "rate pass launch");
-        kern::launch_inv_rate(ev, d_params_.as<double>(), g, d_part_.as<double>(), d_w_.as<double>(), stream_);
-        check(hipGetLastError(), "inverse rate launch");
+        if (fp32_) {
+            if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+            kern::launch_pair(ev, d_params_.as<double>(), 1, g, false, false, d_part_.as<double>(), stream_);
+            check(hipGetLastError(), "rate pass launch");
+            kern::launch_inv_rate(ev, d_params_.as<double>(), g, d_part_.as<double>(), d_w_.as<double>(), stream_);
+            check(hipGetLastError(), "inverse rate launch");
+        } else if (timing_) {
+            check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        }
         if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
-        kern::launch_locgrad(ev, d_params_.as<double>(), d_w_.as<double>(), g, d_part_.as<double>(),
-                             d_locgrad_.as<double>(), stream_);
-        check(hipGetLastError(), "location gradient launch");
+        const TuneConfig cfg = config_for(1, true);
+        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && D <= 3) {
+            SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+            sym_.enqueue_locgrad(ref, D, d_params_.as<double>(), d_w_.as<double>(), d_locgrad_.as<double>(),
+                                 cfg.rows_per_lane, cfg.target_blocks, skip_zero, stream_);
+        } else {
+            kern::launch_locgrad(ev, d_params_.as<double>(), d_w_.as<double>(), g, d_part_.as<double>(),
+                                 d_locgrad_.as<double>(), stream_);
+            check(hipGetLastError(), "location gradient launch");
+        }
         if (timing_) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(out, d_locgrad_.ptr, sizeof(double) * (size_t)N * D, hipMemcpyDeviceToHost, stream_),
               "download location gradient");
         check(hipStreamSynchronize(stream_), "location gradient sync");
-        if (timing_) record_timing();
+        if (timing_) {
+            record_timing();
+            if (!fp32_) last_ms_[0] = rate_ms;
+        }
     }
 
     // Snapshot / restore of parameters, locations and the last log-likelihood: the intent of
@@ -624,7 +679,7 @@ class HipHawkes : public AbstractHawkes {
     }
 
   private:
-    void invalidate() { ll_valid_ = grad_valid_ = probs_valid_ = false; }
+    void invalidate() { ll_valid_ = grad_valid_ = probs_valid_ = w_valid_ = false; }
 
     void require_ready() const {
         if (!have_locs_) throw NotInitialized("locations are not set (updateLocations)");
@@ -728,13 +783,20 @@ class HipHawkes : public AbstractHawkes {
         const kern::Events ev = events();
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         double *probs_dev = (probs_host || write_probs) ? d_probs_.as<double>() : nullptr;
+        double *w_dev = nullptr; // 1 / lambda of vector 0 (sorted order), kept for the location gradient
+        if (write_probs) {
+            d_w_.ensure(sizeof(double) * npad_);
+            w_dev = d_w_.as<double>();
+        }
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+            last_fast_ = !force_natural_ && fast_ok(extents_, params, K);
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, fast_ok(extents_, params, K), probs_dev,
+                         cfg.target_blocks, skip_zero, last_fast_, probs_dev, w_dev,
                          d_result_.as<double>(), stream_,
                          timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
+            last_fast_ = false;
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
             g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
@@ -746,7 +808,7 @@ class HipHawkes : public AbstractHawkes {
             check(hipGetLastError(), "pair kernel launch");
             if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
             kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
-                                  d_block_.as<double>(), probs_dev, stream_);
+                                  d_block_.as<double>(), probs_dev, w_dev, stream_);
             check(hipGetLastError(), "finalize kernel launch");
             kern::launch_reduce(ev, K, fin_blocks, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
             check(hipGetLastError(), "reduce kernel launch");
@@ -776,6 +838,9 @@ class HipHawkes : public AbstractHawkes {
         if (!have_params_) throw NotInitialized("parameters are not set (setParameters)");
         run(params_, 1, grad, probs_host, true);
         probs_valid_ = true;
+        // the location gradient takes 1/lambda from natural-form FP64 passes only: its components
+        // cancel between neighbouring events, so it needs rates to about 1e-15
+        w_valid_ = !fp32_ && !last_fast_;
         const double *r = h_result_.as<double>();
         ll_ = r[0];
         ll_valid_ = true;
@@ -803,7 +868,8 @@ class HipHawkes : public AbstractHawkes {
     std::vector<double> batch_;
 
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
-    bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false;
+    bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
+    bool force_natural_ = false, last_fast_ = false;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
     double stored_params_[6] = {0, 0, 0, 0, 0, 0};
@@ -1004,7 +1070,7 @@ class HipHawkesBatch {
             refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
                                          d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
         const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
-        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, fast, nullptr,
+        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, fast, nullptr, nullptr,
                         d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
                         events ? ev_[2] : nullptr);
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,

@@ -67,9 +67,9 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
 
 // Per-row log-likelihood terms and gradient terms, reduced per block in double-double.
 // block_out layout: [K][row_blocks][7][2]. probs (may be null) receives p_i for k == 0 in the
-// caller's order.
+// caller's order, inv_lam (may be null) 1 / lambda_i for k == 0 in sorted order.
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
-                     const double *part, double *block_out, double *probs, hipStream_t s);
+                     const double *part, double *block_out, double *probs, double *inv_lam, hipStream_t s);
 
 // Sums the per-block partials (nblocks finalize blocks of 256 rows) per k in a fixed order:
 // result layout [K][7] (only [k][0] without grad).
@@ -144,10 +144,17 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
 void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart, double *colpart,
                 hipStream_t s);
 // Finalize (per row, then per 256-row block in double-double) and reduction (per dataset and
-// vector: result[(res0 + k) * 7 + q]); probs (may be null) receives dataset 0, vector 0.
+// vector: result[(res0 + k) * 7 + q]); probs and inv_lam (may be null) receive dataset 0,
+// vector 0 (as launch_finalize).
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
-                         const double *colpart, double *block_out, double *probs, hipStream_t s);
+                         const double *colpart, double *block_out, double *probs, double *inv_lam, hipStream_t s);
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
+// Zeroes the work-queue counter (a one-thread kernel, so that the sequence can be graph-captured).
+void launch_queue_reset(const SymLaunch &L, hipStream_t s);
+// Symmetric location-gradient pass over dataset 0, vector 0 of a plan made with grad = true
+// (D <= 3): w = 1/lambda in sorted order; out = d ll / d x, N x D in the caller's order.
+void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *w, double *rowpart,
+                        double *colpart, double *out, hipStream_t s);
 
 } // namespace kern
 } // namespace hph

@@ -151,7 +151,8 @@ template <bool GRAD>
 __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, const double *__restrict__ params,
                                                                  PairGrid g, const double *__restrict__ part,
                                                                  double *__restrict__ block_out,
-                                                                 double *__restrict__ probs) {
+                                                                 double *__restrict__ probs,
+                                                                 double *__restrict__ inv_lam) {
     constexpr int NACC = GRAD ? 6 : 2;
     __shared__ double2 sh[kRowsPerBlock];
     const int k = blockIdx.y;
@@ -176,8 +177,10 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
             s[0] = a6[0];
             s[3] = a6[NACC - 1];
         }
-        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v);
+        double lam;
+        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v, lam);
         if (probs != nullptr && k == 0) probs[ev.perm[i]] = p;
+        if (inv_lam != nullptr && k == 0) inv_lam[i] = 1.0 / lam;
     }
     store_block_sums<GRAD>(v, sh, block_out, k, g.row_blocks, rb);
 }
@@ -260,14 +263,14 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
 }
 
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
-                     const double *part, double *block_out, double *probs, hipStream_t s) {
+                     const double *part, double *block_out, double *probs, double *inv_lam, hipStream_t s) {
     dim3 grid((unsigned)g.row_blocks, (unsigned)K);
     if (grad)
         hipLaunchKernelGGL((finalize_kernel<true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
-                           block_out, probs);
+                           block_out, probs, inv_lam);
     else
         hipLaunchKernelGGL((finalize_kernel<false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
-                           block_out, probs);
+                           block_out, probs, inv_lam);
 }
 
 void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,

@@ -26,11 +26,6 @@ namespace kern {
 // Excitation direction of a tile relative to a wavefront's rows.
 enum { kLNone = 0, kLBefore = 1, kLAfter = 2, kLMixed = 3 };
 
-// Stride (doubles) of one column in the LDS tile: D coordinates, t, w, padded to 16 bytes.
-template <int D> struct LocStride {
-    static constexpr int value = (D + 3) & ~1;
-};
-
 template <int D, bool DOB, int SMODE>
 __device__ __forceinline__ void locgrad_tile(const double *__restrict__ tile, int cnt, const double (&xk)[D],
                                              double tk, double wk, double cBx, double cBt, double cSx, double omega,
@@ -173,6 +168,317 @@ __global__ __launch_bounds__(kRowsPerBlock) void locgrad_finalize_kernel(Events 
         double s = 0.0;
         for (int c = 0; c < g.chunks; ++c) s += part[((size_t)c * D + d) * npad + i];
         out[(size_t)u * D + d] = s;
+    }
+}
+
+// ---- symmetric form (D <= 3): each background exp serves both events of a pair ----
+//
+// Same work decomposition, queue and partial layout as the symmetric likelihood pass with
+// gradient (plan_sym(..., grad = true): 6 row and 3 column accumulators, of which the first D
+// are used). A row block is paired with earlier columns only; for a row i and an earlier column
+// j, f = fB eB_ij (w_i + w_j) + fS eS_ij w_i (eS_ji = 0), and d ll / d x_i gets -f (x_i - x_j),
+// d ll / d x_j gets +f (x_i - x_j) (rotating column accumulators, as in the likelihood pass).
+// The diagonal block is paired in both orders, each order adding to its row only.
+template <int D, int A, bool DOB, int SMODE, bool TAIL>
+__device__ __forceinline__ void loc_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
+                                            const double (&ti)[A], const double (&wi)[A], const double (&vm)[A],
+                                            double cBx, double cBt, double cSx, double nomega, double fB, double fS,
+                                            double (&acc)[A][D], double (&ca)[D]) {
+    const int src = ((lane + 1) & 63) << 2;
+#pragma unroll 2
+    for (int s = 0; s < 64; ++s) {
+        // structure-of-arrays subtile: x (D arrays), t, w; component d of entry u at sub[d * 128 + u]
+        double q[D + 2];
+#pragma unroll
+        for (int d = 0; d < D + 2; ++d) q[d] = sub[d * 128 + lane + s];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            double dx[D];
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                dx[d] = xi[a][d] - q[d];
+                r2 = __builtin_fma(dx[d], dx[d], r2);
+            }
+            const double dt = ti[a] - q[D];
+            double f = 0.0;
+            if (DOB) f = fB * background_exp<false>(cBx, cBt, r2, dt * dt) * (wi[a] + q[D + 1]);
+            if (SMODE != kSNone) {
+                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
+                f = __builtin_fma(fS * es, wi[a], f);
+            }
+            const double fc = TAIL ? f * vm[a] : f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                acc[a][d] = __builtin_fma(-f, dx[d], acc[a][d]);
+                ca[d] = __builtin_fma(fc, dx[d], ca[d]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) ca[d] = lane_shift_down(ca[d], src);
+    }
+}
+
+template <int D, int A, bool TAIL>
+__device__ __forceinline__ void loc_dispatch(int mode, const double *__restrict__ sub, int lane,
+                                             const double (&xi)[A][D], const double (&ti)[A], const double (&wi)[A],
+                                             const double (&vm)[A], double cBx, double cBt, double cSx,
+                                             double nomega, double fB, double fS, double (&acc)[A][D],
+                                             double (&ca)[D]) {
+    switch (mode) {
+    case 4 + kSNone:
+        loc_subtile<D, A, true, kSNone, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        break;
+    case 4 + kSFull:
+        loc_subtile<D, A, true, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        break;
+    case 4 + kSMixed:
+        loc_subtile<D, A, true, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        break;
+    case kSFull:
+        loc_subtile<D, A, false, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        break;
+    case kSMixed:
+        loc_subtile<D, A, false, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        break;
+    default: break; // every term of the subtile is exactly zero
+    }
+}
+
+template <int D, int A>
+__global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L, const double *__restrict__ params,
+                                                                     const double *__restrict__ w,
+                                                                     double *__restrict__ rowpart,
+                                                                     double *__restrict__ colpart) {
+    constexpr int LS = LocStride<D>::value;
+    constexpr int NR = 6, NC = 3; // partial layout of the gradient plan
+    constexpr int R = kSymWaves * 64 * A;
+    // diagonal items: kTile events x LS doubles; chunk items: 4 subtiles x (D + 2) x 128 doubles
+    __shared__ double tile[(2 * kTile * (D + 2) > kTile * LS) ? 2 * kTile * (D + 2) : kTile * LS];
+    __shared__ double colbuf[kSymWaves * D * kTile];
+    __shared__ int s_item;
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(L.counter, 1);
+        __syncthreads();
+        const int idx = s_item;
+        __syncthreads();
+        if (idx >= L.items) break;
+
+        const int4 it = L.item[idx];
+        const SymData &ds = L.data[it.x];
+        const int P = it.y, chunk = it.z, k = it.w;
+        const double *pk = params + 6 * (size_t)(ds.param0 + k);
+        const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3], theta = pk[4], mu0 = pk[5];
+        const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT, cSx = -0.5 * sigma * sigma;
+        const double nomega = -omega;
+        const double fB = mu0 * ipow(tauX, D) * tauT * 0.15915494309189533577 * tauX * tauX;
+        const double fS = theta * omega * ipow(sigma, D) * 0.39894228040143267794 * sigma * sigma;
+        const double *__restrict__ ex = ds.x;
+        const double *__restrict__ et = ds.t;
+        const int n = ds.n, npad = ds.npad;
+        const int P0 = P * R;
+
+        const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + wv * 64 * A);
+        double xi[A][D], ti[A], wi[A], vm[A];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const int r = wrow0 + a * 64 + lane;
+            const int rc = min(r, n - 1);
+#pragma unroll
+            for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
+            ti[a] = et[rc];
+            wi[a] = w[rc];
+            vm[a] = r < n ? 1.0 : 0.0;
+        }
+        const bool wave_live = wrow0 < n;
+        const double t_lo = et[min(wrow0, n - 1)];
+
+        double acc[A][D];
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+#pragma unroll
+            for (int d = 0; d < D; ++d) acc[a][d] = 0.0;
+
+        int slot;
+        if (chunk < 0) {
+            slot = ds.max_chunks;
+            const int c1 = min(P0 + R, n);
+            for (int j0 = P0; j0 < c1; j0 += kTile) {
+                const int cnt = min(kTile, c1 - j0);
+                __syncthreads();
+                if (tid < cnt) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) tile[tid * LS + d] = ex[(size_t)d * npad + j0 + tid];
+                    tile[tid * LS + D] = et[j0 + tid];
+                    tile[tid * LS + D + 1] = w[j0 + tid];
+                }
+                __syncthreads();
+                if (wave_live) {
+#pragma unroll
+                    for (int a = 0; a < A; ++a)
+                        locgrad_tile<D, true, kLMixed>(tile, cnt, xi[a], ti[a], wi[a], cBx, cBt, cSx, omega, fB, fS,
+                                                      acc[a]);
+                }
+            }
+        } else {
+            slot = chunk;
+            const int c0 = chunk * ds.L;
+            const int c1 = min(c0 + ds.L, P0);
+            const bool tail = P0 + R > n;
+            double *cp = colpart + ds.col0 + (size_t)k * ds.col_kstride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
+            const double t_blk = et[min(P0, n - 1)];
+            for (int j0 = c0; j0 < c1; j0 += kTile) {
+                // c1 - c0 and P0 are multiples of kTile: every tile here is full.
+                if (L.skip_zero) {
+                    // workgroup-wide skip when no term survives for the block's first row
+                    const double tl = et[j0 + kTile - 1];
+                    const double gap = t_blk - tl;
+                    if (tl < t_blk && -cBt * gap * gap >= kZeroExpArg && omega >= 0.0 && omega * gap >= kZeroExpArg) {
+#pragma unroll
+                        for (int c = 0; c < D; ++c) cp[(size_t)c * P0 + j0 + tid] = 0.0;
+                        continue;
+                    }
+                }
+                __syncthreads();
+                {
+                    const int e0 = (tid >> 6) * (D + 2) * 128 + (tid & 63);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const double v = ex[(size_t)d * npad + j0 + tid];
+                        tile[e0 + d * 128] = v;
+                        tile[e0 + d * 128 + 64] = v;
+                    }
+                    const double tv = et[j0 + tid], wv2 = w[j0 + tid];
+                    tile[e0 + D * 128] = tv;
+                    tile[e0 + D * 128 + 64] = tv;
+                    tile[e0 + (D + 1) * 128] = wv2;
+                    tile[e0 + (D + 1) * 128 + 64] = wv2;
+                }
+                __syncthreads();
+                if (wave_live) {
+                    const double tl = tile[(3 * (D + 2) + D) * 128 + 63];
+                    int smode = tl < t_lo ? kSFull : kSMixed;
+                    int dob = 1;
+                    if (L.skip_zero) {
+                        const double gap = tl < t_lo ? t_lo - tl : 0.0;
+                        if (-cBt * gap * gap >= kZeroExpArg) dob = 0;
+                        if (smode == kSFull && omega >= 0.0 && omega * gap >= kZeroExpArg) smode = kSNone;
+                    }
+                    const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
+                    for (int sb = 0; sb < 4; ++sb) {
+                        double ca[D];
+#pragma unroll
+                        for (int c = 0; c < D; ++c) ca[c] = 0.0;
+                        const double *sub = tile + sb * (D + 2) * 128;
+                        if (tail)
+                            loc_dispatch<D, A, true>(mode, sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS,
+                                                     acc, ca);
+                        else
+                            loc_dispatch<D, A, false>(mode, sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB,
+                                                      fS, acc, ca);
+#pragma unroll
+                        for (int c = 0; c < D; ++c) colbuf[(wv * D + c) * kTile + sb * 64 + lane] = ca[c];
+                    }
+                } else {
+#pragma unroll
+                    for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+                        for (int c = 0; c < D; ++c) colbuf[(wv * D + c) * kTile + sb * 64 + lane] = 0.0;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    double v = colbuf[(0 * D + c) * kTile + tid];
+#pragma unroll
+                    for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * D + c) * kTile + tid];
+                    cp[(size_t)c * P0 + j0 + tid] = v;
+                }
+            }
+        }
+        const size_t rpad = (size_t)ds.nb * R;
+        double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const int r = wrow0 + a * 64 + lane;
+#pragma unroll
+            for (int d = 0; d < D; ++d) rp[(size_t)d * rpad + r] = acc[a][d];
+        }
+    }
+}
+
+// Row sums of the symmetric location-gradient pass (row chunks, diagonal block, column side of
+// every later block, in a fixed order) scattered to out[perm[i] * D + d] (dataset 0, vector 0).
+template <int D>
+__global__ __launch_bounds__(kRowsPerBlock) void sym_locgrad_finalize_kernel(SymLaunch L,
+                                                                             const double *__restrict__ rowpart,
+                                                                             const double *__restrict__ colpart,
+                                                                             double *__restrict__ out) {
+    constexpr int NR = 6, NC = 3;
+    const int4 it = L.fin_item[blockIdx.x];
+    if (it.x != 0 || it.y != 0) return;
+    const SymData &ds = L.data[0];
+    const int i = it.z * kRowsPerBlock + threadIdx.x;
+    const int n = ds.n, R = L.R;
+    if (i >= n) return;
+    const int P = i / R;
+    const int nch = (P * R + ds.L - 1) / ds.L;
+    const size_t rpad = (size_t)ds.nb * R;
+    const double *rbase = rowpart + ds.row0;
+    double a[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) a[d] = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        const double *rp = rbase + (size_t)c * NR * rpad;
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[d] += rp[(size_t)d * rpad + i];
+    }
+    {
+        const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[d] += rp[(size_t)d * rpad + i];
+    }
+    double cs[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) cs[d] = 0.0;
+    const double *cbase = colpart + ds.col0;
+    for (int Q = P + 1; Q < ds.nb; ++Q) {
+        const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
+        const size_t Q0 = (size_t)Q * R;
+#pragma unroll
+        for (int d = 0; d < D; ++d) cs[d] += cp[(size_t)d * Q0 + i];
+    }
+    const int u = ds.perm[i];
+#pragma unroll
+    for (int d = 0; d < D; ++d) out[(size_t)u * D + d] = a[d] + cs[d];
+}
+
+template <int D>
+static void launch_sym_locgrad_d(const SymLaunch &L, const double *params, const double *w, double *rowpart,
+                                 double *colpart, double *out, hipStream_t s) {
+    if (L.A == 2)
+        hipLaunchKernelGGL((sym_locgrad_kernel<D, 2>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L, params,
+                           w, rowpart, colpart);
+    else
+        hipLaunchKernelGGL((sym_locgrad_kernel<D, 1>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L, params,
+                           w, rowpart, colpart);
+    hipLaunchKernelGGL((sym_locgrad_finalize_kernel<D>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
+                       rowpart, colpart, out);
+}
+
+void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *w, double *rowpart,
+                        double *colpart, double *out, hipStream_t s) {
+    if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
+    if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
+    if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
+    launch_queue_reset(L, s);
+    switch (L.dim) {
+    case 1: launch_sym_locgrad_d<1>(L, params, w, rowpart, colpart, out, s); break;
+    case 2: launch_sym_locgrad_d<2>(L, params, w, rowpart, colpart, out, s); break;
+    case 3: launch_sym_locgrad_d<3>(L, params, w, rowpart, colpart, out, s); break;
+    default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
 }
 

@@ -24,14 +24,6 @@
 namespace hph {
 namespace kern {
 
-constexpr int kSymWaves = 4;
-
-__device__ __forceinline__ double lane_shift_down(double v, int byte_addr) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)u);
-    const int hi = __builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)(u >> 32));
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
 
 // Per-vector constants of the pair loops, in one of two forms.
 //  natural (FAST false): times as given; exponents in e-units with the clamped exp_fp64:
@@ -383,7 +375,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
                                                                      const double *__restrict__ rowpart,
                                                                      const double *__restrict__ colpart,
                                                                      double *__restrict__ block_out,
-                                                                     double *__restrict__ probs) {
+                                                                     double *__restrict__ probs,
+                                                                     double *__restrict__ inv_lam) {
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     __shared__ double2 sh[kRowsPerBlock];
@@ -439,8 +432,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
             s[0] = a6[0] + cs[0];
             s[3] = a6[NR - 1];
         }
-        const double p = row_terms<GRAD>(s, params + 6 * (size_t)(ds.param0 + k), L.dim, ds.t[i], ds.t_last, v);
+        double lam;
+        const double p =
+            row_terms<GRAD>(s, params + 6 * (size_t)(ds.param0 + k), L.dim, ds.t[i], ds.t_last, v, lam);
         if (probs != nullptr && it.x == 0 && k == 0) probs[ds.perm[i]] = p;
+        if (inv_lam != nullptr && it.x == 0 && k == 0) inv_lam[i] = 1.0 / lam;
     }
     const size_t blk = (size_t)ds.blk0 + (size_t)k * ds.fin_blocks + fb;
     for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
@@ -593,12 +589,16 @@ static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, do
     }
 }
 
+void launch_queue_reset(const SymLaunch &L, hipStream_t s) {
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
+}
+
 void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart, double *colpart,
                 hipStream_t s) {
     if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
     if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
+    launch_queue_reset(L, s);
     if (fp32) {
         if (L.fast)
             launch_sym_p<true, true>(L, params, grad, rowpart, colpart, s);
@@ -613,13 +613,13 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 }
 
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
-                         const double *colpart, double *block_out, double *probs, hipStream_t s) {
+                         const double *colpart, double *block_out, double *probs, double *inv_lam, hipStream_t s) {
     if (grad)
         hipLaunchKernelGGL((sym_finalize_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
-                           params, rowpart, colpart, block_out, probs);
+                           params, rowpart, colpart, block_out, probs, inv_lam);
     else
         hipLaunchKernelGGL((sym_finalize_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
-                           params, rowpart, colpart, block_out, probs);
+                           params, rowpart, colpart, block_out, probs, inv_lam);
 }
 
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s) {
