@@ -159,6 +159,30 @@ def cpu_baseline(locs, times, params, seconds):
     }
 
 
+def cpu_config1(seconds):
+    """The reference's CPU-runnable case (N = 1000 uniform events, 100 repeated log-likelihood
+    evaluations) on the three CPU flavours restated from the reference engines: serial scalar
+    FP64 (1 core), AVX2 4-wide FP64 (1 core), OpenMP rows + AVX2 (all host threads). The number of
+    evaluations is cut below 100 only when the budget `seconds` would not cover the serial one."""
+    import oracle
+    from hawkes_amd import synth
+    locs, times, p0 = synth.workload("uniform_n1000")
+    n = len(times)
+    t0 = time.perf_counter()
+    oracle.cpu_baseline(oracle.SERIAL, locs, times, p0, grad=False)
+    t_serial = time.perf_counter() - t0
+    evals = int(max(1, min(100, seconds / (2.0 * max(t_serial, 1e-6)))))
+    out = {"N": n, "evals": evals, "unit": "pair-evals/s"}
+    for name, flavour, cores in (("serial", oracle.SERIAL, 1), ("avx", oracle.AVX, 1),
+                                 ("threaded", oracle.THREADED, oracle.num_threads())):
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            oracle.cpu_baseline(flavour, locs, times, p0, grad=False)
+        dt = time.perf_counter() - t0
+        out[name] = {"value": evals * n * n / dt, "seconds": dt, "cores": cores}
+    return out
+
+
 def dataset_batch(count):
     """Uneven multi-dataset batch: `count` cluster-process datasets (N log-uniform in 2e3..2e5),
     8 jittered parameter vectors each, LL + gradient for all in one graph-replayed launch, FP64
@@ -448,6 +472,7 @@ def main():
     small = small_n() if "small" in legs else None
 
     cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    cpu1 = cpu_config1(args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     out = {
         "metric": METRIC,
@@ -473,6 +498,7 @@ def main():
         "large_n1e6": large,
         "small_n1000": small,
         "location_gradient": locgrad,
+        "cpu_config1": cpu1,
     }
     print(json.dumps(out))
 
