@@ -119,20 +119,6 @@ __device__ __forceinline__ double ipow(double x, int e) {
     return r;
 }
 
-// Fixed-order tree reduction of one double-double per thread over a 256-thread block.
-__device__ double2 block_reduce_dd(double2 v, double2 *sh) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int s = kRowsPerBlock / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] = dd_add(sh[tid], sh[tid + s]);
-        __syncthreads();
-    }
-    const double2 r = sh[0];
-    __syncthreads();
-    return r;
-}
-
 // Per-row log-likelihood term and gradient terms from the row's exp sums (src/NewHawkes.hpp:652-655):
 //   ll_i = log(lambda_i) + theta (e^{-omega (T - t_i)} - 1) - mu0 (Phi(tauT (T - t_i)) - Phi(-tauT t_i))
 // s holds (sum eB, sum eB r^2, sum eB dt^2, sum eS, sum eS r^2, sum eS dt) with
@@ -174,16 +160,44 @@ __device__ __forceinline__ double row_terms(const double (&s)[6], const double *
         return se / (bg + se);
 }
 
-// Writes the block's double-double sums of v (GRAD: all 7, else v[0]) to block_out[k][blk][q][2].
+// Fixed-order tree reduction of W double-doubles per thread over a 256-thread block, all W in
+// one set of barriers: sh holds W x 256 entries; afterwards v holds the block sums (in every
+// thread). The pairing is fixed, so results are bitwise reproducible.
+template <int W>
+__device__ void block_reduce_dd_w(double2 (&v)[W], double2 *sh) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < W; ++q) sh[q * kRowsPerBlock + tid] = v[q];
+    __syncthreads();
+    for (int s = kRowsPerBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+#pragma unroll
+            for (int q = 0; q < W; ++q)
+                sh[q * kRowsPerBlock + tid] = dd_add(sh[q * kRowsPerBlock + tid], sh[q * kRowsPerBlock + tid + s]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < W; ++q) v[q] = sh[q * kRowsPerBlock];
+    __syncthreads();
+}
+
+// Writes the block's double-double sums of v (GRAD: all 7, else v[0]) to block_out[k][blk][q][2];
+// sh holds 7 x 256 entries.
 template <bool GRAD>
 __device__ __forceinline__ void store_block_sums(const double (&v)[7], double2 *sh, double *__restrict__ block_out,
                                                  int k, int nblocks, int blk) {
-    for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
-        const double2 r = block_reduce_dd(make_double2(v[q], 0.0), sh);
-        if (threadIdx.x == 0) {
+    constexpr int W = GRAD ? 7 : 1;
+    double2 r[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) r[q] = make_double2(v[q], 0.0);
+    block_reduce_dd_w<W>(r, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
             double *o = block_out + (((size_t)k * nblocks + blk) * kResultWidth + q) * 2;
-            o[0] = r.x;
-            o[1] = r.y;
+            o[0] = r[q].x;
+            o[1] = r[q].y;
         }
     }
 }

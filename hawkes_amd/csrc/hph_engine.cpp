@@ -604,11 +604,12 @@ class HipHawkes : public AbstractHawkes {
         const int K = use_batch ? (int)(batch_.size() / 6) : 1;
         const double *params = use_batch ? batch_.data() : params_;
         std::vector<TuneConfig> cands;
-        for (int target : {1024, 2048, 4096, 8192}) {
+        for (int target : {1024, 2048, 4096, 8192, 65536}) {
             TuneConfig c;
             c.kind = HPH_PAIR_KERNEL_ROWWISE;
             c.target_blocks = target;
             cands.push_back(c);
+            if (target > 8192) continue; // only the row-wise pass changes layout (narrow) at 65536
             if (embeddingDimension <= 3 && kernel_ != HPH_PAIR_KERNEL_ROWWISE) {
                 for (int A : {1, 2}) {
                     TuneConfig d;
@@ -728,8 +729,8 @@ class HipHawkes : public AbstractHawkes {
     }
 
     // Pair-pass configuration for K parameter vectors: the autotuned one when this engine or an
-    // earlier one in the process has tuned this shape, else the built-in choice (symmetric pass
-    // with 2 rows per lane for D <= 3 and at least 4096 events, row-wise otherwise).
+    // earlier one in the process has tuned this shape, else the built-in choice (row-wise with the
+    // narrow layout for small N or D > 3, the symmetric pass above; thresholds below).
     TuneConfig config_for(int K, bool grad) const {
         TuneConfig c;
         const bool sym_ok = embeddingDimension <= 3;
@@ -737,10 +738,17 @@ class HipHawkes : public AbstractHawkes {
             c = forced_;
         } else if (!sym_ok) {
             c.kind = HPH_PAIR_KERNEL_ROWWISE;
+            if (locationCount < kNarrowMaxEvents) c.target_blocks = 65536;
         } else {
             TuneConfig t;
             if (tune_cache().lookup(tune_key(K, grad), &t)) return t;
-            c.kind = locationCount >= kSymMinEvents ? HPH_PAIR_KERNEL_SYMMETRIC : HPH_PAIR_KERNEL_ROWWISE;
+            if (locationCount < kNarrowMaxEvents) {
+                c.kind = HPH_PAIR_KERNEL_ROWWISE;
+                c.target_blocks = 65536; // the narrow layout below ~32k events
+            } else {
+                c.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+                c.rows_per_lane = locationCount < kSymA1MaxEvents ? 1 : 2;
+            }
         }
         return c;
     }
@@ -850,7 +858,11 @@ class HipHawkes : public AbstractHawkes {
         }
     }
 
-    static constexpr int kSymMinEvents = 4096;
+    // Built-in choice without tuning (one MI355X, LL + gradient, K = 1): row-wise narrow below
+    // 12,288 events (41 us at N = 1e3, 70 us at 5e3, 99 us at 8e3), symmetric with 1 row per lane
+    // below 24,576 (229 us at 1.6e4), symmetric with 2 rows per lane above (551 us at 3.3e4).
+    static constexpr int kNarrowMaxEvents = 12288;
+    static constexpr int kSymA1MaxEvents = 24576;
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
     Extents extents_;

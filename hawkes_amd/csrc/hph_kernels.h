@@ -28,9 +28,10 @@ struct Events {
 
 // Work decomposition of the pair pass.
 struct PairGrid {
-    int row_blocks;    // ceil(n / kRowsPerBlock)
+    int row_blocks;    // ceil(n / kRowsPerBlock) (finalize blocks)
     int chunks;        // column chunks per row block
-    int chunk_len;     // events per chunk (multiple of kTile)
+    int chunk_len;     // events per chunk (multiple of kTile; kNarrow in the narrow layout)
+    int narrow;        // 1: work items of kNarrow rows x kTile columns, 4 wavefronts of 64 columns
     int skip_zero;     // 1: skip tiles whose terms are all negligible (background) or zero (excitation)
     double zero_arg_b; // background tiles whose every exponent is <= -zero_arg_b are skipped
     double zero_arg_s; // excitation exp arguments <= -zero_arg_s give exactly +0.0
@@ -54,6 +55,11 @@ inline double background_skip_arg(long long n, bool fp32) {
     const double negl = 50.0 + std::log((double)(n > 1 ? n : 1));
     return negl < exact ? negl : exact;
 }
+
+// Narrow layout: for small N the 256 x 256 work items cannot fill the device (N = 1000 gives 16
+// of them, each lane looping over 256 columns); items of 64 rows x 256 columns whose four
+// wavefronts take 64 columns each give 4x the items and a quarter of the per-lane loop.
+constexpr int kNarrow = 64;
 
 PairGrid plan_pair_grid(int n, int K, int target_blocks);
 

@@ -144,6 +144,84 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
     }
 }
 
+// Narrow layout: one workgroup per (64-row group, 256-column chunk, vector), grid (chunks, row
+// groups, K). The four wavefronts share the 64 rows and take 64 columns each, so the per-lane
+// loop is 64 pairs long; their sums are added in LDS in a fixed order into one partial per
+// (chunk, row), the partial layout of pair_kernel.
+template <int D, bool GRAD, bool F32>
+__global__ __launch_bounds__(kRowsPerBlock) void pair_kernel_narrow(Events ev, const double *__restrict__ params,
+                                                                    PairGrid g, double *__restrict__ part) {
+    constexpr int ST = TileStride<D>::value;
+    constexpr int NACC = GRAD ? 6 : 2;
+    __shared__ double tile[kTile * ST];
+    __shared__ double red[3][NACC][kNarrow];
+
+    const int c = blockIdx.x, rg = blockIdx.y, k = blockIdx.z;
+    const double *pk = params + 6 * k;
+    const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
+    const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
+    const double cSx = -0.5 * sigma * sigma, nomega = -omega;
+    const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n = ev.n, npad = ev.npad;
+    const int i = rg * kNarrow + lane;
+    const int ic = min(i, n - 1);
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = ev.x[(size_t)d * npad + ic];
+    const double ti = ev.t[ic];
+    const int r0 = rg * kNarrow;
+    const double t_lo = ev.t[r0];
+    const double t_hi = ev.t[min(r0 + kNarrow - 1, n - 1)];
+
+    const int j0 = c * kTile;
+    const int cnt_all = min(kTile, n - j0);
+    if (tid < cnt_all) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) tile[tid * ST + d] = ev.x[(size_t)d * npad + j0 + tid];
+        tile[tid * ST + D] = ev.t[j0 + tid];
+    }
+    __syncthreads();
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int cnt = min(kNarrow, cnt_all - w * kNarrow); // this wavefront's columns (may be <= 0)
+    if (cnt > 0) {
+        const double *sub = tile + w * kNarrow * ST;
+        const double tf = sub[D], tl = sub[(cnt - 1) * ST + D];
+        int smode = tl < t_lo ? kSFull : (tf >= t_hi ? kSNone : kSMixed);
+        int dob = 1;
+        if (g.skip_zero) {
+            const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
+            if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
+            if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
+        }
+        switch (dob * 4 + smode) {
+        case 4 + kSNone: tile_pass<D, GRAD, true, kSNone, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+        case 4 + kSFull: tile_pass<D, GRAD, true, kSFull, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+        case 4 + kSMixed: tile_pass<D, GRAD, true, kSMixed, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+        case kSFull: tile_pass<D, GRAD, false, kSFull, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+        case kSMixed: tile_pass<D, GRAD, false, kSMixed, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
+        default: break; // every term of the sub-tile is skipped
+        }
+    }
+    constexpr int IDX[6] = {0, GRAD ? 1 : 3, 2, 3, 4, 5}; // accumulators kept: all 6, or sums B and S
+    if (w > 0) {
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) red[w - 1][a][lane] = acc[IDX[a]];
+    }
+    __syncthreads();
+    if (w == 0) {
+        double *out = part + ((size_t)k * g.chunks + c) * NACC * npad;
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) {
+            double v = acc[IDX[a]];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v += red[q][a][lane];
+            out[(size_t)a * npad + i] = v;
+        }
+    }
+}
+
 // Per-row terms: ll_i = log(lambda_i) + theta (e^{-omega (T - t_i)} - 1)
 //                       - mu0 (Phi(tauT (T - t_i)) - Phi(-tauT t_i))   (src/NewHawkes.hpp:652-655)
 // and d ll_i / d(sigma, tauX, tauT, omega, theta, mu0).
@@ -154,7 +232,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
                                                                  double *__restrict__ probs,
                                                                  double *__restrict__ inv_lam) {
     constexpr int NACC = GRAD ? 6 : 2;
-    __shared__ double2 sh[kRowsPerBlock];
+    __shared__ double2 sh[kResultWidth * kRowsPerBlock];
     const int k = blockIdx.y;
     const int rb = blockIdx.x;
     const int i = rb * kRowsPerBlock + threadIdx.x;
@@ -187,25 +265,28 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
 
 // One block per k: fixed-order double-double sum of the block partials, plus the
 // N (D-1) log(1/sqrt(2 pi)) constant (src/NewHawkes.hpp:659).
-__global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nblocks, int width,
+template <int W>
+__global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nblocks,
                                                                const double *__restrict__ block_out,
                                                                double *__restrict__ result) {
-    __shared__ double2 sh[kRowsPerBlock];
+    __shared__ double2 sh[W * kRowsPerBlock];
     const int k = blockIdx.x;
-    for (int q = 0; q < width; ++q) {
-        double2 a = make_double2(0.0, 0.0);
-        for (int b = threadIdx.x; b < nblocks; b += kRowsPerBlock) {
+    double2 a[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) a[q] = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < nblocks; b += kRowsPerBlock) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
             const double *o = block_out + (((size_t)k * nblocks + b) * kResultWidth + q) * 2;
-            a = dd_add(a, make_double2(o[0], o[1]));
+            a[q] = dd_add(a[q], make_double2(o[0], o[1]));
         }
-        double2 r = block_reduce_dd(a, sh);
-        if (threadIdx.x == 0) {
-            if (q == 0) {
-                const double c = -0.91893853320467274178 * (double)ev.n * (double)(ev.dim - 1);
-                r = dd_add(r, make_double2(c, 0.0));
-            }
-            result[(size_t)k * kResultWidth + q] = r.x + r.y;
-        }
+    }
+    block_reduce_dd_w<W>(a, sh);
+    if (threadIdx.x == 0) {
+        const double c = -0.91893853320467274178 * (double)ev.n * (double)(ev.dim - 1);
+        a[0] = dd_add(a[0], make_double2(c, 0.0));
+#pragma unroll
+        for (int q = 0; q < W; ++q) result[(size_t)k * kResultWidth + q] = a[q].x + a[q].y;
     }
 }
 
@@ -215,6 +296,19 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     PairGrid g;
     g.row_blocks = (n + kRowsPerBlock - 1) / kRowsPerBlock;
     const int max_chunks = (n + kTile - 1) / kTile;
+    g.skip_zero = 1;
+    g.zero_arg_b = background_skip_arg(n, false);
+    g.zero_arg_s = kZeroExpArg;
+    // Narrow layout when the widest split of 256 x 256 items reaches under a quarter of the
+    // requested grid: 64-row groups x 256-column chunks (the widest split's partials).
+    const long long full = (long long)K * g.row_blocks * max_chunks;
+    if (4 * full < target_blocks) {
+        g.narrow = 1;
+        g.chunk_len = kTile;
+        g.chunks = max_chunks;
+        return g;
+    }
+    g.narrow = 0;
     long long per_k = (long long)K * g.row_blocks;
     int want = (int)((target_blocks + per_k - 1) / per_k);
     if (want < 1) want = 1;
@@ -222,15 +316,22 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     const int per_chunk = (n + chunks - 1) / chunks;
     g.chunk_len = ((per_chunk + kTile - 1) / kTile) * kTile;
     g.chunks = (n + g.chunk_len - 1) / g.chunk_len;
-    g.skip_zero = 1;
-    g.zero_arg_b = background_skip_arg(n, false);
-    g.zero_arg_s = kZeroExpArg;
     return g;
 }
 
 template <int D, bool F32>
 static void launch_pair_d(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
                           double *part, hipStream_t s) {
+    if (g.narrow) {
+        dim3 grid((unsigned)g.chunks, (unsigned)((ev.n + kNarrow - 1) / kNarrow), (unsigned)K);
+        if (grad)
+            hipLaunchKernelGGL((pair_kernel_narrow<D, true, F32>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g,
+                               part);
+        else
+            hipLaunchKernelGGL((pair_kernel_narrow<D, false, F32>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g,
+                               part);
+        return;
+    }
     dim3 grid((unsigned)(g.row_blocks * g.chunks), (unsigned)K);
     if (grad)
         hipLaunchKernelGGL((pair_kernel<D, true, F32>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part);
@@ -275,8 +376,12 @@ void launch_finalize(const Events &ev, const double *params, int K, const PairGr
 
 void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,
                    hipStream_t s) {
-    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, nblocks,
-                       grad ? kResultWidth : 1, block_out, result);
+    if (grad)
+        hipLaunchKernelGGL((reduce_kernel<kResultWidth>), dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, nblocks,
+                           block_out, result);
+    else
+        hipLaunchKernelGGL((reduce_kernel<1>), dim3((unsigned)K), dim3(kRowsPerBlock), 0, s, ev, nblocks, block_out,
+                           result);
 }
 
 } // namespace kern

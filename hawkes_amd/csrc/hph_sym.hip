@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
                                                                      double *__restrict__ inv_lam) {
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
-    __shared__ double2 sh[kRowsPerBlock];
+    __shared__ double2 sh[kResultWidth * kRowsPerBlock];
     const int4 it = L.fin_item[blockIdx.x];
     const SymData &ds = L.data[it.x];
     const int k = it.y, fb = it.z;
@@ -439,40 +439,47 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
         if (inv_lam != nullptr && it.x == 0 && k == 0) inv_lam[i] = 1.0 / lam;
     }
     const size_t blk = (size_t)ds.blk0 + (size_t)k * ds.fin_blocks + fb;
-    for (int q = 0; q < (GRAD ? 7 : 1); ++q) {
-        const double2 r = block_reduce_dd(make_double2(v[q], 0.0), sh);
-        if (threadIdx.x == 0) {
+    constexpr int W = GRAD ? 7 : 1;
+    double2 r[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) r[q] = make_double2(v[q], 0.0);
+    block_reduce_dd_w<W>(r, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
             double *o = block_out + (blk * kResultWidth + q) * 2;
-            o[0] = r.x;
-            o[1] = r.y;
+            o[0] = r[q].x;
+            o[1] = r[q].y;
         }
     }
 }
 
 // One workgroup per (dataset, vector): fixed-order double-double sum of its block partials plus
 // the N (D-1) log(1/sqrt(2 pi)) constant (src/NewHawkes.hpp:659).
-__global__ __launch_bounds__(kRowsPerBlock) void sym_reduce_kernel(SymLaunch L, int width,
-                                                                   const double *__restrict__ block_out,
+template <int W>
+__global__ __launch_bounds__(kRowsPerBlock) void sym_reduce_kernel(SymLaunch L, const double *__restrict__ block_out,
                                                                    double *__restrict__ result) {
-    __shared__ double2 sh[kRowsPerBlock];
+    __shared__ double2 sh[W * kRowsPerBlock];
     const int2 it = L.res_item[blockIdx.x];
     const SymData &ds = L.data[it.x];
     const int k = it.y;
     const size_t blk0 = (size_t)ds.blk0 + (size_t)k * ds.fin_blocks;
-    for (int q = 0; q < width; ++q) {
-        double2 a = make_double2(0.0, 0.0);
-        for (int b = threadIdx.x; b < ds.fin_blocks; b += kRowsPerBlock) {
+    double2 a[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) a[q] = make_double2(0.0, 0.0);
+    for (int b = threadIdx.x; b < ds.fin_blocks; b += kRowsPerBlock) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
             const double *o = block_out + ((blk0 + b) * kResultWidth + q) * 2;
-            a = dd_add(a, make_double2(o[0], o[1]));
+            a[q] = dd_add(a[q], make_double2(o[0], o[1]));
         }
-        double2 r = block_reduce_dd(a, sh);
-        if (threadIdx.x == 0) {
-            if (q == 0) {
-                const double c = -0.91893853320467274178 * (double)ds.n * (double)(L.dim - 1);
-                r = dd_add(r, make_double2(c, 0.0));
-            }
-            result[(size_t)(ds.res0 + k) * kResultWidth + q] = r.x + r.y;
-        }
+    }
+    block_reduce_dd_w<W>(a, sh);
+    if (threadIdx.x == 0) {
+        const double c = -0.91893853320467274178 * (double)ds.n * (double)(L.dim - 1);
+        a[0] = dd_add(a[0], make_double2(c, 0.0));
+#pragma unroll
+        for (int q = 0; q < W; ++q) result[(size_t)(ds.res0 + k) * kResultWidth + q] = a[q].x + a[q].y;
     }
 }
 
@@ -623,8 +630,12 @@ void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, co
 }
 
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s) {
-    hipLaunchKernelGGL(sym_reduce_kernel, dim3((unsigned)L.res_items), dim3(kRowsPerBlock), 0, s, L,
-                       grad ? kResultWidth : 1, block_out, result);
+    if (grad)
+        hipLaunchKernelGGL((sym_reduce_kernel<kResultWidth>), dim3((unsigned)L.res_items), dim3(kRowsPerBlock), 0, s,
+                           L, block_out, result);
+    else
+        hipLaunchKernelGGL((sym_reduce_kernel<1>), dim3((unsigned)L.res_items), dim3(kRowsPerBlock), 0, s, L,
+                           block_out, result);
 }
 
 } // namespace kern

@@ -21,7 +21,8 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-6
 
 CONFIGS = {"rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096), "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
-           "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024)}
+           "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
 
 
 def fp32_engine(locs, times, params, kernel):
@@ -42,7 +43,7 @@ def errors(e, ref):
     return float(ll_err), float(g_err.max()), float(p_err.max())
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow"])
 def test_fp32_cluster(kernel):
     locs, times = synth.cluster_process(3500, seed=123)
     ref = oracle.total(locs, times, synth.P0, probs=True)

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 KERNELS = {"rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096), "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
-           "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024)}
+           "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
 
 
 def engine(locs, times, params, flags=_lib.OPENCL, kernel="rowwise"):
@@ -86,7 +87,7 @@ def test_unsorted_times_and_ties(kernel):
     check(locs, times, [1.1, 0.8, 0.02, 0.3, 0.9, 0.7], "ties", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow"])
 def test_row_counts_around_block_sizes(kernel):
     rng = np.random.default_rng(305)
     for n in (1, 2, 63, 255, 256, 257, 511, 512, 513, 1023, 1025, 1537):

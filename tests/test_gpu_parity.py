@@ -26,7 +26,8 @@ PROB_TOL = 1e-12
 KERNELS = {"auto": (_lib.PAIR_KERNEL_AUTO, 2, 4096), "rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096),
            "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
            "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
-           "rowwise_small_grid": (_lib.PAIR_KERNEL_ROWWISE, 2, 64)}
+           "rowwise_small_grid": (_lib.PAIR_KERNEL_ROWWISE, 2, 64),
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
 
 
 def make_engine(locs, times, params, kernel="auto", flags=_lib.OPENCL):
@@ -83,14 +84,15 @@ def test_uniform_n1000_config(kernel):
     assert_matches_oracle(locs, times, p, name="uniform_n1000", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_small_grid"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_small_grid", "rowwise_narrow"])
 def test_cluster_process_small(kernel):
     locs, times = synth.cluster_process(2800, side=np.sqrt(4000 / 100), seed=777)
     assert_matches_oracle(locs, times, synth.P0, name="cluster_4k", kernel=kernel)
 
 
 @pytest.mark.parametrize("D,kernel", [(1, "rowwise"), (3, "rowwise"), (5, "rowwise"), (8, "rowwise"),
-                                      (1, "symmetric"), (3, "symmetric")])
+                                      (1, "symmetric"), (3, "symmetric"), (1, "rowwise_narrow"),
+                                      (5, "rowwise_narrow")])
 def test_dimensions(D, kernel):
     rng = np.random.default_rng(100 + D)
     n = 1700
@@ -100,7 +102,7 @@ def test_dimensions(D, kernel):
     assert_matches_oracle(locs, times, p, name="D%d" % D, kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow"])
 def test_unsorted_times_and_ties(kernel):
     rng = np.random.default_rng(21)
     n = 1900
@@ -110,7 +112,7 @@ def test_unsorted_times_and_ties(kernel):
     assert_matches_oracle(locs, times, p, name="unsorted_ties", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow"])
 def test_row_counts_around_block_sizes(kernel):
     rng = np.random.default_rng(4)
     for n in (1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1025, 1537):
@@ -119,7 +121,7 @@ def test_row_counts_around_block_sizes(kernel):
         assert_matches_oracle(locs, times, [1.0, 0.9, 0.1, 0.5, 0.6, 0.9], name="n%d" % n, kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow"])
 def test_batch_equals_single_calls_bitwise(kernel):
     locs, times = synth.cluster_process(1500, seed=31)
     P = synth.jitter_params(8)
