@@ -163,9 +163,11 @@ class SymRunner {
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
-                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, bool fast,
-                 double *probs_dev, double *inv_lam_dev, double *result, hipStream_t s, hipEvent_t ev0 = nullptr,
+                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
+                 const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
+                 hipStream_t s, hipEvent_t ev0 = nullptr,
                  hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
+        if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
         const kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
@@ -183,7 +185,7 @@ class SymRunner {
     // gradient evaluation, shared with it): w = 1/lambda (device, sorted), out = N x D (device).
     void enqueue_locgrad(const DataRef &ref, int dim, const double *params, const double *w, double *out,
                          int rows_per_lane, int target_items, int skip_zero, hipStream_t s) {
-        const kern::SymLaunch L = prepare({ref}, {1}, dim, true, false, rows_per_lane, target_items, skip_zero, false);
+        const kern::SymLaunch L = prepare({ref}, {1}, dim, true, false, rows_per_lane, target_items, skip_zero, {0});
         kern::launch_sym_locgrad(L, params, w, d_row_.as<double>(), d_col_.as<double>(), out, s);
         check(hipGetLastError(), "symmetric location gradient launch");
     }
@@ -192,8 +194,10 @@ class SymRunner {
 
   private:
     // Plans (if the shape changed), uploads the tables (if they changed) and fills the launch.
+    // fast[d]: dataset d runs the base-2 form.
     kern::SymLaunch prepare(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, bool grad,
-                            bool fp32, int rows_per_lane, int target_items, int skip_zero, bool fast) {
+                            bool fp32, int rows_per_lane, int target_items, int skip_zero,
+                            const std::vector<char> &fast) {
         std::vector<int> n(refs.size());
         for (size_t d = 0; d < refs.size(); ++d) n[d] = refs[d].n;
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
@@ -230,11 +234,13 @@ class SymRunner {
             refs_same = refs_[d].x == refs[d].x && refs_[d].t == refs[d].t && refs_[d].perm == refs[d].perm &&
                         refs_[d].npad == refs[d].npad && refs_[d].t_last == refs[d].t_last;
         for (size_t d = 0; refs_same && d < refs.size(); ++d)
-            refs_same = plan_.data[d].zero_arg_b == kern::background_skip_arg(refs[d].n, fp32);
+            refs_same = plan_.data[d].zero_arg_b == kern::background_skip_arg(refs[d].n, fp32) &&
+                        plan_.data[d].fast == (fast[d] ? 1 : 0);
         if (!refs_same) {
             for (size_t d = 0; d < refs.size(); ++d) {
                 kern::SymData &sd = plan_.data[d];
                 sd.zero_arg_b = kern::background_skip_arg(refs[d].n, fp32);
+                sd.fast = fast[d] ? 1 : 0;
                 sd.x = refs[d].x;
                 sd.t = refs[d].t;
                 sd.perm = refs[d].perm;
@@ -263,7 +269,8 @@ class SymRunner {
         L.R = plan_.R;
         L.skip_zero = skip_zero;
         L.zero_arg_s = kern::kZeroExpArg;
-        L.fast = fast ? 1 : 0;
+        L.fast = 0;
+        for (char f : fast) L.fast |= f ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
@@ -800,7 +807,7 @@ class HipHawkes : public AbstractHawkes {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             last_fast_ = !force_natural_ && fast_ok(extents_, params, K);
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, last_fast_, probs_dev, w_dev,
+                         cfg.target_blocks, skip_zero, {last_fast_ ? (char)1 : (char)0}, probs_dev, w_dev,
                          d_result_.as<double>(), stream_,
                          timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
@@ -1028,8 +1035,8 @@ class HipHawkesBatch {
         d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
-        const bool fast = fast_all();
-        if (graph_exec_ && fast != graph_fast_) drop_graph(); // the graph holds one kernel variant
+        const std::vector<char> fast = fast_flags();
+        if (graph_exec_ && fast != graph_fast_) drop_graph(); // the graph holds one variant and table
         if (graph_mode_) {
             if (!graph_exec_) {
                 enqueue(false, fast); // plans, allocates and uploads the tables outside the capture
@@ -1063,17 +1070,17 @@ class HipHawkesBatch {
     double lastMs() const { return last_ms_; }
 
   private:
-    // True when every dataset and vector of the current parameters allows the base-2 pass form.
-    bool fast_all() const {
-        bool fast = true;
+    // Per dataset: whether all its vectors of the current parameters allow the base-2 form.
+    std::vector<char> fast_flags() const {
+        std::vector<char> fast(counts_.size());
         const double *hp = h_params_.as<double>();
         for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
-            fast = fast && fast_ok(ext_[d], hp + 6 * k0, K_[d]);
+            fast[d] = fast_ok(ext_[d], hp + 6 * k0, K_[d]) ? 1 : 0;
         return fast;
     }
 
     // Parameter upload, pass, finalize, reduction and result download on the batch's stream.
-    void enqueue(bool events, bool fast) {
+    void enqueue(bool events, const std::vector<char> &fast) {
         check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)total_, hipMemcpyHostToDevice,
                              stream_),
               "upload params");
@@ -1109,7 +1116,8 @@ class HipHawkesBatch {
     std::vector<bool> have_;
     std::vector<int> K_;
     int total_ = 0;
-    bool have_params_ = false, grad_ = true, graph_mode_ = false, graph_fast_ = false;
+    bool have_params_ = false, grad_ = true, graph_mode_ = false;
+    std::vector<char> graph_fast_;
     int A_ = 2, target_ = 8192;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

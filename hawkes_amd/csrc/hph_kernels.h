@@ -112,6 +112,7 @@ struct SymData {
     int fin_blocks;         // ceil(n / 256)
     double t_last;          // the caller's last time (T of the model)
     double zero_arg_b;      // background skip point of this dataset (background_skip_arg)
+    int fast;               // 1: this dataset's items run the base-2 form (SymLaunch::fast set)
     long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
     long long col0, col_kstride; // column partials: the same
     long long blk0;         // index of its (k = 0, block 0) block partial
@@ -127,8 +128,8 @@ struct SymLaunch {
     int dim, A, R;
     int skip_zero;          // as PairGrid::skip_zero (background point: SymData::zero_arg_b)
     double zero_arg_s;      // as PairGrid::zero_arg_s
-    int fast;               // 1: base-2 form (scaled times, clamp-free exp2); the host has checked
-                            // that exponents stay below 1e9 and scaled times keep their order
+    int fast;               // 1: some dataset runs the base-2 form (scaled times, clamp-free exp2;
+                            // SymData::fast); the host has checked its exponents, order and spans
     const int4 *fin_item;   // (dataset, k, finalize block, 0)
     int fin_items;
     const int2 *res_item;   // (dataset, k)

@@ -177,17 +177,189 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
     }
 }
 
-// Persistent pass: every workgroup takes work items from the queue until it is empty.
-// Three waves per SIMD (at most 168 VGPRs): measured slightly faster than the two that the
-// unconstrained allocation (up to 228 VGPRs for 2 rows per lane) leaves room for.
+// One work item of the symmetric pass: row block P of dataset it.x, vector it.w, against a chunk
+// of earlier columns (it.z >= 0) or its own diagonal block (it.z < 0). FAST: the base-2 form.
 template <int D, bool GRAD, int A, bool F32, bool FAST>
-__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void sym_kernel(SymLaunch L, const double *__restrict__ params,
-                                                             double *__restrict__ rowpart,
-                                                             double *__restrict__ colpart) {
+__device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
+                                         double *__restrict__ rowpart, double *__restrict__ colpart,
+                                         double *__restrict__ tile, double *__restrict__ colbuf, int tid, int lane,
+                                         int w) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int R = kSymWaves * 64 * A;
+    const SymData &ds = L.data[it.x];
+    const int P = it.y, chunk = it.z, k = it.w;
+    const double *pk = params + 6 * (size_t)(ds.param0 + k);
+    const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
+    const double *__restrict__ ex = ds.x;
+    const double *__restrict__ et = ds.t;
+    const int n = ds.n, npad = ds.npad;
+    const int P0 = P * R;
+
+    // base-2 form: times relative to the block's first event, scaled by st
+    const double st = FAST ? time_scale(tauT) : 1.0;
+    const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
+    PairConst c;
+    if (FAST) {
+        c.cB0 = tauX * tauX * (0.5 * kLog2e);
+        c.cB1 = 0.0;
+        c.cS0 = -sigma * sigma * (0.5 * kLog2e);
+        c.cS1 = -omega * kLog2e / st;
+    } else {
+        c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
+        c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);
+        c.cS0 = -0.5 * sigma * sigma;
+        c.cS1 = -omega;
+    }
+    // skip tests: a tile is skipped when the smallest time gap alone drives every background
+    // exponent past the negligible point or every excitation exponent past the underflow
+    // point (in the units of the form)
+    const double zb = FAST ? ds.zero_arg_b * kLog2e : ds.zero_arg_b;
+    const double zs = FAST ? L.zero_arg_s * kLog2e : L.zero_arg_s;
+    const double gb = FAST ? 1.0 : 0.5 * tauT * tauT; // background exponent per gap^2
+    const double gs = FAST ? -c.cS1 : omega;          // excitation exponent per gap
+    const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
+    double xi[A][D], ti[A], vm[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int r = wrow0 + a * 64 + lane;
+        const int rc = min(r, n - 1);
+#pragma unroll
+        for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
+        ti[a] = FAST ? (et[rc] - t0) * st : et[rc];
+        vm[a] = r < n ? 1.0 : 0.0;
+    }
+    const bool wave_live = wrow0 < n;
+    const double t_lo = FAST ? (et[min(wrow0, n - 1)] - t0) * st : et[min(wrow0, n - 1)];
+
+    double acc[A][6];
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[a][q] = 0.0;
+
+    int slot;
+    if (chunk < 0) {
+        slot = ds.max_chunks;
+        const int c1 = min(P0 + R, n);
+        for (int j0 = P0; j0 < c1; j0 += kTile) {
+            const int cnt = min(kTile, c1 - j0);
+            __syncthreads();
+            if (tid < cnt) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) tile[tid * ST + d] = ex[(size_t)d * npad + j0 + tid];
+                tile[tid * ST + D] = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
+            }
+            __syncthreads();
+            if (wave_live) {
+#pragma unroll
+                for (int a = 0; a < A; ++a)
+                    diag_pass<D, GRAD, F32, FAST>(tile, cnt, xi[a], ti[a], c, acc[a]);
+            }
+        }
+    } else {
+        slot = chunk;
+        const int c0 = chunk * ds.L;
+        const int c1 = min(c0 + ds.L, P0);
+        const bool tail = P0 + R > n;
+        double *cp = colpart + ds.col0 + (size_t)k * ds.col_kstride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
+        // the block's first row: the smallest time gap to any earlier tile is its gap
+        const double t_blk = FAST ? (et[min(P0, n - 1)] - t0) * st : et[min(P0, n - 1)];
+        for (int j0 = c0; j0 < c1; j0 += kTile) {
+            // c1 - c0 and P0 are multiples of kTile: every tile here is full.
+            if (L.skip_zero) {
+                // Workgroup-wide skip, decided as the per-wave test below decides it for the
+                // first wave (whose gap is the smallest): when neither term survives there, it
+                // survives for no row of the block; the tile's column partials are then zero.
+                const double tl = FAST ? (et[j0 + kTile - 1] - t0) * st : et[j0 + kTile - 1];
+                const double gap = t_blk - tl;
+                if (tl < t_blk && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cp[(size_t)c * P0 + j0 + tid] = 0.0;
+                    continue;
+                }
+            }
+            __syncthreads();
+            {
+                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
+                // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
+                const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const double v = ex[(size_t)d * npad + j0 + tid];
+                    tile[e0 + d * 128] = v;
+                    tile[e0 + d * 128 + 64] = v;
+                }
+                const double tv = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
+                tile[e0 + D * 128] = tv;
+                tile[e0 + D * 128 + 64] = tv;
+            }
+            __syncthreads();
+            if (wave_live) {
+                // columns are earlier than the rows: tl <= t_lo; equal times need the mask
+                const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
+                int smode = tl < t_lo ? kSFull : kSMixed;
+                int dob = 1;
+                if (L.skip_zero) {
+                    const double gap = tl < t_lo ? t_lo - tl : 0.0;
+                    if (gb * gap * gap >= zb) dob = 0;
+                    if (smode == kSFull && gs >= 0.0 && gs * gap >= zs) smode = kSNone;
+                }
+                const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
+                for (int sb = 0; sb < 4; ++sb) {
+                    double ca[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) ca[c] = 0.0;
+                    const double *sub = tile + sb * (D + 1) * 128;
+                    if (tail)
+                        sym_dispatch<D, GRAD, A, true, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
+                    else
+                        sym_dispatch<D, GRAD, A, false, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
+                }
+            } else {
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double v = colbuf[(0 * NC + c) * kTile + tid];
+#pragma unroll
+                for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
+                cp[(size_t)c * P0 + j0 + tid] = v;
+            }
+        }
+    }
+    const size_t rpad = (size_t)ds.nb * R;
+    double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+        const int r = wrow0 + a * 64 + lane;
+        if (GRAD) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
+        } else {
+            rp[r] = acc[a][0];
+            rp[rpad + r] = acc[a][3];
+        }
+    }
+}
+
+// Persistent pass: every workgroup takes work items from the queue until it is empty. The FAST
+// instantiation runs each item in the base-2 form where its dataset allows it (SymData::fast)
+// and in the natural form otherwise.
+// Three waves per SIMD (at most 168 VGPRs): measured slightly faster than the two that the
+// unconstrained allocation (up to 228 VGPRs for 2 rows per lane) leaves room for.
+template <int D, bool GRAD, int A, bool F32, bool FAST>
+__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void sym_kernel(
+    SymLaunch L, const double *__restrict__ params, double *__restrict__ rowpart, double *__restrict__ colpart) {
+    constexpr int ST = TileStride<D>::value;
+    constexpr int NC = GRAD ? 3 : 1;
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x (D + 1) x 128 doubles
     __shared__ double tile[(2 * kTile * (D + 1) > kTile * ST) ? 2 * kTile * (D + 1) : kTile * ST];
     __shared__ double colbuf[kSymWaves * NC * kTile];
@@ -200,168 +372,11 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         const int idx = s_item;
         __syncthreads();
         if (idx >= L.items) break;
-
         const int4 it = L.item[idx];
-        const SymData &ds = L.data[it.x];
-        const int P = it.y, chunk = it.z, k = it.w;
-        const double *pk = params + 6 * (size_t)(ds.param0 + k);
-        const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3];
-        const double *__restrict__ ex = ds.x;
-        const double *__restrict__ et = ds.t;
-        const int n = ds.n, npad = ds.npad;
-        const int P0 = P * R;
-
-        // base-2 form: times relative to the block's first event, scaled by st
-        const double st = FAST ? time_scale(tauT) : 1.0;
-        const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
-        PairConst c;
-        if (FAST) {
-            c.cB0 = tauX * tauX * (0.5 * kLog2e);
-            c.cB1 = 0.0;
-            c.cS0 = -sigma * sigma * (0.5 * kLog2e);
-            c.cS1 = -omega * kLog2e / st;
-        } else {
-            c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
-            c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);
-            c.cS0 = -0.5 * sigma * sigma;
-            c.cS1 = -omega;
-        }
-        // skip tests: a tile is skipped when the smallest time gap alone drives every background
-        // exponent past the negligible point or every excitation exponent past the underflow
-        // point (in the units of the form)
-        const double zb = FAST ? ds.zero_arg_b * kLog2e : ds.zero_arg_b;
-        const double zs = FAST ? L.zero_arg_s * kLog2e : L.zero_arg_s;
-        const double gb = FAST ? 1.0 : 0.5 * tauT * tauT; // background exponent per gap^2
-        const double gs = FAST ? -c.cS1 : omega;          // excitation exponent per gap
-        const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
-        double xi[A][D], ti[A], vm[A];
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-            const int r = wrow0 + a * 64 + lane;
-            const int rc = min(r, n - 1);
-#pragma unroll
-            for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
-            ti[a] = FAST ? (et[rc] - t0) * st : et[rc];
-            vm[a] = r < n ? 1.0 : 0.0;
-        }
-        const bool wave_live = wrow0 < n;
-        const double t_lo = FAST ? (et[min(wrow0, n - 1)] - t0) * st : et[min(wrow0, n - 1)];
-
-        double acc[A][6];
-#pragma unroll
-        for (int a = 0; a < A; ++a)
-#pragma unroll
-            for (int q = 0; q < 6; ++q) acc[a][q] = 0.0;
-
-        int slot;
-        if (chunk < 0) {
-            slot = ds.max_chunks;
-            const int c1 = min(P0 + R, n);
-            for (int j0 = P0; j0 < c1; j0 += kTile) {
-                const int cnt = min(kTile, c1 - j0);
-                __syncthreads();
-                if (tid < cnt) {
-#pragma unroll
-                    for (int d = 0; d < D; ++d) tile[tid * ST + d] = ex[(size_t)d * npad + j0 + tid];
-                    tile[tid * ST + D] = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
-                }
-                __syncthreads();
-                if (wave_live) {
-#pragma unroll
-                    for (int a = 0; a < A; ++a)
-                        diag_pass<D, GRAD, F32, FAST>(tile, cnt, xi[a], ti[a], c, acc[a]);
-                }
-            }
-        } else {
-            slot = chunk;
-            const int c0 = chunk * ds.L;
-            const int c1 = min(c0 + ds.L, P0);
-            const bool tail = P0 + R > n;
-            double *cp = colpart + ds.col0 + (size_t)k * ds.col_kstride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
-            // the block's first row: the smallest time gap to any earlier tile is its gap
-            const double t_blk = FAST ? (et[min(P0, n - 1)] - t0) * st : et[min(P0, n - 1)];
-            for (int j0 = c0; j0 < c1; j0 += kTile) {
-                // c1 - c0 and P0 are multiples of kTile: every tile here is full.
-                if (L.skip_zero) {
-                    // Workgroup-wide skip, decided as the per-wave test below decides it for the
-                    // first wave (whose gap is the smallest): when neither term survives there, it
-                    // survives for no row of the block; the tile's column partials are then zero.
-                    const double tl = FAST ? (et[j0 + kTile - 1] - t0) * st : et[j0 + kTile - 1];
-                    const double gap = t_blk - tl;
-                    if (tl < t_blk && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs) {
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) cp[(size_t)c * P0 + j0 + tid] = 0.0;
-                        continue;
-                    }
-                }
-                __syncthreads();
-                {
-                    // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
-                    // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
-                    const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        const double v = ex[(size_t)d * npad + j0 + tid];
-                        tile[e0 + d * 128] = v;
-                        tile[e0 + d * 128 + 64] = v;
-                    }
-                    const double tv = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
-                    tile[e0 + D * 128] = tv;
-                    tile[e0 + D * 128 + 64] = tv;
-                }
-                __syncthreads();
-                if (wave_live) {
-                    // columns are earlier than the rows: tl <= t_lo; equal times need the mask
-                    const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
-                    int smode = tl < t_lo ? kSFull : kSMixed;
-                    int dob = 1;
-                    if (L.skip_zero) {
-                        const double gap = tl < t_lo ? t_lo - tl : 0.0;
-                        if (gb * gap * gap >= zb) dob = 0;
-                        if (smode == kSFull && gs >= 0.0 && gs * gap >= zs) smode = kSNone;
-                    }
-                    const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
-                    for (int sb = 0; sb < 4; ++sb) {
-                        double ca[NC];
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) ca[c] = 0.0;
-                        const double *sub = tile + sb * (D + 1) * 128;
-                        if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
-                        else
-                            sym_dispatch<D, GRAD, A, false, F32, FAST>(mode, sub, lane, xi, ti, vm, c, acc, ca);
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
-                    }
-                } else {
-#pragma unroll
-                    for (int sb = 0; sb < 4; ++sb)
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = 0.0;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    double v = colbuf[(0 * NC + c) * kTile + tid];
-#pragma unroll
-                    for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
-                    cp[(size_t)c * P0 + j0 + tid] = v;
-                }
-            }
-        }
-        const size_t rpad = (size_t)ds.nb * R;
-        double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;
-#pragma unroll
-        for (int a = 0; a < A; ++a) {
-            const int r = wrow0 + a * 64 + lane;
-            if (GRAD) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
-            } else {
-                rp[r] = acc[a][0];
-                rp[rpad + r] = acc[a][3];
-            }
-        }
+        if (FAST && L.data[it.x].fast)
+            sym_item<D, GRAD, A, F32, true>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
+        else
+            sym_item<D, GRAD, A, F32, false>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
     }
 }
 
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
             s[3] = a6[3];
             s[4] = a6[4];
             s[5] = a6[5];
-            if (L.fast) { // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt
+            if (ds.fast) { // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt
                 const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
                 s[2] /= st * st;
                 s[5] /= st;
