@@ -15,8 +15,9 @@ Prints one JSON line. Options: --steps K --warmup W --gpus N (only 1 is supporte
 
 value counts the N^2 ordered pairs the reference forms per call; the engine forms each
 background pair once for both events and skips tiles whose background terms are all below
-e^-(50 + ln N) or whose excitation terms are all exactly zero (roofline.pairs_formed gives the
-pairs actually formed, all_pairs the rate with no skipping).
+e^-(50 + ln N), or whose excitation terms are all below e^-X_S (excitation_skip_arg; rates change
+by < 2e-22 relative) (roofline.pairs_formed gives the pairs actually formed, all_pairs the rate
+with no skipping).
 """
 import argparse
 import json
@@ -49,6 +50,19 @@ def background_skip_arg(n):
     return min(ZERO_ARG, 50.0 + np.log(max(n, 1)))
 
 
+def excitation_skip_arg(params, n, D=2):
+    """Excitation skip point of the kernels in likelihood / gradient passes: min(746,
+    50 + ln N + max(0, ln(|cS| / |cB|))), cB = mu0 tauX^D tauT / (2 pi), cS = theta omega sigma^D /
+    sqrt(2 pi) (skipped terms change every event's rate by < 2e-22 relative)."""
+    sigma, tauX, tauT, omega, theta, mu0 = params
+    cB = abs(mu0 * tauX ** D * tauT) / (2 * np.pi)
+    cS = abs(theta * omega * sigma ** D) / np.sqrt(2 * np.pi)
+    if not cB > 0:
+        return ZERO_ARG
+    r = np.log(cS / cB) if cS > 0 else 0.0
+    return min(ZERO_ARG, 50.0 + np.log(max(n, 1)) + max(0.0, r))
+
+
 def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256):
     """Background and excitation pairs the pair kernel forms for sorted times: per (wavefront of
     64 rows, tile of 256 columns), the tile's background is skipped where every term is
@@ -58,6 +72,7 @@ def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256):
     t = np.sort(np.asarray(times))
     n = t.size
     zero_b = background_skip_arg(n)
+    zero_s = excitation_skip_arg(params, n)
     w0 = np.arange(0, n, rows_per_wave)
     w_lo, w_hi = t[w0], t[np.minimum(w0 + rows_per_wave - 1, n - 1)]
     rows = np.minimum(w0 + rows_per_wave, n) - w0
@@ -73,7 +88,7 @@ def pairs_formed(times, params, skip_zero, rows_per_wave=64, tile=256):
                        np.where(t_f[None, :] > w_hi[:, None], t_f[None, :] - w_hi[:, None], 0.0))
         b_on = 0.5 * params[2] ** 2 * gap * gap < zero_b
         if params[3] >= 0:
-            s_on &= ~(before & (params[3] * gap >= ZERO_ARG))
+            s_on &= ~(before & (params[3] * gap >= zero_s))
     # the kernel runs whole 64-lane wavefronts; count lanes of real rows only
     pair_counts = rows[:, None] * cols[None, :]
     return float((pair_counts * b_on).sum()), float((pair_counts * s_on).sum())
@@ -88,6 +103,7 @@ def pairs_formed_symmetric(times, params, skip_zero, waves=4, rows_per_lane=2, t
     t = np.sort(np.asarray(times))
     n = t.size
     zero_b = background_skip_arg(n)
+    zero_s = excitation_skip_arg(params, n)
     wrows = 64 * rows_per_lane
     R = waves * wrows
     nb = (n + R - 1) // R
@@ -107,7 +123,7 @@ def pairs_formed_symmetric(times, params, skip_zero, waves=4, rows_per_lane=2, t
     if skip_zero:
         b_on &= 0.5 * params[2] ** 2 * gap * gap < zero_b
         if params[3] >= 0:
-            s_on &= ~(before & (params[3] * gap >= ZERO_ARG))
+            s_on &= ~(before & (params[3] * gap >= zero_s))
     nB = float((rows[:, None] * tile * b_on).sum())
     nS = float((rows[:, None] * tile * s_on).sum())
     for P in range(nb):

@@ -163,12 +163,13 @@ class SymRunner {
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
-                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
+                 bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, int cut_s,
                  const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
                  hipStream_t s, hipEvent_t ev0 = nullptr,
                  hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
         if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
-        const kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
+        kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
+        L.cut_s = skip_zero && cut_s ? 1 : 0;
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "symmetric pair kernel launch");
@@ -269,6 +270,7 @@ class SymRunner {
         L.R = plan_.R;
         L.skip_zero = skip_zero;
         L.zero_arg_s = kern::kZeroExpArg;
+        L.cut_s = 0;
         L.fast = 0;
         for (char f : fast) L.fast |= f ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
@@ -807,7 +809,8 @@ class HipHawkes : public AbstractHawkes {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             last_fast_ = !force_natural_ && fast_ok(extents_, params, K);
             sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, {last_fast_ ? (char)1 : (char)0}, probs_dev, w_dev,
+                         cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0}, probs_dev,
+                         w_dev,
                          d_result_.as<double>(), stream_,
                          timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
         } else {
@@ -816,6 +819,7 @@ class HipHawkes : public AbstractHawkes {
             g.skip_zero = skip_zero;
             g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
             g.zero_arg_s = kern::kZeroExpArg;
+            g.cut_s = skip_zero && !exact_s_ ? 1 : 0;
             const int nacc = grad ? 6 : 2;
             d_part_.ensure(sizeof(double) * (size_t)K * g.chunks * nacc * npad_);
             if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
@@ -849,10 +853,20 @@ class HipHawkes : public AbstractHawkes {
         last_ms_[1] = b;
     }
 
+    // Single-vector evaluation; caches ll (and the gradient). The probabilities it writes are kept
+    // only when the pass skipped exact zeros alone: a request for probabilities runs such a pass
+    // (excitation_skip_arg bounds the cut absolutely, not relatively, for tiny probabilities).
     void evaluate_single(bool grad, double *probs_host) {
         if (!have_params_) throw NotInitialized("parameters are not set (setParameters)");
-        run(params_, 1, grad, probs_host, true);
-        probs_valid_ = true;
+        exact_s_ = probs_host != nullptr || (flags & ALL_PAIRS);
+        try {
+            run(params_, 1, grad, probs_host, true);
+        } catch (...) {
+            exact_s_ = false;
+            throw;
+        }
+        probs_valid_ = exact_s_;
+        exact_s_ = false;
         // the location gradient takes 1/lambda from natural-form FP64 passes only: its components
         // cancel between neighbouring events, so it needs rates to about 1e-15
         w_valid_ = !fp32_ && !last_fast_;
@@ -888,7 +902,7 @@ class HipHawkes : public AbstractHawkes {
 
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
     bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
-    bool force_natural_ = false, last_fast_ = false;
+    bool force_natural_ = false, last_fast_ = false, exact_s_ = false;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
     double stored_params_[6] = {0, 0, 0, 0, 0, 0};
@@ -1089,7 +1103,7 @@ class HipHawkesBatch {
             refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
                                          d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
         const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
-        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, fast, nullptr, nullptr,
+        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, 1, fast, nullptr, nullptr,
                         d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
                         events ? ev_[2] : nullptr);
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,

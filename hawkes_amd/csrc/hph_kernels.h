@@ -35,6 +35,7 @@ struct PairGrid {
     int skip_zero;     // 1: skip tiles whose terms are all negligible (background) or zero (excitation)
     double zero_arg_b; // background tiles whose every exponent is <= -zero_arg_b are skipped
     double zero_arg_s; // excitation exp arguments <= -zero_arg_s give exactly +0.0
+    int cut_s;         // 1: excitation tiles below the negligible point are skipped as well
 };
 
 // exp(x) is exactly +0.0 in FP64 for x <= -745.14 (both here and in the reference's exp);
@@ -54,6 +55,29 @@ inline double background_skip_arg(long long n, bool fp32) {
     const double exact = fp32 ? kZeroExpArg32 : kZeroExpArg;
     const double negl = 50.0 + std::log((double)(n > 1 ? n : 1));
     return negl < exact ? negl : exact;
+}
+
+// Excitation tiles may be skipped where every term is below e^-X_S, X_S = min(exact-zero point,
+// 50 + ln n + max(0, ln(|cS| / |cB|))) with cB = mu0 tauX^D tauT / (2 pi) and
+// cS = theta omega sigma^D / sqrt(2 pi) (the constants of lambda = cB sum eB + cS sum eS). Bound:
+// every event's own background pair gives lambda_i >= |cB|, so the skipped terms (each <= e^-X_S;
+// their e dt and e r^2 moments are likewise bounded, omega dt >= X_S) change lambda_i, and the
+// excitation parts of the gradient terms, by at most |cS| n e^-X_S <= e^-50 |cB| = 2e-22 |cB|,
+// i.e. relatively by under 2e-22. The self-excitation probability S/(B+S) then moves by under
+// 2e-22 in absolute terms, which is not a relative bound for tiny probabilities: passes that
+// return probabilities skip exact zeros only. Without a background (cB = 0) nothing is cut.
+__host__ __device__ inline double excitation_skip_arg(const double *pk, int D, long long n) {
+    double sD = 1.0, xD = 1.0;
+    for (int q = 0; q < D; ++q) {
+        sD *= pk[0];
+        xD *= pk[1];
+    }
+    const double cB = fabs(pk[5] * xD * pk[2]) * 0.15915494309189533577;
+    const double cS = fabs(pk[4] * pk[3] * sD) * 0.39894228040143267794;
+    if (!(cB > 0.0) || !(cS < 1e300)) return kZeroExpArg;
+    const double r = cS > 0.0 ? log(cS / cB) : 0.0;
+    const double x = 50.0 + log((double)(n > 1 ? n : 1)) + (r > 0.0 ? r : 0.0);
+    return x < kZeroExpArg ? x : kZeroExpArg;
 }
 
 // Narrow layout: for small N the 256 x 256 work items cannot fill the device (N = 1000 gives 16
@@ -128,6 +152,7 @@ struct SymLaunch {
     int dim, A, R;
     int skip_zero;          // as PairGrid::skip_zero (background point: SymData::zero_arg_b)
     double zero_arg_s;      // as PairGrid::zero_arg_s
+    int cut_s;              // as PairGrid::cut_s
     int fast;               // 1: some dataset runs the base-2 form (scaled times, clamp-free exp2;
                             // SymData::fast); the host has checked its exponents, order and spans
     const int4 *fin_item;   // (dataset, k, finalize block, 0)

@@ -83,6 +83,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
     // background-exponential constants (log2 e scaled for the FP32-bulk v_exp_f32)
     const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
+    const double zs = g.cut_s ? excitation_skip_arg(pk, D, ev.n) : g.zero_arg_s;
 
     const int tid = threadIdx.x;
     const int n = ev.n, npad = ev.npad;
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel(Events ev, const do
                 // (background_skip_arg); excitation (tile before the rows only): arg <= -omega dt,
                 // skipped where every term is exactly +0.0.
                 if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
-                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
+                if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= zs) smode = kSNone;
             }
             const int mode = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
             switch (mode) {
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel_narrow(Events ev, c
     const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT;
     const double cSx = -0.5 * sigma * sigma, nomega = -omega;
     const double eBx = F32 ? cBx * kLog2e : cBx, eBt = F32 ? cBt * kLog2e : cBt;
+    const double zs = g.cut_s ? excitation_skip_arg(pk, D, ev.n) : g.zero_arg_s;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = ev.n, npad = ev.npad;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void pair_kernel_narrow(Events ev, c
         if (g.skip_zero) {
             const double gap = tl < t_lo ? t_lo - tl : (tf > t_hi ? tf - t_hi : 0.0);
             if (-cBt * gap * gap >= g.zero_arg_b) dob = 0;
-            if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= g.zero_arg_s) smode = kSNone;
+            if (smode == kSFull && nomega <= 0.0 && -nomega * gap >= zs) smode = kSNone;
         }
         switch (dob * 4 + smode) {
         case 4 + kSNone: tile_pass<D, GRAD, true, kSNone, F32>(sub, cnt, xi, ti, eBx, eBt, cSx, nomega, acc); break;
@@ -299,6 +301,7 @@ PairGrid plan_pair_grid(int n, int K, int target_blocks) {
     g.skip_zero = 1;
     g.zero_arg_b = background_skip_arg(n, false);
     g.zero_arg_s = kZeroExpArg;
+    g.cut_s = 0;
     // Narrow layout when the widest split of 256 x 256 items reaches under a quarter of the
     // requested grid: 64-row groups x 256-column chunks (the widest split's partials).
     const long long full = (long long)K * g.row_blocks * max_chunks;

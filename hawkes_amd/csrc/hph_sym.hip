@@ -216,7 +216,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     // exponent past the negligible point or every excitation exponent past the underflow
     // point (in the units of the form)
     const double zb = FAST ? ds.zero_arg_b * kLog2e : ds.zero_arg_b;
-    const double zs = FAST ? L.zero_arg_s * kLog2e : L.zero_arg_s;
+    const double xs = L.cut_s ? excitation_skip_arg(pk, D, n) : L.zero_arg_s;
+    const double zs = FAST ? xs * kLog2e : xs;
     const double gb = FAST ? 1.0 : 0.5 * tauT * tauT; // background exponent per gap^2
     const double gs = FAST ? -c.cS1 : omega;          // excitation exponent per gap
     const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);

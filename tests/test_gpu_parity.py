@@ -145,10 +145,11 @@ def test_batch_equals_single_calls_bitwise(kernel):
 @pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
 def test_tile_skip_matches_all_pairs(name, params, kernel):
     """Tiles are skipped where every background term is below e^-(50 + ln N) (each event's sums
-    change by < 5e-20 relative; its own pair contributes 1) or every excitation term is exactly
-    +0.0. Results equal the every-pair evaluation (HPH_FLAG_ALL_PAIRS) to within rounding:
-    ll and probabilities to 1e-15 relative, gradient component k to 1e-15 of its summation scale
-    S_k (the oracle's sum of |per-event terms|)."""
+    change by < 5e-20 relative; its own pair contributes 1) or, in likelihood and gradient passes,
+    every excitation term is below e^-X_S (rates change by < 2e-22 relative; passes that return
+    probabilities skip exact zeros only). Results equal the every-pair evaluation
+    (HPH_FLAG_ALL_PAIRS) to within rounding: ll and probabilities to 1e-15 relative, gradient
+    component k to 1e-15 of its summation scale S_k (the oracle's sum of |per-event terms|)."""
     locs, times = synth.cluster_process(6000, side=20.0, horizon=43_800.0, seed=41)
     scale = oracle.total(locs, times, params, precision="double")["scale"][1:]
     out = []
