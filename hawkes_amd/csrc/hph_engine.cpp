@@ -271,8 +271,9 @@ class SymRunner {
         L.skip_zero = skip_zero;
         L.zero_arg_s = kern::kZeroExpArg;
         L.cut_s = 0;
-        L.fast = 0;
-        for (char f : fast) L.fast |= f ? 1 : 0;
+        int nfast = 0;
+        for (char f : fast) nfast += f ? 1 : 0;
+        L.fast = nfast == 0 ? 0 : (nfast == (int)fast.size() ? 1 : 2);
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();

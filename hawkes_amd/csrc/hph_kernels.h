@@ -153,8 +153,9 @@ struct SymLaunch {
     int skip_zero;          // as PairGrid::skip_zero (background point: SymData::zero_arg_b)
     double zero_arg_s;      // as PairGrid::zero_arg_s
     int cut_s;              // as PairGrid::cut_s
-    int fast;               // 1: some dataset runs the base-2 form (scaled times, clamp-free exp2;
-                            // SymData::fast); the host has checked its exponents, order and spans
+    int fast;               // base-2 form (scaled times, clamp-free exp2; the host has checked the
+                            // exponents, order and spans): 0 no dataset, 1 every dataset, 2 some
+                            // (per SymData::fast)
     const int4 *fin_item;   // (dataset, k, finalize block, 0)
     int fin_items;
     const int2 *res_item;   // (dataset, k)

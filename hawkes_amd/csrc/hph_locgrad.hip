@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L
     for (;;) {
         if (tid == 0) s_item = atomicAdd(L.counter, 1);
         __syncthreads();
-        const int idx = s_item;
+        const int idx = __builtin_amdgcn_readfirstlane(s_item); // uniform: item data in scalar registers
         __syncthreads();
         if (idx >= L.items) break;
 

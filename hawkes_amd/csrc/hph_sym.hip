@@ -351,13 +351,14 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     }
 }
 
-// Persistent pass: every workgroup takes work items from the queue until it is empty. The FAST
-// instantiation runs each item in the base-2 form where its dataset allows it (SymData::fast)
-// and in the natural form otherwise.
+// Persistent pass: every workgroup takes work items from the queue until it is empty. FM = 0:
+// every item in the natural form; 1: every item in the base-2 form; 2: each item in the base-2
+// form where its dataset allows it (SymData::fast), else natural (both paths in one kernel, at
+// some register cost, so it is used only for mixed dataset tables).
 // Three waves per SIMD (at most 168 VGPRs): measured slightly faster than the two that the
 // unconstrained allocation (up to 228 VGPRs for 2 rows per lane) leaves room for.
-template <int D, bool GRAD, int A, bool F32, bool FAST>
-__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void sym_kernel(
+template <int D, bool GRAD, int A, bool F32, int FM>
+__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(FM == 2 ? 2 : 3, 3))) void sym_kernel(
     SymLaunch L, const double *__restrict__ params, double *__restrict__ rowpart, double *__restrict__ colpart) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NC = GRAD ? 3 : 1;
@@ -370,11 +371,11 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     for (;;) {
         if (tid == 0) s_item = atomicAdd(L.counter, 1);
         __syncthreads();
-        const int idx = s_item;
+        const int idx = __builtin_amdgcn_readfirstlane(s_item); // uniform: item data in scalar registers
         __syncthreads();
         if (idx >= L.items) break;
         const int4 it = L.item[idx];
-        if (FAST && L.data[it.x].fast)
+        if (FM == 1 || (FM == 2 && L.data[it.x].fast))
             sym_item<D, GRAD, A, F32, true>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
         else
             sym_item<D, GRAD, A, F32, false>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
@@ -588,26 +589,26 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     return plan;
 }
 
-template <int D, bool GRAD, bool F32, bool FAST>
+template <int D, bool GRAD, bool F32, int FM>
 static void launch_sym_d(const SymLaunch &L, const double *params, double *rowpart, double *colpart, hipStream_t s) {
     if (L.A == 2)
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32, FAST>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 2, F32, FM>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
     else
-        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32, FAST>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
+        hipLaunchKernelGGL((sym_kernel<D, GRAD, 1, F32, FM>), dim3((unsigned)L.grid), dim3(kSymWaves * 64), 0, s, L,
                            params, rowpart, colpart);
 }
 
-template <bool F32, bool FAST>
+template <bool F32, int FM>
 static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, double *rowpart, double *colpart,
                          hipStream_t s) {
     switch (L.dim * 2 + (grad ? 1 : 0)) {
-    case 2: launch_sym_d<1, false, F32, FAST>(L, params, rowpart, colpart, s); break;
-    case 3: launch_sym_d<1, true, F32, FAST>(L, params, rowpart, colpart, s); break;
-    case 4: launch_sym_d<2, false, F32, FAST>(L, params, rowpart, colpart, s); break;
-    case 5: launch_sym_d<2, true, F32, FAST>(L, params, rowpart, colpart, s); break;
-    case 6: launch_sym_d<3, false, F32, FAST>(L, params, rowpart, colpart, s); break;
-    case 7: launch_sym_d<3, true, F32, FAST>(L, params, rowpart, colpart, s); break;
+    case 2: launch_sym_d<1, false, F32, FM>(L, params, rowpart, colpart, s); break;
+    case 3: launch_sym_d<1, true, F32, FM>(L, params, rowpart, colpart, s); break;
+    case 4: launch_sym_d<2, false, F32, FM>(L, params, rowpart, colpart, s); break;
+    case 5: launch_sym_d<2, true, F32, FM>(L, params, rowpart, colpart, s); break;
+    case 6: launch_sym_d<3, false, F32, FM>(L, params, rowpart, colpart, s); break;
+    case 7: launch_sym_d<3, true, F32, FM>(L, params, rowpart, colpart, s); break;
     default: throw std::invalid_argument("symmetric pass is built for dimensions 1..3");
     }
 }
@@ -622,16 +623,13 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
     launch_queue_reset(L, s);
-    if (fp32) {
-        if (L.fast)
-            launch_sym_p<true, true>(L, params, grad, rowpart, colpart, s);
-        else
-            launch_sym_p<true, false>(L, params, grad, rowpart, colpart, s);
-    } else {
-        if (L.fast)
-            launch_sym_p<false, true>(L, params, grad, rowpart, colpart, s);
-        else
-            launch_sym_p<false, false>(L, params, grad, rowpart, colpart, s);
+    switch ((fp32 ? 3 : 0) + (L.fast >= 0 && L.fast <= 2 ? L.fast : 0)) {
+    case 0: launch_sym_p<false, 0>(L, params, grad, rowpart, colpart, s); break;
+    case 1: launch_sym_p<false, 1>(L, params, grad, rowpart, colpart, s); break;
+    case 2: launch_sym_p<false, 2>(L, params, grad, rowpart, colpart, s); break;
+    case 3: launch_sym_p<true, 0>(L, params, grad, rowpart, colpart, s); break;
+    case 4: launch_sym_p<true, 1>(L, params, grad, rowpart, colpart, s); break;
+    default: launch_sym_p<true, 2>(L, params, grad, rowpart, colpart, s); break;
     }
 }
 
