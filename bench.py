@@ -152,8 +152,24 @@ def load_traffic(kind):
     return rec.get("bytes_per_launch") if rec.get("kernel", "").startswith(kind + "_kernel") else None
 
 
+def host_cpu():
+    """Model name and logical CPU count of this host (recorded beside every CPU number)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(locs, times, params, seconds):
-    """Threaded + AVX2 restatement of the reference CPU engine on a bounded sample of rows."""
+    """The restated reference CPU engines on bounded row samples of the same workload (LL + 6-gradient,
+    every row against all N columns): OpenMP rows + AVX2 on all host threads (the reported value,
+    the reference's TBB/AVX engine), and single-core serial scalar and AVX2 4-wide FP64."""
     import oracle
     n = len(times)
     mid = n // 2
@@ -165,6 +181,14 @@ def cpu_baseline(locs, times, params, seconds):
     t0 = time.perf_counter()
     oracle.cpu_baseline(oracle.THREADED, locs, times, params, mid, mid + rows)
     dt = time.perf_counter() - t0
+    flavours = {}
+    for name, flavour in (("serial", oracle.SERIAL), ("avx", oracle.AVX)):
+        r = 8
+        t1 = time.perf_counter()
+        oracle.cpu_baseline(flavour, locs, times, params, mid, mid + r)
+        d1 = time.perf_counter() - t1
+        flavours[name] = {"value": r * n / d1, "cores": 1, "rows": r}
+    flavours["threaded"] = {"value": rows * n / dt, "cores": oracle.num_threads(), "rows": rows}
     return {
         "value": rows * n / dt,
         "unit": "pair-evals/s",
@@ -172,6 +196,8 @@ def cpu_baseline(locs, times, params, seconds):
         "kind": "port",
         "sample": "%d rows x all %d columns of the same workload (LL + 6-gradient), %.1f s, "
                   "OpenMP rows + AVX2 4-wide FP64 restating the reference's TBB/AVX engine" % (rows, n, dt),
+        "flavours": flavours,
+        "host": host_cpu(),
     }
 
 
@@ -188,7 +214,7 @@ def cpu_config1(seconds):
     oracle.cpu_baseline(oracle.SERIAL, locs, times, p0, grad=False)
     t_serial = time.perf_counter() - t0
     evals = int(max(1, min(100, seconds / (2.0 * max(t_serial, 1e-6)))))
-    out = {"N": n, "evals": evals, "unit": "pair-evals/s"}
+    out = {"N": n, "evals": evals, "unit": "pair-evals/s", "host": host_cpu()}
     for name, flavour, cores in (("serial", oracle.SERIAL, 1), ("avx", oracle.AVX, 1),
                                  ("threaded", oracle.THREADED, oracle.num_threads())):
         t0 = time.perf_counter()
