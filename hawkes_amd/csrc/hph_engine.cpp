@@ -459,9 +459,9 @@ class HipHawkes : public AbstractHawkes {
         return ll_;
     }
 
-    // The finalize kernel writes the probabilities on every single-vector evaluation, so after
-    // getSumOfLikContribs / getLogLikeGradient with unchanged data and parameters this is a copy
-    // (the reference recomputes, src/NewHawkes.hpp:279 notes the missing caching).
+    // A pass that skips exact excitation zeros only (evaluate_single with a probability output),
+    // unless the last evaluation of the unchanged state already was one (its probabilities are
+    // kept); the reference always recomputes (src/NewHawkes.hpp:279 notes the missing caching).
     void getProbsSelfExcite(double *result, size_t length) override {
         if (result == nullptr) throw std::invalid_argument("result pointer is null");
         if (length != (size_t)locationCount) throw std::invalid_argument("getProbsSelfExcite: length must be locationCount");
