@@ -523,7 +523,7 @@ class HipHawkes : public AbstractHawkes {
         }
         if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         const TuneConfig cfg = config_for(1, true);
-        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && D <= 3) {
+        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && D <= 3 && sym_fits(1, cfg, true)) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             sym_.enqueue_locgrad(ref, D, d_params_.as<double>(), d_w_.as<double>(), d_locgrad_.as<double>(),
                                  cfg.rows_per_lane, cfg.target_blocks, skip_zero, stream_);
@@ -763,6 +763,23 @@ class HipHawkes : public AbstractHawkes {
         return c;
     }
 
+    // Whether the symmetric pass's partial sums (row partials per chunk, column partials per
+    // (block, earlier column): ~3 N^2 / (2 R) doubles per vector with gradient) fit in 80% of the
+    // device memory; memoised per shape.
+    bool sym_fits(int K, const TuneConfig &cfg, bool grad) {
+        const long long key = (((long long)K * 4 + cfg.rows_per_lane) * (1LL << 21) + cfg.target_blocks) * 2 + (grad ? 1 : 0);
+        auto it = sym_fits_.find(key);
+        if (it != sym_fits_.end()) return it->second;
+        const kern::SymPlan plan = kern::plan_sym({locationCount}, {K}, cfg.rows_per_lane, cfg.target_blocks, grad);
+        const double bytes = 8.0 * ((double)plan.rowpart_doubles + (double)plan.colpart_doubles +
+                                    (double)plan.block_partials * kern::kResultWidth * 2);
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const bool fits = bytes <= 0.8 * (double)total_b;
+        sym_fits_[key] = fits;
+        return fits;
+    }
+
     TuneKey tune_key(int K, bool grad) const {
         TuneKey key;
         int b = 0;
@@ -783,7 +800,7 @@ class HipHawkes : public AbstractHawkes {
         run_with(params, K, grad, probs_host, config_for(K, grad), write_probs);
     }
 
-    void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg,
+    void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg_in,
                   bool write_probs = false) {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
@@ -801,6 +818,13 @@ class HipHawkes : public AbstractHawkes {
         const kern::Events ev = events();
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         double *probs_dev = (probs_host || write_probs) ? d_probs_.as<double>() : nullptr;
+        TuneConfig cfg = cfg_in;
+        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && !sym_fits(K, cfg, grad)) {
+            if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC)
+                throw std::runtime_error("the symmetric pass's partial sums do not fit in device memory at this N "
+                                         "and K; choose the row-wise pass");
+            cfg = TuneConfig{}; // row-wise: partials of K x chunks x 6 x N doubles only
+        }
         double *w_dev = nullptr; // 1 / lambda of vector 0 (sorted order), kept for the location gradient
         if (write_probs) {
             d_w_.ensure(sizeof(double) * npad_);
@@ -904,6 +928,7 @@ class HipHawkes : public AbstractHawkes {
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
     bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
     bool force_natural_ = false, last_fast_ = false, exact_s_ = false;
+    std::map<long long, bool> sym_fits_;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
     double stored_params_[6] = {0, 0, 0, 0, 0, 0};
