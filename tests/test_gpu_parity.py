@@ -209,6 +209,23 @@ def test_near_tied_times_keep_the_strict_mask(kernel):
     assert_matches_oracle(locs, times, synth.P0, name="near-ties", kernel=kernel)
 
 
+@pytest.mark.parametrize("name,params", [
+    ("theta0", [1.0, 0.8, 0.05, 0.5, 0.0, 0.9]),
+    ("omega0", [1.0, 0.8, 0.05, 0.0, 0.7, 0.9]),
+    ("sigma0", [0.0, 0.8, 0.05, 0.5, 0.7, 0.9]),
+    ("negative_times", [1.1, 0.7, 0.04, 0.6, 0.8, 0.7]),
+])
+@pytest.mark.parametrize("kernel", ["rowwise_narrow", "symmetric"])
+def test_boundary_parameters_and_negative_times(name, params, kernel):
+    """theta = 0, omega = 0 or sigma = 0 switch the excitation term off (its constant is zero) or
+    make it time-independent; times may be negative (Phi(-tauT t_i) > 1/2)."""
+    rng = np.random.default_rng(81)
+    n = 2600
+    locs = rng.uniform(0, 3, (n, 2))
+    times = np.sort(rng.uniform(-50, 50, n))
+    assert_matches_oracle(locs, times, params, name=name, kernel=kernel)
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
