@@ -366,6 +366,7 @@ class HipHawkes : public AbstractHawkes {
         : AbstractHawkes(D, N, flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
         if (D < 1 || D > kern::kMaxDim) throw std::invalid_argument("embeddingDimension must be in 1..8");
         if (N < 1) throw std::invalid_argument("locationCount must be >= 1");
+        if (N > (1 << 30)) throw std::invalid_argument("locationCount must be at most 2^30");
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device_ >= count) throw std::invalid_argument("device index out of range");
@@ -955,8 +956,10 @@ class HipHawkesBatch {
         : D_(D), counts_(counts), flags_(flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
         if (D < 1 || D > 3) throw std::invalid_argument("batched datasets support dimensions 1..3");
         if (counts.empty()) throw std::invalid_argument("dataset_count must be >= 1");
-        for (int c : counts)
+        for (int c : counts) {
             if (c < 1) throw std::invalid_argument("every dataset needs at least one event");
+            if (c > (1 << 30)) throw std::invalid_argument("a dataset has at most 2^30 events");
+        }
         int count = 0;
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device_ >= count) throw std::invalid_argument("device index out of range");
