@@ -1153,7 +1153,8 @@ class HipHawkesBatch {
     int device_;
     bool fp32_;
     hipStream_t stream_ = nullptr;
-    std::vector<int> off_, npad_;
+    std::vector<long long> off_; // start of dataset d in the concatenated device arrays
+    std::vector<int> npad_;
     std::vector<std::vector<double>> locs_, times_;
     std::vector<Extents> ext_;
     std::vector<bool> have_;
