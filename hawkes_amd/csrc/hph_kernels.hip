@@ -244,7 +244,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
         double a6[NACC];
 #pragma unroll
         for (int a = 0; a < NACC; ++a) a6[a] = 0.0;
-        for (int c = 0; c < g.chunks; ++c) {
+#pragma unroll 4
+        for (int c = 0; c < g.chunks; ++c) { // unrolled: several chunks' loads in flight
             const double *base = part + ((size_t)k * g.chunks + c) * NACC * npad;
 #pragma unroll
             for (int a = 0; a < NACC; ++a) a6[a] += base[(size_t)a * npad + i];

@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void inv_rate_kernel(Events ev, cons
     if (i >= npad) return;
     double sb = 0.0, ss = 0.0;
     if (i < n) {
+#pragma unroll 4
         for (int c = 0; c < g.chunks; ++c) {
             const double *base = part + (size_t)c * 2 * npad;
             sb += base[i];
@@ -166,6 +167,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void locgrad_finalize_kernel(Events 
     const int u = ev.perm[i];
     for (int d = 0; d < D; ++d) {
         double s = 0.0;
+        #pragma unroll 4
         for (int c = 0; c < g.chunks; ++c) s += part[((size_t)c * D + d) * npad + i];
         out[(size_t)u * D + d] = s;
     }
@@ -430,6 +432,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_locgrad_finalize_kernel(Sym
     double a[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) a[d] = 0.0;
+    #pragma unroll 4
     for (int c = 0; c < nch; ++c) {
         const double *rp = rbase + (size_t)c * NR * rpad;
 #pragma unroll
@@ -444,6 +447,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_locgrad_finalize_kernel(Sym
 #pragma unroll
     for (int d = 0; d < D; ++d) cs[d] = 0.0;
     const double *cbase = colpart + ds.col0;
+    #pragma unroll 4
     for (int Q = P + 1; Q < ds.nb; ++Q) {
         const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
         const size_t Q0 = (size_t)Q * R;

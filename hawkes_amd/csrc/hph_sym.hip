@@ -412,6 +412,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
         double a6[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) a6[q] = 0.0;
+        // unrolled so that several slots' loads are in flight (one grid of 256-row blocks is too
+        // small to cover HBM latency with one load per thread)
+#pragma unroll 4
         for (int c = 0; c < nch; ++c) {
             const double *rp = rbase + (size_t)c * NR * rpad;
 #pragma unroll
@@ -426,6 +429,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
 #pragma unroll
         for (int c = 0; c < NC; ++c) cs[c] = 0.0;
         const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+#pragma unroll 4
         for (int Q = P + 1; Q < ds.nb; ++Q) {
             const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
             const size_t Q0 = (size_t)Q * R;
