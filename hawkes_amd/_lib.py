@@ -24,7 +24,7 @@ AVX = 1 << 6
 AVX512 = 1 << 7
 ALL_PAIRS = 1 << 16  # extension: no exact-zero tile skipping
 
-PAIR_KERNEL_AUTO, PAIR_KERNEL_ROWWISE, PAIR_KERNEL_SYMMETRIC = 0, 1, 2
+PAIR_KERNEL_AUTO, PAIR_KERNEL_ROWWISE, PAIR_KERNEL_SYMMETRIC, PAIR_KERNEL_SPLIT = 0, 1, 2, 3
 
 # Every symbol include/hph.h declares, with (restype, argtypes).
 _dp = ctypes.POINTER(ctypes.c_double)

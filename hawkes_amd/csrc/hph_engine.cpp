@@ -155,7 +155,8 @@ class SymRunner {
     };
 
     ~SymRunner() {
-        for (DevBuf *b : {&d_data_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_}) b->release();
+        for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_})
+            b->release();
     }
 
     // Plans (if the shape changed) and enqueues the pass, finalize and reduction on stream s.
@@ -174,10 +175,55 @@ class SymRunner {
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "symmetric pair kernel launch");
         if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
-        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), d_block_.as<double>(),
-                                  probs_dev, inv_lam_dev, s);
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), nullptr,
+                                  d_block_.as<double>(), probs_dev, inv_lam_dev, s);
         check(hipGetLastError(), "finalize kernel launch");
         kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
+        check(hipGetLastError(), "reduce kernel launch");
+        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+    }
+
+    // The split passes over one dataset: the background pass on the culled order (culled: that
+    // order's events, perm = culled position -> time-sorted position, gstat = its group boxes),
+    // its per-event sums into bsum (device, K x (grad ? 3 : 1) x npad), the excitation pass on the
+    // time-sorted order, then the finalize (background moments from bsum) and the reduction.
+    // Arguments otherwise as enqueue; ev1 is recorded after both passes.
+    void enqueue_split(const DataRef &sorted, const DataRef &culled, const double *gstat, int K, int dim,
+                       const double *params, bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
+                       int cut_s, bool fast, double *bsum, double *probs_dev, double *inv_lam_dev, double *result,
+                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+        kern::SymLaunch L =
+            prepare({sorted}, {K}, dim, grad, fp32, rows_per_lane, target_items, skip_zero, {fast ? (char)1 : (char)0});
+        // the background pass's table: the sorted table with the culled order's arrays
+        kern::SymData b = plan_.data[0];
+        b.x = culled.x;
+        b.t = culled.t;
+        b.perm = culled.perm;
+        b.gstat = gstat;
+        if (!b_valid_ || std::memcmp(&b, &b_data_, sizeof(b)) != 0) {
+            d_data_b_.ensure(sizeof(kern::SymData));
+            check(hipMemcpy(d_data_b_.ptr, &b, sizeof(b), hipMemcpyHostToDevice), "upload culled dataset table");
+            b_data_ = b;
+            b_valid_ = true;
+        }
+        kern::SymLaunch Lb = L;
+        Lb.data = d_data_b_.as<kern::SymData>();
+        Lb.pm = kern::kPassCulledB;
+        kern::SymLaunch Ls = L;
+        Ls.pm = kern::kPassExcitation;
+        Ls.cut_s = skip_zero && cut_s ? 1 : 0;
+        if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
+        kern::launch_sym(Lb, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
+        check(hipGetLastError(), "background pass launch");
+        kern::launch_sym_bsums(Lb, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, s);
+        check(hipGetLastError(), "background sums launch");
+        kern::launch_sym(Ls, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
+        check(hipGetLastError(), "excitation pass launch");
+        if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
+        kern::launch_sym_finalize(Ls, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum,
+                                  d_block_.as<double>(), probs_dev, inv_lam_dev, s);
+        check(hipGetLastError(), "finalize kernel launch");
+        kern::launch_sym_reduce(Ls, grad, d_block_.as<double>(), result, s);
         check(hipGetLastError(), "reduce kernel launch");
         if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
     }
@@ -229,6 +275,7 @@ class SymRunner {
             d_block_.ensure(sizeof(double) * std::max<size_t>(plan_.block_partials, 1) * kern::kResultWidth * 2);
             refs_.clear();
             valid_ = true;
+            b_valid_ = false;
         }
         bool refs_same = refs_.size() == refs.size();
         for (size_t d = 0; refs_same && d < refs.size(); ++d)
@@ -286,7 +333,9 @@ class SymRunner {
     std::vector<int> n_, K_;
     int A_ = 0, target_ = 0, cus_ = 0;
     std::vector<DataRef> refs_;
-    DevBuf d_data_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_;
+    kern::SymData b_data_{};
+    bool b_valid_ = false;
+    DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_;
 };
 
 // Pair-pass configuration chosen by the autotuner.
@@ -385,7 +434,7 @@ class HipHawkes : public AbstractHawkes {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_w_,
-                          &d_locgrad_})
+                          &d_locgrad_, &d_xc_, &d_tc_, &d_cperm_, &d_gstat_, &d_bsum_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -524,7 +573,8 @@ class HipHawkes : public AbstractHawkes {
         }
         if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         const TuneConfig cfg = config_for(1, true);
-        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && D <= 3 && sym_fits(1, cfg, true)) {
+        if ((cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC || cfg.kind == HPH_PAIR_KERNEL_SPLIT) && D <= 3 &&
+            sym_fits(1, cfg, true)) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             sym_.enqueue_locgrad(ref, D, d_params_.as<double>(), d_w_.as<double>(), d_locgrad_.as<double>(),
                                  cfg.rows_per_lane, cfg.target_blocks, skip_zero, stream_);
@@ -631,7 +681,7 @@ class HipHawkes : public AbstractHawkes {
                 }
             }
         }
-        if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC) {
+        if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC || kernel_ == HPH_PAIR_KERNEL_SPLIT) {
             std::vector<TuneConfig> only;
             for (auto &c : cands)
                 if (c.kind == HPH_PAIR_KERNEL_SYMMETRIC) only.push_back(c);
@@ -662,7 +712,10 @@ class HipHawkes : public AbstractHawkes {
 
     void getTunedConfig(int *out, size_t length) override {
         if (out == nullptr || length < 3) throw std::invalid_argument("tuned-config output needs 3 values");
-        const TuneConfig c = config_for(1, true);
+        TuneConfig c = config_for(1, true);
+        if (c.kind == HPH_PAIR_KERNEL_SYMMETRIC && kernel_ == HPH_PAIR_KERNEL_AUTO && !(flags & ALL_PAIRS) &&
+            have_params_ && !data_dirty_ && split_pays(params_, 1))
+            c.kind = HPH_PAIR_KERNEL_SPLIT;
         out[0] = c.kind;
         out[1] = c.rows_per_lane;
         out[2] = c.target_blocks;
@@ -671,9 +724,9 @@ class HipHawkes : public AbstractHawkes {
     void setPairKernel(int kind) override { setPairConfig(kind, 2, 4096); }
 
     void setPairConfig(int kind, int rows_per_lane, int target_blocks) override {
-        if (kind < HPH_PAIR_KERNEL_AUTO || kind > HPH_PAIR_KERNEL_SYMMETRIC)
+        if (kind < HPH_PAIR_KERNEL_AUTO || kind > HPH_PAIR_KERNEL_SPLIT)
             throw std::invalid_argument("unknown pair kernel kind");
-        if (kind == HPH_PAIR_KERNEL_SYMMETRIC && embeddingDimension > 3)
+        if ((kind == HPH_PAIR_KERNEL_SYMMETRIC || kind == HPH_PAIR_KERNEL_SPLIT) && embeddingDimension > 3)
             throw std::invalid_argument("the symmetric pair kernel is built for dimensions 1..3");
         if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
         if (target_blocks < 64 || target_blocks > (1 << 20))
@@ -725,6 +778,101 @@ class HipHawkes : public AbstractHawkes {
         check(hipStreamSynchronize(stream_), "upload sync");
         extents_ = extents_of(locs_, ht, (size_t)N, D);
         data_dirty_ = false;
+        culled_valid_ = false;
+    }
+
+    // The culled order of the split passes: the time-sorted events cut into windows of
+    // cull_window() events, each window ordered along a Morton curve of its locations (21 bits per axis of
+    // the bounding box), so that every group of kern::kGroup consecutive events spans a small box
+    // in space and a window in time. Uploads that order's locations and times, its map to sorted
+    // positions and the groups' boxes (empty groups get an inverted box that every test skips).
+    void build_culled() {
+        const int N = locationCount, D = embeddingDimension;
+        std::vector<double> lo(D), hi(D);
+        for (int d = 0; d < D; ++d) {
+            lo[d] = hi[d] = locs_[d];
+            for (int i = 1; i < N; ++i) {
+                lo[d] = std::min(lo[d], locs_[(size_t)i * D + d]);
+                hi[d] = std::max(hi[d], locs_[(size_t)i * D + d]);
+            }
+        }
+        std::vector<unsigned long long> code(N);
+        for (int s = 0; s < N; ++s) {
+            const int u = perm_[s];
+            unsigned long long c = 0;
+            for (int d = 0; d < D; ++d) {
+                const double w = hi[d] > lo[d] ? (locs_[(size_t)u * D + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+                const unsigned long long q =
+                    (unsigned long long)std::min(2097151.0, std::max(0.0, std::floor(w * 2097152.0)));
+                for (int b = 0; b < 21; ++b) c |= ((q >> b) & 1ULL) << (b * D + d);
+            }
+            code[s] = c;
+        }
+        std::vector<int> cperm(N); // culled position -> sorted position
+        std::iota(cperm.begin(), cperm.end(), 0);
+        const int W = cull_window();
+        for (int w0 = 0; w0 < N; w0 += W) {
+            const int w1 = std::min(N, w0 + W);
+            std::stable_sort(cperm.begin() + w0, cperm.begin() + w1,
+                             [&](int a, int b) { return code[a] < code[b]; });
+        }
+        const size_t bytes = sizeof(double) * (size_t)(D + 1) * npad_ + sizeof(int) * (size_t)npad_;
+        h_stage_.ensure(bytes);
+        double *hx = h_stage_.as<double>();
+        double *ht = hx + (size_t)D * npad_;
+        int *hp = reinterpret_cast<int *>(ht + npad_);
+        for (int r = 0; r < npad_; ++r) {
+            const int sp = cperm[r < N ? r : N - 1];
+            const int u = perm_[sp];
+            for (int d = 0; d < D; ++d) hx[(size_t)d * npad_ + r] = locs_[(size_t)u * D + d];
+            ht[r] = times_[u];
+            hp[r] = r < N ? sp : 0;
+        }
+        // boxes for every row of the largest row blocks (512 events), past npad
+        const int groups = (N + 511) / 512 * (512 / kern::kGroup);
+        const int GS = 2 * D + 2;
+        std::vector<double> gs((size_t)groups * GS);
+        for (int g = 0; g < groups; ++g) {
+            double *e = gs.data() + (size_t)g * GS;
+            for (int d = 0; d <= D; ++d) {
+                e[d] = 1e300;
+                e[D + 1 + d] = -1e300;
+            }
+            for (int r = g * kern::kGroup; r < std::min(N, (g + 1) * kern::kGroup); ++r) {
+                for (int d = 0; d < D; ++d) {
+                    e[d] = std::min(e[d], hx[(size_t)d * npad_ + r]);
+                    e[D + 1 + d] = std::max(e[D + 1 + d], hx[(size_t)d * npad_ + r]);
+                }
+                e[D] = std::min(e[D], ht[r]);
+                e[2 * D + 1] = std::max(e[2 * D + 1], ht[r]);
+            }
+        }
+        d_xc_.ensure(sizeof(double) * (size_t)D * npad_);
+        d_tc_.ensure(sizeof(double) * (size_t)npad_);
+        d_cperm_.ensure(sizeof(int) * (size_t)npad_);
+        d_gstat_.ensure(sizeof(double) * gs.size());
+        check(hipMemcpyAsync(d_xc_.ptr, hx, sizeof(double) * (size_t)D * npad_, hipMemcpyHostToDevice, stream_),
+              "upload culled x");
+        check(hipMemcpyAsync(d_tc_.ptr, ht, sizeof(double) * npad_, hipMemcpyHostToDevice, stream_), "upload culled t");
+        check(hipMemcpyAsync(d_cperm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_),
+              "upload culled order");
+        check(hipStreamSynchronize(stream_), "culled order sync");
+        check(hipMemcpy(d_gstat_.ptr, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice),
+              "upload group boxes");
+        culled_valid_ = true;
+    }
+
+    // Whether the split passes are expected to pay: the bounding box's diagonal spans at least
+    // kSplitMinRatio times the distance at which every background term of every vector is
+    // negligible (tauX r / sqrt(2 skip point)), so that groups far apart in space are skipped.
+    bool split_pays(const double *params, int K) const {
+        if (embeddingDimension > 3 || locationCount < kSplitMinEvents) return false;
+        const double arg = kern::background_skip_arg(locationCount, fp32_);
+        for (int k = 0; k < K; ++k) {
+            const double tx = std::fabs(params[6 * (size_t)k + 1]);
+            if (!(tx * std::sqrt(extents_.r2max) >= kSplitMinRatio * std::sqrt(2.0 * arg))) return false;
+        }
+        return true;
     }
 
     kern::Events events() const {
@@ -820,8 +968,9 @@ class HipHawkes : public AbstractHawkes {
         const int skip_zero = (flags & ALL_PAIRS) ? 0 : 1;
         double *probs_dev = (probs_host || write_probs) ? d_probs_.as<double>() : nullptr;
         TuneConfig cfg = cfg_in;
-        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC && !sym_fits(K, cfg, grad)) {
-            if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC)
+        const bool sym_kind = cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC || cfg.kind == HPH_PAIR_KERNEL_SPLIT;
+        if (sym_kind && !sym_fits(K, cfg, grad)) {
+            if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC || kernel_ == HPH_PAIR_KERNEL_SPLIT)
                 throw std::runtime_error("the symmetric pass's partial sums do not fit in device memory at this N "
                                          "and K; choose the row-wise pass");
             cfg = TuneConfig{}; // row-wise: partials of K x chunks x 6 x N doubles only
@@ -831,16 +980,31 @@ class HipHawkes : public AbstractHawkes {
             d_w_.ensure(sizeof(double) * npad_);
             w_dev = d_w_.as<double>();
         }
-        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC) {
+        if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC || cfg.kind == HPH_PAIR_KERNEL_SPLIT) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
             last_fast_ = !force_natural_ && fast_ok(extents_, params, K);
-            sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                         cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0}, probs_dev,
-                         w_dev,
-                         d_result_.as<double>(), stream_,
-                         timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
+            const bool split = cfg.kind == HPH_PAIR_KERNEL_SPLIT ||
+                               (kernel_ == HPH_PAIR_KERNEL_AUTO && skip_zero && split_pays(params, K));
+            if (split) {
+                if (!culled_valid_) build_culled();
+                d_bsum_.ensure(sizeof(double) * (size_t)K * (grad ? 3 : 1) * npad_);
+                SymRunner::DataRef culled{d_xc_.as<double>(), d_tc_.as<double>(), d_cperm_.as<int>(), ev.n, ev.npad,
+                                          ev.t_last};
+                sym_.enqueue_split(ref, culled, d_gstat_.as<double>(), K, embeddingDimension, d_params_.as<double>(),
+                                   grad, fp32_, cfg.rows_per_lane, cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
+                                   last_fast_, d_bsum_.as<double>(), probs_dev, w_dev, d_result_.as<double>(), stream_,
+                                   timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
+                                   timing_ ? ev_[2] : nullptr);
+            } else {
+                sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
+                             cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
+                             probs_dev, w_dev, d_result_.as<double>(), stream_, timing_ ? ev_[0] : nullptr,
+                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
+            }
+            last_split_ = split;
         } else {
             last_fast_ = false;
+            last_split_ = false;
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
             g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
@@ -910,6 +1074,20 @@ class HipHawkes : public AbstractHawkes {
     // below 24,576 (229 us at 1.6e4), symmetric with 2 rows per lane above (551 us at 3.3e4).
     static constexpr int kNarrowMaxEvents = 12288;
     static constexpr int kSymA1MaxEvents = 24576;
+    // Split passes (AUTO): chosen from kSplitMinEvents events on when the locations' diagonal is
+    // at least kSplitMinRatio times the negligible-background distance (counted on cluster
+    // processes with windows of 32,768: ratio 3.6 forms 0.54x the background pairs of the
+    // time-sorted pass, ratio 7.9 0.23x, ratio 2.6 about as many).
+    static constexpr int kSplitMinEvents = 65536;
+    static constexpr double kSplitMinRatio = 3.0;
+
+    // Events per window of the culled order: N / 8 rounded down to a power of two, in
+    // [1024, 32768] (longer windows make groups smaller in space but longer in time).
+    int cull_window() const {
+        int w = 1024;
+        while (w < 32768 && 2 * w <= locationCount / 8) w *= 2;
+        return w;
+    }
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
     Extents extents_;
@@ -928,7 +1106,7 @@ class HipHawkes : public AbstractHawkes {
 
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
     bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
-    bool force_natural_ = false, last_fast_ = false, exact_s_ = false;
+    bool force_natural_ = false, last_fast_ = false, exact_s_ = false, last_split_ = false;
     std::map<long long, bool> sym_fits_;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
@@ -943,6 +1121,9 @@ class HipHawkes : public AbstractHawkes {
 
     DevBuf d_x_, d_t_, d_perm_, d_probs_, d_params_, d_part_, d_block_, d_result_, d_w_, d_locgrad_;
     HostBuf h_stage_, h_result_, h_params_;
+    // culled order of the split passes (built on first use after a data change)
+    bool culled_valid_ = false;
+    DevBuf d_xc_, d_tc_, d_cperm_, d_gstat_, d_bsum_;
 };
 
 

@@ -137,6 +137,10 @@ struct SymData {
     double t_last;          // the caller's last time (T of the model)
     double zero_arg_b;      // background skip point of this dataset (background_skip_arg)
     int fast;               // 1: this dataset's items run the base-2 form (SymLaunch::fast set)
+    const double *gstat;    // background pass on the culled order (SymLaunch::pm = kPassCulledB):
+                            // per group of 64 events, the minima of the D coordinates and of time,
+                            // then their maxima (2 D + 2 doubles); perm then maps a position of
+                            // the culled order to the time-sorted position
     long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
     long long col0, col_kstride; // column partials: the same
     long long blk0;         // index of its (k = 0, block 0) block partial
@@ -160,7 +164,20 @@ struct SymLaunch {
     int fin_items;
     const int2 *res_item;   // (dataset, k)
     int res_items;
+    int pm;                 // pass mode: kPassBoth, kPassCulledB or kPassExcitation
 };
+
+// Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted
+// events in one pass. For spatially extended data the two terms run as separate passes:
+// kPassCulledB forms the background terms on the culled order (the time-sorted events cut into
+// windows of consecutive events, each window ordered along a Morton curve of its locations, so
+// that groups of 64 events are compact in space and time) and skips every (wave, 64-column group)
+// and (block, tile) pair whose bounding boxes put every background exponent past the negligible
+// point; kPassExcitation forms the excitation terms of the time-sorted events alone.
+constexpr int kPassBoth = 0;
+constexpr int kPassCulledB = 1;
+constexpr int kPassExcitation = 2;
+constexpr int kGroup = 64; // events per bounding-box group of the culled order
 
 // Host-side plan of a symmetric pass over datasets of sizes n[d] with K[d] vectors each.
 struct SymPlan {
@@ -179,8 +196,17 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 // Finalize (per row, then per 256-row block in double-double) and reduction (per dataset and
 // vector: result[(res0 + k) * 7 + q]); probs and inv_lam (may be null) receive dataset 0,
 // vector 0 (as launch_finalize).
+// bsum (may be null): background sums per (k, moment, time-sorted event), written by
+// launch_sym_bsums; the finalize then takes the background moments from it and the excitation
+// moments from the row partials of a kPassExcitation pass.
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
-                         const double *colpart, double *block_out, double *probs, double *inv_lam, hipStream_t s);
+                         const double *colpart, const double *bsum, double *block_out, double *probs,
+                         double *inv_lam, hipStream_t s);
+// Background row sums of a kPassCulledB pass over dataset 0 (row partials + column partials),
+// scattered to time-sorted positions: bsum[(k * NB + q) * npad + sorted index], NB = 3 with the
+// gradient (sum, r^2 and dt^2 moments), else 1.
+void launch_sym_bsums(const SymLaunch &L, bool grad, const double *rowpart, const double *colpart, double *bsum,
+                      hipStream_t s);
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
 // Zeroes the work-queue counter (a one-thread kernel, so that the sequence can be graph-captured).
 void launch_queue_reset(const SymLaunch &L, hipStream_t s);

@@ -1,0 +1,95 @@
+// Split symmetric passes for spatially extended data: the background terms on the culled order
+// (bounding-box skips) and the excitation terms on the time-sorted order, as two passes, and
+// the kernel that turns the background pass's partials into per-event sums.
+#include "hph_sym_device.h"
+
+#include <stdexcept>
+
+namespace hph {
+namespace kern {
+
+void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart,
+                      double *colpart, hipStream_t s) {
+    if (L.fast != 0 && L.fast != 1) throw std::invalid_argument("split passes take one form for the whole pass");
+    const int v = (fp32 ? 2 : 0) + L.fast;
+    if (L.pm == kPassCulledB) {
+        switch (v) {
+        case 0: launch_sym_p<false, 0, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
+        case 1: launch_sym_p<false, 1, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
+        case 2: launch_sym_p<true, 0, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
+        default: launch_sym_p<true, 1, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
+        }
+    } else if (L.pm == kPassExcitation) {
+        // the excitation terms are FP64 in both precision modes
+        switch (L.fast) {
+        case 0: launch_sym_p<false, 0, kPassExcitation>(L, params, grad, rowpart, colpart, s); break;
+        default: launch_sym_p<false, 1, kPassExcitation>(L, params, grad, rowpart, colpart, s); break;
+        }
+    } else {
+        throw std::invalid_argument("unknown pass mode");
+    }
+}
+
+// Background sums per event of a kPassCulledB pass over dataset 0: for culled position r, its
+// row-side chunks, its diagonal block and the column side of every later block, in a fixed
+// order, written to its time-sorted position perm[r]. One block of 256 positions per workgroup.
+template <bool GRAD>
+__global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, const double *__restrict__ rowpart,
+                                                                 const double *__restrict__ colpart,
+                                                                 double *__restrict__ bsum) {
+    constexpr int NR = GRAD ? 6 : 2;
+    constexpr int NC = GRAD ? 3 : 1;
+    constexpr int QS = GRAD ? 3 : 1;
+    const int4 it = L.fin_item[blockIdx.x];
+    if (it.x != 0) return;
+    const SymData &ds = L.data[0];
+    const int k = it.y;
+    const int r = it.z * kRowsPerBlock + threadIdx.x;
+    if (r >= ds.n) return;
+    const int R = L.R;
+    const int P = r / R;
+    const int nch = (P * R + ds.L - 1) / ds.L;
+    const size_t rpad = (size_t)ds.nb * R;
+    const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
+    double a[QS];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) a[q] = 0.0;
+#pragma unroll 4
+    for (int c = 0; c < nch; ++c) {
+        const double *rp = rbase + (size_t)c * NR * rpad;
+#pragma unroll
+        for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
+    }
+    {
+        const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
+#pragma unroll
+        for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
+    }
+    double cs[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cs[c] = 0.0;
+    const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+#pragma unroll 4
+    for (int Q = P + 1; Q < ds.nb; ++Q) {
+        const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
+        const size_t Q0 = (size_t)Q * R;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cs[c] += cp[(size_t)c * Q0 + r];
+    }
+    const int i = ds.perm[r];
+#pragma unroll
+    for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = a[q] + cs[q];
+}
+
+void launch_sym_bsums(const SymLaunch &L, bool grad, const double *rowpart, const double *colpart, double *bsum,
+                      hipStream_t s) {
+    if (grad)
+        hipLaunchKernelGGL((sym_bsum_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, rowpart,
+                           colpart, bsum);
+    else
+        hipLaunchKernelGGL((sym_bsum_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, rowpart,
+                           colpart, bsum);
+}
+
+} // namespace kern
+} // namespace hph

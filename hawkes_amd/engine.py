@@ -109,7 +109,7 @@ class Engine:
         return (ll, g) if gradient else ll
 
     def set_pair_kernel(self, kind):
-        """_lib.PAIR_KERNEL_AUTO / PAIR_KERNEL_ROWWISE / PAIR_KERNEL_SYMMETRIC."""
+        """_lib.PAIR_KERNEL_AUTO / PAIR_KERNEL_ROWWISE / PAIR_KERNEL_SYMMETRIC / PAIR_KERNEL_SPLIT."""
         check(_lib.lib().hph_set_pair_kernel(self._h, int(kind)))
 
     def set_pair_config(self, kind, rows_per_lane=2, target_blocks=4096):

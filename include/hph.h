@@ -118,12 +118,18 @@ int hph_set_timing(hph_engine *engine, int enabled);
 int hph_get_timing(hph_engine *engine, double *out, size_t length);
 
 /* ---- Pair-pass selection (extension) ----
- * AUTO: symmetric pass for dimensions 1..3 and at least 4096 events, row-wise pass otherwise.
+ * AUTO: the tuned or built-in choice among the others (row-wise below ~12k events or above
+ * dimension 3; symmetric above; split for dimensions 1..3, at least 65,536 events and locations
+ * spread over at least 3 times the distance at which background terms become negligible).
  * ROWWISE: every ordered pair, one event row per lane. SYMMETRIC: background terms once per
- * unordered pair (dimensions 1..3). Results agree to rounding; each is deterministic. */
+ * unordered pair (dimensions 1..3). SPLIT: the symmetric background terms in a pass of their own
+ * over events reordered within time windows along a space-filling curve, skipping groups whose
+ * bounding boxes are far apart, then the excitation terms in a second pass (dimensions 1..3).
+ * Results agree to rounding; each is deterministic. */
 #define HPH_PAIR_KERNEL_AUTO 0
 #define HPH_PAIR_KERNEL_ROWWISE 1
 #define HPH_PAIR_KERNEL_SYMMETRIC 2
+#define HPH_PAIR_KERNEL_SPLIT 3
 int hph_set_pair_kernel(hph_engine *engine, int kind);
 /* Full form: rows_per_lane (1 or 2) applies to the symmetric pass; target_blocks (64..2^20) is
  * the grid size the work decomposition aims for. hph_set_pair_kernel(e, k) is

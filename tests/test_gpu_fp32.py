@@ -22,7 +22,7 @@ TOL = 1e-6
 
 CONFIGS = {"rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096), "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
            "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
-           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536), "split": (_lib.PAIR_KERNEL_SPLIT, 2, 4096)}
 
 
 def fp32_engine(locs, times, params, kernel):
@@ -43,7 +43,7 @@ def errors(e, ref):
     return float(ll_err), float(g_err.max()), float(p_err.max())
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow", "split"])
 def test_fp32_cluster(kernel):
     locs, times = synth.cluster_process(3500, seed=123)
     ref = oracle.total(locs, times, synth.P0, probs=True)
@@ -78,7 +78,7 @@ def test_fp32_config2_size():
     assert ll_err <= TOL and g_err <= TOL and p_err <= TOL
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_fp32_tile_skip_matches_all_pairs(kernel):
     """FP32-bulk with and without tile skipping (skipped background terms are below
     e^-(50 + ln N) relative to each event's sums): equal to within rounding."""

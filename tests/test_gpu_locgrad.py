@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 KERNELS = {"rowwise": (_lib.PAIR_KERNEL_ROWWISE, 2, 4096), "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
            "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
-           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536), "split": (_lib.PAIR_KERNEL_SPLIT, 2, 4096)}
 
 
 def engine(locs, times, params, flags=_lib.OPENCL, kernel="rowwise"):
@@ -44,7 +44,7 @@ def check(locs, times, params, name, flags=_lib.OPENCL, kernel="rowwise"):
     return g
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_golden_vectors(golden, kernel):
     for c in golden:
         if kernel != "rowwise" and c["D"] > 3:

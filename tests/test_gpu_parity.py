@@ -27,7 +27,8 @@ KERNELS = {"auto": (_lib.PAIR_KERNEL_AUTO, 2, 4096), "rowwise": (_lib.PAIR_KERNE
            "symmetric": (_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096),
            "symmetric_a1": (_lib.PAIR_KERNEL_SYMMETRIC, 1, 1024),
            "rowwise_small_grid": (_lib.PAIR_KERNEL_ROWWISE, 2, 64),
-           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536)}
+           "rowwise_narrow": (_lib.PAIR_KERNEL_ROWWISE, 2, 65536),
+           "split": (_lib.PAIR_KERNEL_SPLIT, 2, 4096), "split_a1": (_lib.PAIR_KERNEL_SPLIT, 1, 1024)}
 
 
 def make_engine(locs, times, params, kernel="auto", flags=_lib.OPENCL):
@@ -61,10 +62,10 @@ def assert_matches_oracle(locs, times, params, probs=True, precision="long", nam
     return ref
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_golden_vectors(golden, kernel):
     for c in golden:
-        if kernel == "symmetric" and c["D"] > 3:
+        if kernel in ("symmetric", "split") and c["D"] > 3:
             continue
         print("golden case", c["name"], kernel, flush=True)
         e = make_engine(c["locations"], c["times"], c["params"], kernel=kernel)
@@ -78,13 +79,14 @@ def test_golden_vectors(golden, kernel):
         e.close()
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_uniform_n1000_config(kernel):
     locs, times, p = synth.workload("uniform_n1000")
     assert_matches_oracle(locs, times, p, name="uniform_n1000", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_small_grid", "rowwise_narrow"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_small_grid", "rowwise_narrow",
+                                    "split", "split_a1"])
 def test_cluster_process_small(kernel):
     locs, times = synth.cluster_process(2800, side=np.sqrt(4000 / 100), seed=777)
     assert_matches_oracle(locs, times, synth.P0, name="cluster_4k", kernel=kernel)
@@ -92,7 +94,7 @@ def test_cluster_process_small(kernel):
 
 @pytest.mark.parametrize("D,kernel", [(1, "rowwise"), (3, "rowwise"), (5, "rowwise"), (8, "rowwise"),
                                       (1, "symmetric"), (3, "symmetric"), (1, "rowwise_narrow"),
-                                      (5, "rowwise_narrow")])
+                                      (5, "rowwise_narrow"), (1, "split"), (2, "split"), (3, "split")])
 def test_dimensions(D, kernel):
     rng = np.random.default_rng(100 + D)
     n = 1700
@@ -102,7 +104,7 @@ def test_dimensions(D, kernel):
     assert_matches_oracle(locs, times, p, name="D%d" % D, kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow", "split"])
 def test_unsorted_times_and_ties(kernel):
     rng = np.random.default_rng(21)
     n = 1900
@@ -112,7 +114,7 @@ def test_unsorted_times_and_ties(kernel):
     assert_matches_oracle(locs, times, p, name="unsorted_ties", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "rowwise_narrow", "split", "split_a1"])
 def test_row_counts_around_block_sizes(kernel):
     rng = np.random.default_rng(4)
     for n in (1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1025, 1537):
@@ -121,7 +123,7 @@ def test_row_counts_around_block_sizes(kernel):
         assert_matches_oracle(locs, times, [1.0, 0.9, 0.1, 0.5, 0.6, 0.9], name="n%d" % n, kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "rowwise_narrow", "split"])
 def test_batch_equals_single_calls_bitwise(kernel):
     locs, times = synth.cluster_process(1500, seed=31)
     P = synth.jitter_params(8)
@@ -142,7 +144,7 @@ def test_batch_equals_single_calls_bitwise(kernel):
     ("small_omega_large_tauT", [1.0, 1 / 1.6, 1 / 3.0, 0.05, 1.0, 1.0]),
     ("large_omega", [1.0, 1 / 1.6, 1 / 336.0, 40.0, 1.0, 1.0]),
 ])
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_tile_skip_matches_all_pairs(name, params, kernel):
     """Tiles are skipped where every background term is below e^-(50 + ln N) (each event's sums
     change by < 5e-20 relative; its own pair contributes 1) or, in likelihood and gradient passes,
@@ -163,7 +165,7 @@ def test_tile_skip_matches_all_pairs(name, params, kernel):
     np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-15, atol=0, err_msg=name)
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_probs_from_likelihood_pass_equal_fresh_pass(kernel):
     locs, times = synth.cluster_process(3000, seed=19)
     e = make_engine(locs, times, synth.P0, kernel=kernel)
@@ -179,7 +181,7 @@ def test_probs_from_likelihood_pass_equal_fresh_pass(kernel):
     e.close()
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_huge_exponents_use_the_clamped_exp(kernel):
     """Exponents far beyond 1e9 (tauT^2 dt^2 / 2 up to 5e9 here): the engine must select the
     clamped exp (the clamp-free shifter form is exact only below ~1.4e9)."""
@@ -191,7 +193,7 @@ def test_huge_exponents_use_the_clamped_exp(kernel):
     assert_matches_oracle(locs, times, p, name="huge", kernel=kernel)
 
 
-@pytest.mark.parametrize("kernel", ["symmetric", "symmetric_a1", "rowwise"])
+@pytest.mark.parametrize("kernel", ["symmetric", "symmetric_a1", "rowwise", "split"])
 def test_near_tied_times_keep_the_strict_mask(kernel):
     """Pairs of events one ulp apart in time excite each other (dt > 0), exact ties do not. Scaled
     times would merge the one-ulp pairs, so the engine must keep the original times for them; the
@@ -215,7 +217,7 @@ def test_near_tied_times_keep_the_strict_mask(kernel):
     ("sigma0", [0.0, 0.8, 0.05, 0.5, 0.7, 0.9]),
     ("negative_times", [1.1, 0.7, 0.04, 0.6, 0.8, 0.7]),
 ])
-@pytest.mark.parametrize("kernel", ["rowwise_narrow", "symmetric"])
+@pytest.mark.parametrize("kernel", ["rowwise_narrow", "symmetric", "split"])
 def test_boundary_parameters_and_negative_times(name, params, kernel):
     """theta = 0, omega = 0 or sigma = 0 switch the excitation term off (its constant is zero) or
     make it time-independent; times may be negative (Phi(-tauT t_i) > 1/2)."""
@@ -267,7 +269,7 @@ def test_config2_full_parity_and_finite_differences(config2_reference, kernel):
         assert abs(fd - g[k]) <= 1e-5 * max(abs(g[k]), 1e-3 * ref["scale"][1 + k]), (k, fd, g[k])
 
 
-@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1"])
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "symmetric_a1", "split"])
 def test_full_size_rows_against_oracle(kernel):
     """N ~ 1e5 (70,000 background events): per-event probabilities of sampled rows against
     the oracle's row sums over all N columns, and translation invariance of the total."""
@@ -289,12 +291,13 @@ def test_full_size_rows_against_oracle(kernel):
 def test_kernels_agree_at_full_size():
     locs, times, p = synth.workload("cluster_b70k")
     res = []
-    for kernel in ("rowwise", "symmetric"):
+    for kernel in ("rowwise", "symmetric", "split"):
         e = make_engine(locs, times, p, kernel=kernel)
         res.append(e.log_likelihood_and_gradient())
         e.close()
-    assert rel_err(res[1][0], res[0][0]) <= 1e-13
-    assert np.all(np.abs(res[1][1] - res[0][1]) <= 1e-12 * np.maximum(np.abs(res[0][1]), 1.0))
+    for r in res[1:]:
+        assert rel_err(r[0], res[0][0]) <= 1e-13
+        assert np.all(np.abs(r[1] - res[0][1]) <= 1e-12 * np.maximum(np.abs(res[0][1]), 1.0))
 
 
 def test_autotune_picks_a_candidate_and_keeps_parity(tmp_path, monkeypatch):

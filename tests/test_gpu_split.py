@@ -1,0 +1,129 @@
+"""Split passes (background terms on the culled order with bounding-box skips, then the excitation
+terms) on data spread over a domain much wider than the background kernel's reach.
+
+  * against the oracle (FP64: ll 1e-12 relative, gradient 1e-12 of each component's scale,
+    probabilities 1e-12 relative or 1e-300 absolute for subnormal ones) in D = 1, 2, 3, with
+    many windows of the culled order;
+  * against the every-pair evaluation (HPH_FLAG_ALL_PAIRS) to 1e-15: every skipped group pair
+    holds only background terms below e^-(50 + ln N);
+  * at N ~ 1.5e5 on 100 km: the built-in choice picks the split passes, which agree with the
+    symmetric pass (ll 1e-13, gradient 1e-12), FP32-bulk within 1e-6 of FP64, K-batches equal
+    to single calls to rounding.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from hawkes_amd import Engine, _lib, synth
+
+pytestmark = pytest.mark.gpu
+
+SPLIT = (_lib.PAIR_KERNEL_SPLIT, 2, 4096)
+
+
+def engine(locs, times, params, config=SPLIT, flags=_lib.OPENCL):
+    locs = np.asarray(locs, dtype=np.float64)
+    if locs.ndim == 1:
+        locs = locs[:, None]
+    e = Engine(locs.shape[1], locs.shape[0], flags=flags)
+    if config is not None:
+        e.set_pair_config(*config)
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters(params)
+    return e
+
+
+def spread(n, D, side, horizon, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, side, (n, D)), np.sort(rng.uniform(0.0, horizon, n))
+
+
+@pytest.mark.parametrize("D,side", [(1, 3000.0), (2, 400.0), (3, 120.0)])
+def test_spread_data_against_oracle(D, side):
+    locs, times = spread(4500, D, side, 8760.0, 300 + D)
+    p = synth.P0
+    ref = oracle.total(locs, times, p, probs=True)
+    e = engine(locs, times, p)
+    ll, g = e.log_likelihood_and_gradient()
+    print("D", D, "ll", ll, flush=True)
+    assert rel_err(ll, ref["ll"]) <= 1e-12
+    scale = ref["scale"][1:]
+    assert np.all(np.abs(g - ref["grad"]) <= 1e-12 * np.maximum(np.abs(ref["grad"]), 1e-3 * scale))
+    # events far from every earlier one have probabilities in the subnormal range (down to
+    # ~1e-321 here), where doubles hold a few bits: those agree to 1e-300 absolute
+    np.testing.assert_allclose(e.probs_self_excite(), ref["probs"], rtol=1e-12, atol=1e-300)
+    e.close()
+
+
+@pytest.mark.parametrize("A", [1, 2])
+def test_skipped_groups_match_every_pair(A):
+    locs, times = synth.cluster_process(7000, side=250.0, horizon=20_000.0, seed=81)
+    p = synth.P0
+    scale = oracle.total(locs, times, p, precision="double")["scale"][1:]
+    out = []
+    for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
+        e = engine(locs, times, p, config=(_lib.PAIR_KERNEL_SPLIT, A, 2048), flags=flags)
+        ll, g = e.log_likelihood_and_gradient()
+        out.append((ll, g, e.probs_self_excite()))
+        e.close()
+    assert rel_err(out[0][0], out[1][0]) <= 1e-15
+    assert np.all(np.abs(out[0][1] - out[1][1]) <= 1e-15 * scale)
+    np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-15, atol=1e-300)
+
+
+@pytest.fixture(scope="module")
+def wide():
+    locs, times = synth.cluster_process(105_000, side=100.0, horizon=17_520.0, seed=91)
+    print("N =", len(times), flush=True)
+    return locs, times
+
+
+def test_built_in_choice_and_agreement_with_symmetric(wide):
+    locs, times = wide
+    p = synth.P0
+    auto = engine(locs, times, p, config=None)
+    ll_a, g_a = auto.log_likelihood_and_gradient()
+    assert auto.tuned_config()[0] == _lib.PAIR_KERNEL_SPLIT
+    probs_a = auto.probs_self_excite()
+    auto.close()
+    sym = engine(locs, times, p, config=(_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096))
+    ll_s, g_s = sym.log_likelihood_and_gradient()
+    probs_s = sym.probs_self_excite()
+    sym.close()
+    assert rel_err(ll_a, ll_s) <= 1e-13
+    assert np.all(np.abs(g_a - g_s) <= 1e-12 * np.maximum(np.abs(g_s), 1e-3 * np.abs(g_s).max()))
+    np.testing.assert_allclose(probs_a, probs_s, rtol=1e-12, atol=1e-300)
+    # the benchmark's 20 km workload stays on the symmetric pass
+    l2, t2, p2 = synth.workload("cluster_b70k")
+    e2 = engine(l2, t2, p2, config=None)
+    e2.log_likelihood()
+    assert e2.tuned_config()[0] == _lib.PAIR_KERNEL_SYMMETRIC
+    e2.close()
+
+
+def test_fp32_bulk_and_batches(wide):
+    locs, times = wide
+    P = synth.jitter_params(4, seed=5)
+    e = engine(locs, times, P[0])
+    e.set_parameters_batch(P)
+    ll_b, g_b = e.log_likelihood_batch()
+    for k in range(4):
+        # the batch's work items split the columns differently (more vectors, shorter chunks):
+        # equal to rounding
+        e.set_parameters(P[k])
+        ll, g = e.log_likelihood_and_gradient()
+        assert rel_err(ll, ll_b[k]) <= 1e-14
+        assert np.all(np.abs(g - g_b[k]) <= 1e-13 * np.maximum(np.abs(g), 1e-3 * np.abs(g).max()))
+    e.set_parameters(P[0])
+    ll64, g64 = e.log_likelihood_and_gradient()
+    p64 = e.probs_self_excite()
+    e.close()
+    f = engine(locs, times, P[0], flags=_lib.OPENCL | _lib.FLOAT)
+    ll32, g32 = f.log_likelihood_and_gradient()
+    p32 = f.probs_self_excite()
+    f.close()
+    assert rel_err(ll32, ll64) <= 1e-6
+    assert np.all(np.abs(g32 - g64) <= 1e-6 * np.maximum(np.abs(g64), 1e-3 * np.abs(g64).max()))
+    assert np.max(np.abs(p32 - p64)) <= 1e-6
