@@ -155,7 +155,8 @@ class SymRunner {
     };
 
     ~SymRunner() {
-        for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_})
+        for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_,
+                          &d_flag_})
             b->release();
     }
 
@@ -272,6 +273,7 @@ class SymRunner {
             }
             d_row_.ensure(sizeof(double) * std::max<size_t>(plan_.rowpart_doubles, 1));
             d_col_.ensure(sizeof(double) * std::max<size_t>(plan_.colpart_doubles, 1));
+            d_flag_.ensure(std::max<size_t>(plan_.colflag_bytes, 1));
             d_block_.ensure(sizeof(double) * std::max<size_t>(plan_.block_partials, 1) * kern::kResultWidth * 2);
             refs_.clear();
             valid_ = true;
@@ -325,6 +327,7 @@ class SymRunner {
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
         L.res_items = (int)plan_.res_items.size();
+        L.colflag = d_flag_.as<unsigned char>();
         return L;
     }
 
@@ -335,7 +338,7 @@ class SymRunner {
     std::vector<DataRef> refs_;
     kern::SymData b_data_{};
     bool b_valid_ = false;
-    DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_;
+    DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_, d_flag_;
 };
 
 // Pair-pass configuration chosen by the autotuner.

@@ -143,6 +143,7 @@ struct SymData {
                             // the culled order to the time-sorted position
     long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
     long long col0, col_kstride; // column partials: the same
+    long long flag0, flag_kstride; // column-tile flags (bytes): the same
     long long blk0;         // index of its (k = 0, block 0) block partial
     int res0;               // index of its (k = 0) result
 };
@@ -165,6 +166,10 @@ struct SymLaunch {
     const int2 *res_item;   // (dataset, k)
     int res_items;
     int pm;                 // pass mode: kPassBoth, kPassCulledB or kPassExcitation
+    unsigned char *colflag; // per (earlier column tile t, block P), at flag0 + k flag_kstride +
+                            // t nb + P: 1 when the pass wrote the tile's column partials, 0 when it
+                            // skipped the tile (its partials are then zero and left unwritten);
+                            // readers skip unflagged tiles
 };
 
 // Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted
@@ -185,7 +190,7 @@ struct SymPlan {
     std::vector<SymData> data;   // pointers filled in by the caller
     std::vector<int4> items, fin_items;
     std::vector<int2> res_items;
-    size_t rowpart_doubles = 0, colpart_doubles = 0, block_partials = 0;
+    size_t rowpart_doubles = 0, colpart_doubles = 0, block_partials = 0, colflag_bytes = 0;
     int results = 0;
 };
 

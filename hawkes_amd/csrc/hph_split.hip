@@ -40,42 +40,46 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int QS = GRAD ? 3 : 1;
+    __shared__ int qlist[kQChunk];
+    __shared__ int qcount;
     const int4 it = L.fin_item[blockIdx.x];
-    if (it.x != 0) return;
+    if (it.x != 0) return; // uniform: the whole workgroup
     const SymData &ds = L.data[0];
     const int k = it.y;
     const int r = it.z * kRowsPerBlock + threadIdx.x;
-    if (r >= ds.n) return;
     const int R = L.R;
-    const int P = r / R;
-    const int nch = (P * R + ds.L - 1) / ds.L;
-    const size_t rpad = (size_t)ds.nb * R;
-    const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
-    double a[QS];
+    // column side: the workgroup's 256 positions are column tile it.z of every later block; the
+    // first kQChunk later blocks that wrote it are added with the row partials, the rest after
+    const int Pw = it.z * kRowsPerBlock / R;
+    const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+    const unsigned char *ftile = L.colflag + ds.flag0 + (size_t)k * ds.flag_kstride + (size_t)it.z * ds.nb;
+    const int cnt0 = flagged_blocks(ftile, Pw + 1, min(ds.nb, Pw + 1 + kQChunk), qlist, &qcount);
+    const bool live = r < ds.n;
+    double a[QS], cs[NC];
 #pragma unroll
     for (int q = 0; q < QS; ++q) a[q] = 0.0;
-#pragma unroll 4
-    for (int c = 0; c < nch; ++c) {
-        const double *rp = rbase + (size_t)c * NR * rpad;
-#pragma unroll
-        for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
-    }
-    {
-        const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
-#pragma unroll
-        for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
-    }
-    double cs[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) cs[c] = 0.0;
-    const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+    if (live) {
+        const int P = r / R;
+        const int nch = (P * R + ds.L - 1) / ds.L;
+        const size_t rpad = (size_t)ds.nb * R;
+        const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
 #pragma unroll 4
-    for (int Q = P + 1; Q < ds.nb; ++Q) {
-        const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
-        const size_t Q0 = (size_t)Q * R;
+        for (int c = 0; c < nch; ++c) {
+            const double *rp = rbase + (size_t)c * NR * rpad;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) cs[c] += cp[(size_t)c * Q0 + r];
+            for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
+        }
+        {
+            const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
+        }
+        add_listed_columns<NC>(cbase, R, r, qlist, cnt0, cs);
     }
+    add_remaining_columns<NC>(cbase, ftile, R, Pw, ds.nb, r, live, cs, qlist, &qcount);
+    if (!live) return;
     const int i = ds.perm[r];
 #pragma unroll
     for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = a[q] + cs[q];

@@ -14,9 +14,9 @@ __global__ void queue_reset_kernel(int *counter) { *counter = 0; }
 
 // Row sums = row-side chunks + diagonal block + column side of every later block. One block of
 // 256 rows of one (dataset, vector) per workgroup.
-// With bsum (split passes): the background moments come from bsum, the excitation moments from
-// the row partials of the excitation pass, and no column partials are read.
-template <bool GRAD>
+// SPLIT (bsum set, split passes): the background moments come from bsum, the excitation moments
+// from the row partials of the excitation pass, and no column partials are read.
+template <bool GRAD, bool SPLIT>
 __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L, const double *__restrict__ params,
                                                                      const double *__restrict__ rowpart,
                                                                      const double *__restrict__ colpart,
@@ -27,6 +27,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     __shared__ double2 sh[kResultWidth * kRowsPerBlock];
+    __shared__ int qlist[kQChunk];
+    __shared__ int qcount;
     const int4 it = L.fin_item[blockIdx.x];
     const SymData &ds = L.data[it.x];
     const int k = it.y, fb = it.z;
@@ -34,17 +36,27 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
     const int n = ds.n;
     const int R = L.R;
     double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // column side: the workgroup's 256 rows are column tile fb of every later block; the later
+    // blocks that wrote it are listed kQChunk at a time (the first list here, added below with
+    // the row partials, the rest after)
+    const int Pw = fb * kRowsPerBlock / R;
+    const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
+    const unsigned char *ftile = L.colflag + ds.flag0 + (size_t)k * ds.flag_kstride + (size_t)fb * ds.nb;
+    const int cnt0 = SPLIT ? 0 : flagged_blocks(ftile, Pw + 1, min(ds.nb, Pw + 1 + kQChunk), qlist, &qcount);
+    double cs[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cs[c] = 0.0;
+    double a6[GRAD ? 6 : 2];
     if (i < n) {
         const int P = i / R;
         const int nch = (P * R + ds.L - 1) / ds.L;
         const size_t rpad = (size_t)ds.nb * R;
         const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
-        double a6[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) a6[q] = 0.0;
         // moments read from the row partials: all, or the excitation ones after split passes
         constexpr int QS = GRAD ? 3 : 1;
-        const int q0 = bsum != nullptr ? QS : 0;
+        constexpr int q0 = SPLIT ? QS : 0;
         // unrolled so that several slots' loads are in flight (one grid of 256-row blocks is too
         // small to cover HBM latency with one load per thread)
 #pragma unroll 4
@@ -60,23 +72,16 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(SymLaunch L
             for (int q = 0; q < NR; ++q)
                 if (q >= q0) a6[q] += rp[(size_t)q * rpad + i];
         }
-        double cs[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) cs[c] = 0.0;
-        if (bsum != nullptr) {
+        if (SPLIT) {
             // background sums of the culled pass, already complete per event
 #pragma unroll
             for (int q = 0; q < QS; ++q) a6[q] = bsum[((size_t)k * QS + q) * ds.npad + i];
         } else {
-            const double *cbase = colpart + ds.col0 + (size_t)k * ds.col_kstride;
-#pragma unroll 4
-            for (int Q = P + 1; Q < ds.nb; ++Q) {
-                const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
-                const size_t Q0 = (size_t)Q * R;
-#pragma unroll
-                for (int c = 0; c < NC; ++c) cs[c] += cp[(size_t)c * Q0 + i];
-            }
+            add_listed_columns<NC>(cbase, R, i, qlist, cnt0, cs);
         }
+    }
+    if (!SPLIT) add_remaining_columns<NC>(cbase, ftile, R, Pw, ds.nb, i, i < n, cs, qlist, &qcount);
+    if (i < n) {
         double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (GRAD) {
             s[0] = a6[0] + cs[0];
@@ -180,7 +185,7 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     while (L > kTile && total_items(L) < target_items) L = ((L / 2 + kTile - 1) / kTile) * kTile;
 
     const int NR = grad ? 6 : 2, NC = grad ? 3 : 1;
-    size_t row0 = 0, col0 = 0, blk0 = 0;
+    size_t row0 = 0, col0 = 0, blk0 = 0, flag0 = 0;
     int res0 = 0, param0 = 0;
     struct Cost {
         long long cost;
@@ -201,10 +206,14 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
         s.row_kstride = (long long)((size_t)(s.max_chunks + 1) * NR * rpad);
         s.col0 = (long long)col0;
         s.col_kstride = (long long)((size_t)NC * R * ((size_t)nb[d] * (nb[d] - 1) / 2));
+        s.flag0 = (long long)flag0;
+        // flags [column tile][block]: tile t of the earlier columns (t < (nb - 1) R / kTile) by row block
+        s.flag_kstride = (long long)((size_t)std::max(nb[d] - 1, 1) * (R / kTile) * nb[d]);
         s.blk0 = (long long)blk0;
         s.res0 = res0;
         row0 += (size_t)K[d] * s.row_kstride;
         col0 += (size_t)K[d] * s.col_kstride;
+        flag0 += (size_t)K[d] * s.flag_kstride;
         blk0 += (size_t)K[d] * s.fin_blocks;
         res0 += K[d];
         param0 += K[d];
@@ -229,6 +238,7 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     for (const Cost &c : items) plan.items.push_back(c.item);
     plan.rowpart_doubles = row0;
     plan.colpart_doubles = col0;
+    plan.colflag_bytes = flag0;
     plan.block_partials = blk0;
     plan.results = res0;
     return plan;
@@ -261,12 +271,22 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                          const double *colpart, const double *bsum, double *block_out, double *probs,
                          double *inv_lam, hipStream_t s) {
-    if (grad)
-        hipLaunchKernelGGL((sym_finalize_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
-                           params, rowpart, colpart, bsum, block_out, probs, inv_lam);
-    else
-        hipLaunchKernelGGL((sym_finalize_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L,
-                           params, rowpart, colpart, bsum, block_out, probs, inv_lam);
+    const dim3 grid((unsigned)L.fin_items), block(kRowsPerBlock);
+    if (bsum != nullptr) {
+        if (grad)
+            hipLaunchKernelGGL((sym_finalize_kernel<true, true>), grid, block, 0, s, L, params, rowpart, colpart, bsum,
+                               block_out, probs, inv_lam);
+        else
+            hipLaunchKernelGGL((sym_finalize_kernel<false, true>), grid, block, 0, s, L, params, rowpart, colpart, bsum,
+                               block_out, probs, inv_lam);
+    } else {
+        if (grad)
+            hipLaunchKernelGGL((sym_finalize_kernel<true, false>), grid, block, 0, s, L, params, rowpart, colpart, bsum,
+                               block_out, probs, inv_lam);
+        else
+            hipLaunchKernelGGL((sym_finalize_kernel<false, false>), grid, block, 0, s, L, params, rowpart, colpart, bsum,
+                               block_out, probs, inv_lam);
+    }
 }
 
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s) {

@@ -194,6 +194,61 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
     }
 }
 
+// Column side of the symmetric pass, read back per event: the later blocks Q whose pass wrote
+// column tile t (flag set) are listed in ascending order, kQChunk at a time, so that the loads
+// of the listed blocks are independent (no branch on a flag between them).
+constexpr int kQChunk = 1024;
+
+// Lists the blocks Q in [qa, qb) (qb - qa <= kQChunk) whose flag is set into list (ascending)
+// and returns their count. ftile: the flags of column tile t, one byte per block Q (the
+// [tile][block] layout of SymLaunch::colflag). Called by every thread of the block; wave 0 builds
+// the list. The caller must barrier before the list is rebuilt.
+__device__ __forceinline__ int flagged_blocks(const unsigned char *__restrict__ ftile, int qa, int qb, int *list,
+                                              int *count) {
+    if (threadIdx.x < 64) {
+        int base = 0;
+        const unsigned long long below = (1ULL << threadIdx.x) - 1ULL;
+        for (int q0 = qa; q0 < qb; q0 += 64) {
+            const int Q = q0 + (int)threadIdx.x;
+            const bool f = Q < qb && ftile[Q] != 0;
+            const unsigned long long m = __ballot(f);
+            if (f) list[base + __popcll(m & below)] = Q;
+            base += __popcll(m);
+        }
+        if (threadIdx.x == 0) *count = base;
+    }
+    __syncthreads();
+    return *count;
+}
+
+// Adds the column partials of event i of the listed blocks (ascending) to cs.
+template <int NC>
+__device__ __forceinline__ void add_listed_columns(const double *__restrict__ cbase, int R, int i, const int *list,
+                                                   int cnt, double (&cs)[NC]) {
+#pragma unroll 4
+    for (int j = 0; j < cnt; ++j) {
+        const int Q = list[j];
+        const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
+        const size_t Q0 = (size_t)Q * R;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cs[c] += cp[(size_t)c * Q0 + i];
+    }
+}
+
+// Column sums of the later blocks Q >= P + 1 + kQChunk (the first kQChunk later blocks are
+// listed and added by the caller), ascending, skipping unflagged tiles. Every thread of the block
+// calls it; rows past n (live false) only take part in the barriers.
+template <int NC>
+__device__ __forceinline__ void add_remaining_columns(const double *__restrict__ cbase,
+                                                      const unsigned char *__restrict__ ftile, int R, int P, int nb,
+                                                      int i, bool live, double (&cs)[NC], int *list, int *count) {
+    for (int qa = P + 1 + kQChunk; qa < nb; qa += kQChunk) {
+        __syncthreads(); // every thread has read the previous list
+        const int cnt = flagged_blocks(ftile, qa, min(nb, qa + kQChunk), list, count);
+        if (live) add_listed_columns<NC>(cbase, R, i, list, cnt, cs);
+    }
+}
+
 // Bounding box of `count` consecutive groups of the culled order, starting at group g:
 // lo/hi[0..D-1] coordinates, lo/hi[D] time (gstat layout of SymData).
 template <int D>
@@ -321,6 +376,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         const int c1 = min(c0 + ds.L, P0);
         const bool tail = P0 + R > n;
         double *cp = colpart + ds.col0 + (size_t)k * ds.col_kstride + (size_t)NC * R * ((size_t)P * (P - 1) / 2);
+        // flag of column tile t for this block: cf[t * nb]
+        unsigned char *cf = L.colflag + ds.flag0 + (size_t)k * ds.flag_kstride + P;
         // the block's first row: the smallest time gap to any earlier tile is its gap
         const double t_blk = FAST ? (et[min(P0, n - 1)] - t0) * st : et[min(P0, n - 1)];
         // culled order: bounding boxes of the block's rows and of this wave's rows
@@ -338,8 +395,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 double tlo[D + 1], thi[D + 1];
                 group_box<D>(ds.gstat, j0 / kGroup, kTile / kGroup, tlo, thi);
                 if (box_background_exponent<D>(blo, bhi, tlo, thi, hx, ht) >= ds.zero_arg_b) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) cp[(size_t)c * P0 + j0 + tid] = 0.0;
+                    if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 0;
                     continue;
                 }
             }
@@ -351,12 +407,12 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             if (PM == kPassBoth && L.skip_zero) {
                 // Workgroup-wide skip, decided as the per-wave test below decides it for the
                 // first wave (whose gap is the smallest): when neither term survives there, it
-                // survives for no row of the block; the tile's column partials are then zero.
+                // survives for no row of the block; the tile's column partials are then zero
+                // (flagged, not written).
                 const double tl = FAST ? (et[j0 + kTile - 1] - t0) * st : et[j0 + kTile - 1];
                 const double gap = t_blk - tl;
                 if (tl < t_blk && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) cp[(size_t)c * P0 + j0 + tid] = 0.0;
+                    if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 0;
                     continue;
                 }
             }
@@ -424,6 +480,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                     for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
                     cp[(size_t)c * P0 + j0 + tid] = v;
                 }
+                if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 1;
             }
         }
     }
