@@ -10,7 +10,8 @@ resident in device memory before timing starts.
 
 Prints one JSON line. Options: --steps K --warmup W --gpus N (only 1 is supported)
 --batch K64 (0 skips the batched leg) --cpu-seconds S (0 skips the CPU baseline) --datasets D
-(0 skips the uneven dataset batch) --large 0|1 (the N ~ 1e6 leg) --legs (extra N ~ 1e5 legs)
+(0 skips the uneven dataset batch) --large 0|1 (the N ~ 1e6 leg) --legs (extra legs: all_pairs,
+fp32, small, locgrad, mh = the adaptive MH sampler on re-evaluated proposals)
 --config (a fixed pair-pass configuration instead of the autotuned one).
 
 value counts the N^2 ordered pairs the reference forms per call; the engine forms each
@@ -316,6 +317,54 @@ def large_dataset(n_leapfrog=20):
     return out
 
 
+def mh_sampler(n_iter=300):
+    """The reference's adaptive Metropolis-Hastings sampler (random scan over sigma, omega, theta,
+    mu0; tauX and tauT fixed, R/Untitled.R:254-422) on the N ~ 5e4 cluster workload: ms per
+    iteration (one log-likelihood per proposal) against a full log-likelihood evaluation. The
+    engine re-evaluates a proposal from the cached per-event sums: theta and mu0 moves reuse both
+    sums, sigma and omega moves reuse the background sums and run the excitation pass."""
+    from hawkes_amd import Engine, _lib, sampler, synth
+    locs, times, p0 = synth.workload("cluster_b35k")
+    n = len(times)
+    e = Engine(2, n)
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters(p0)
+    e.log_likelihood()
+    # full evaluations: tauX moves force the whole pass
+    full = []
+    for k in range(5):
+        p = p0.copy()
+        p[1] *= 1.0 + 0.01 * (k + 1)
+        e.set_parameters(p)
+        t0 = time.perf_counter()
+        e.log_likelihood()
+        full.append(time.perf_counter() - t0)
+    e.set_parameters(p0)
+    e.log_likelihood()
+    kinds = {_lib.EVAL_FULL: 0, _lib.EVAL_EXCITATION: 0, _lib.EVAL_CACHED: 0}
+    rng = np.random.default_rng(666)
+    t0 = time.perf_counter()
+    res = sampler.adaptive_mh(e, n_iter, p0, rng=rng)
+    wall = time.perf_counter() - t0
+    # the mix of evaluation kinds over a second run of the same length (not timed)
+    cur = np.array(p0, dtype=float)
+    for _ in range(100):
+        idx = sampler.MH_INDICES[rng.integers(len(sampler.MH_INDICES))]
+        cur[idx] *= float(np.exp(0.1 * rng.normal()))
+        e.set_parameters(cur)
+        e.log_likelihood()
+        kinds[e.last_evaluation()] += 1
+    e.close()
+    ms_full = 1e3 * float(np.median(full))
+    ms_iter = 1e3 * wall / n_iter
+    return {"N": n, "iterations": n_iter, "ms_per_iteration": ms_iter, "ms_full_loglik": ms_full,
+            "speedup_vs_full": ms_full / ms_iter, "acceptance": res["acceptance"],
+            "evaluation_kinds_per_100_moves": {"full": kinds[_lib.EVAL_FULL],
+                                               "excitation_pass": kinds[_lib.EVAL_EXCITATION],
+                                               "cached_sums": kinds[_lib.EVAL_CACHED]}}
+
+
 def small_n():
     """The N = 1000 workload (uniform on [0,10]^2 x [0, 8760) h): 100 repeated log-likelihood
     evaluations on the plain path, and K = 64 jittered proposals (LL + gradient) in one batched
@@ -367,8 +416,8 @@ def main():
     ap.add_argument("--workload", default="cluster_b70k")
     ap.add_argument("--datasets", type=int, default=32, help="uneven dataset batch size (0 skips the leg)")
     ap.add_argument("--large", type=int, default=1, help="1: N ~ 1e6 FP32-bulk + HMC leg, 0: skip")
-    ap.add_argument("--legs", default="all_pairs,fp32,small,locgrad",
-                    help="comma list of the extra legs to run (all_pairs, fp32, small, locgrad) or 'none'")
+    ap.add_argument("--legs", default="all_pairs,fp32,small,locgrad,mh",
+                    help="comma list of the extra legs to run (all_pairs, fp32, small, locgrad, mh) or 'none'")
     ap.add_argument("--config", default="auto",
                     help="pair-pass configuration: 'auto' (autotuned) or kind,rows_per_lane,target_grid with "
                          "kind sym or rowwise, e.g. sym,2,8192 (no autotuning; for profiling one kernel)")
@@ -521,6 +570,7 @@ def main():
     datasets = dataset_batch(args.datasets) if args.datasets > 0 else None
     large = large_dataset() if args.large else None
     small = small_n() if "small" in legs else None
+    mh = mh_sampler() if "mh" in legs else None
 
     cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
     cpu1 = cpu_config1(args.cpu_seconds) if args.cpu_seconds > 0 else None
@@ -549,6 +599,7 @@ def main():
         "large_n1e6": large,
         "small_n1000": small,
         "location_gradient": locgrad,
+        "mh_sampler": mh,
         "cpu_config1": cpu1,
     }
     print(json.dumps(out))

@@ -25,6 +25,7 @@ AVX512 = 1 << 7
 ALL_PAIRS = 1 << 16  # extension: no exact-zero tile skipping
 
 PAIR_KERNEL_AUTO, PAIR_KERNEL_ROWWISE, PAIR_KERNEL_SYMMETRIC, PAIR_KERNEL_SPLIT = 0, 1, 2, 3
+EVAL_FULL, EVAL_EXCITATION, EVAL_CACHED = 0, 1, 2
 
 # Every symbol include/hph.h declares, with (restype, argtypes).
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -51,6 +52,7 @@ SIGNATURES = {
     "hph_get_log_likelihood_batch": (ctypes.c_int, [_vp, _dp, _dp, _sz]),
     "hph_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_get_timing": (ctypes.c_int, [_vp, _dp, _sz]),
+    "hph_get_last_evaluation": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "hph_set_pair_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_set_pair_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hph_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),

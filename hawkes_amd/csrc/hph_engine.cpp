@@ -164,11 +164,12 @@ class SymRunner {
     // params: device, sum(K) x 6; result: device, sum(K) x 7; probs_dev, inv_lam_dev (may be null):
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
+    // sums (may be null): the per-event moments of dataset 0, vector 0 (natural units, [6][npad]).
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
                  bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, int cut_s,
                  const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
                  hipStream_t s, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr) {
+                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr) {
         if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
         kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
         L.cut_s = skip_zero && cut_s ? 1 : 0;
@@ -176,7 +177,7 @@ class SymRunner {
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "symmetric pair kernel launch");
         if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
-        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), nullptr,
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), nullptr, nullptr, sums,
                                   d_block_.as<double>(), probs_dev, inv_lam_dev, s);
         check(hipGetLastError(), "finalize kernel launch");
         kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
@@ -192,7 +193,7 @@ class SymRunner {
     void enqueue_split(const DataRef &sorted, const DataRef &culled, const double *gstat, int K, int dim,
                        const double *params, bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
                        int cut_s, bool fast, double *bsum, double *probs_dev, double *inv_lam_dev, double *result,
-                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, double *sums) {
         kern::SymLaunch L =
             prepare({sorted}, {K}, dim, grad, fp32, rows_per_lane, target_items, skip_zero, {fast ? (char)1 : (char)0});
         // the background pass's table: the sorted table with the culled order's arrays
@@ -216,15 +217,31 @@ class SymRunner {
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(Lb, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "background pass launch");
-        kern::launch_sym_bsums(Lb, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, s);
+        kern::launch_sym_bsums(Lb, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, s);
         check(hipGetLastError(), "background sums launch");
         kern::launch_sym(Ls, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "excitation pass launch");
         if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
-        kern::launch_sym_finalize(Ls, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum,
+        kern::launch_sym_finalize(Ls, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, nullptr, sums,
                                   d_block_.as<double>(), probs_dev, inv_lam_dev, s);
         check(hipGetLastError(), "finalize kernel launch");
         kern::launch_sym_reduce(Ls, grad, d_block_.as<double>(), result, s);
+        check(hipGetLastError(), "reduce kernel launch");
+        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+    }
+
+    // Finalize and reduction of one vector of one dataset from per-event moments in natural units:
+    // background moments from bsum ([3 or 1][npad]), excitation moments from scache ([3 or
+    // 1][npad]); the moments used are written to sums_out ([6][npad]). Arguments otherwise as
+    // enqueue; ev2 is recorded after the reduction.
+    void enqueue_from_sums(const DataRef &ref, int dim, const double *params, bool grad, bool fp32, int rows_per_lane,
+                           int target_items, const double *bsum, const double *scache, double *sums_out,
+                           double *probs_dev, double *inv_lam_dev, double *result, hipStream_t s, hipEvent_t ev2) {
+        const kern::SymLaunch L = prepare({ref}, {1}, dim, grad, fp32, rows_per_lane, target_items, 1, {0});
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, scache, sums_out,
+                                  d_block_.as<double>(), probs_dev, inv_lam_dev, s);
+        check(hipGetLastError(), "finalize kernel launch");
+        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
         check(hipGetLastError(), "reduce kernel launch");
         if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
     }
@@ -437,7 +454,8 @@ class HipHawkes : public AbstractHawkes {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_w_,
-                          &d_locgrad_, &d_xc_, &d_tc_, &d_cperm_, &d_gstat_, &d_bsum_})
+                          &d_locgrad_, &d_xc_, &d_tc_, &d_cperm_, &d_gstat_, &d_bsum_, &d_sums_[0], &d_sums_[1],
+                          &d_sband_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -740,6 +758,8 @@ class HipHawkes : public AbstractHawkes {
         forced_.target_blocks = target_blocks;
     }
 
+    int getLastEvaluation() override { return last_eval_; }
+
     void getTiming(double *out, size_t length) override {
         if (out == nullptr || length < 2) throw std::invalid_argument("timing output needs 2 values");
         out[0] = last_ms_[0];
@@ -782,6 +802,7 @@ class HipHawkes : public AbstractHawkes {
         extents_ = extents_of(locs_, ht, (size_t)N, D);
         data_dirty_ = false;
         culled_valid_ = false;
+        cache_valid_ = false;
     }
 
     // The culled order of the split passes: the time-sorted events cut into windows of
@@ -948,12 +969,14 @@ class HipHawkes : public AbstractHawkes {
     // h_result_ ([K][7]).
     // probs_host: also download the probabilities (implies write_probs); write_probs: the finalize
     // kernel writes vector 0's probabilities to d_probs_.
-    void run(const double *params, int K, bool grad, double *probs_host, bool write_probs = false) {
-        run_with(params, K, grad, probs_host, config_for(K, grad), write_probs);
+    // sums (may be null; K = 1): receives the per-event moments ([6][npad], natural units).
+    void run(const double *params, int K, bool grad, double *probs_host, bool write_probs = false,
+             double *sums = nullptr) {
+        run_with(params, K, grad, probs_host, config_for(K, grad), write_probs, sums);
     }
 
     void run_with(const double *params, int K, bool grad, double *probs_host, const TuneConfig &cfg_in,
-                  bool write_probs = false) {
+                  bool write_probs = false, double *sums = nullptr) {
         require_ready();
         check(hipSetDevice(device_), "hipSetDevice");
         if (data_dirty_) upload();
@@ -997,12 +1020,12 @@ class HipHawkes : public AbstractHawkes {
                                    grad, fp32_, cfg.rows_per_lane, cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
                                    last_fast_, d_bsum_.as<double>(), probs_dev, w_dev, d_result_.as<double>(), stream_,
                                    timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
-                                   timing_ ? ev_[2] : nullptr);
+                                   timing_ ? ev_[2] : nullptr, sums);
             } else {
                 sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
                              cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
                              probs_dev, w_dev, d_result_.as<double>(), stream_, timing_ ? ev_[0] : nullptr,
-                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr);
+                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr, sums);
             }
             last_split_ = split;
         } else {
@@ -1020,7 +1043,7 @@ class HipHawkes : public AbstractHawkes {
             check(hipGetLastError(), "pair kernel launch");
             if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
             kern::launch_finalize(ev, d_params_.as<double>(), K, g, grad, d_part_.as<double>(),
-                                  d_block_.as<double>(), probs_dev, w_dev, stream_);
+                                  d_block_.as<double>(), probs_dev, w_dev, sums, stream_);
             check(hipGetLastError(), "finalize kernel launch");
             kern::launch_reduce(ev, K, fin_blocks, grad, d_block_.as<double>(), d_result_.as<double>(), stream_);
             check(hipGetLastError(), "reduce kernel launch");
@@ -1046,23 +1069,106 @@ class HipHawkes : public AbstractHawkes {
         last_ms_[1] = b;
     }
 
+    // Re-evaluation from the per-event moments of the last full single-vector evaluation (the
+    // moments the reference's random-scan sampler leaves unchanged, R/Untitled.R:254-422, which
+    // proposes sigma, omega, theta and mu0 only): with tauX and tauT unchanged the background
+    // moments are reused; with sigma and omega unchanged as well the excitation moments are reused
+    // too (theta and mu0 enter only as coefficients), else an excitation pass forms them. Reused
+    // excitation moments were cut at the skip point of their own vector; they serve while the
+    // current vector's point is at most ln 2 above it (the cut then moves every rate by under
+    // 4e-22 relative). Dimensions 1..3; returns false when a full evaluation is needed.
+    bool evaluate_incremental(bool grad) {
+        if (!cache_valid_ || data_dirty_ || embeddingDimension > 3) return false;
+        if (grad && !cache_grad_) return false;
+        if (params_[1] != cache_params_[1] || params_[2] != cache_params_[2]) return false;
+        const double xs = kern::excitation_skip_arg(params_, embeddingDimension, locationCount);
+        const bool reuse_s = params_[0] == cache_params_[0] && params_[3] == cache_params_[3] &&
+                             xs <= cache_xs_ + 0.69314718055994530942;
+        TuneConfig cfg = config_for(1, grad);
+        if (cfg.kind != HPH_PAIR_KERNEL_SYMMETRIC && cfg.kind != HPH_PAIR_KERNEL_SPLIT) {
+            cfg.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+            cfg.rows_per_lane = locationCount < kSymA1MaxEvents ? 1 : 2;
+            cfg.target_blocks = 4096;
+        }
+        if (!sym_fits(1, cfg, grad)) return false;
+        check(hipSetDevice(device_), "hipSetDevice");
+        d_params_.ensure(sizeof(double) * 6);
+        const int fin_blocks = (locationCount + kern::kRowsPerBlock - 1) / kern::kRowsPerBlock;
+        d_block_.ensure(sizeof(double) * (size_t)fin_blocks * kern::kResultWidth * 2);
+        d_result_.ensure(sizeof(double) * kern::kResultWidth);
+        h_result_.ensure(sizeof(double) * kern::kResultWidth);
+        h_params_.ensure(sizeof(double) * 6);
+        d_w_.ensure(sizeof(double) * npad_);
+        std::memcpy(h_params_.ptr, params_, sizeof(params_));
+        check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6, hipMemcpyHostToDevice, stream_),
+              "upload params");
+        const int nxt = 1 - sums_cur_;
+        d_sums_[nxt].ensure(sizeof(double) * 6 * (size_t)npad_);
+        const kern::Events ev = events();
+        SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+        last_fast_ = false;
+        const double *cached = d_sums_[sums_cur_].as<double>();
+        const double *scache = cached + 3 * (size_t)npad_;
+        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        if (!reuse_s) {
+            d_sband_.ensure(sizeof(double) * 3 * (size_t)npad_);
+            kern::launch_excitation_band(ev, d_params_.as<double>(), grad, d_sband_.as<double>(), stream_);
+            check(hipGetLastError(), "excitation pass launch");
+            scache = d_sband_.as<double>();
+        }
+        if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+        sym_.enqueue_from_sums(ref, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
+                               cfg.target_blocks, cached, scache, d_sums_[nxt].as<double>(), d_probs_.as<double>(),
+                               d_w_.as<double>(), d_result_.as<double>(), stream_, timing_ ? ev_[2] : nullptr);
+        check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth, hipMemcpyDeviceToHost,
+                             stream_),
+              "download result");
+        check(hipStreamSynchronize(stream_), "stream sync");
+        if (timing_) record_timing();
+        sums_cur_ = nxt;
+        last_eval_ = reuse_s ? HPH_EVAL_CACHED : HPH_EVAL_EXCITATION;
+        if (!grad) cache_grad_ = false; // only moments 0 and 3 were written
+        std::memcpy(cache_params_, params_, sizeof(params_));
+        if (!reuse_s) cache_xs_ = xs;
+        return true;
+    }
+
     // Single-vector evaluation; caches ll (and the gradient). The probabilities it writes are kept
     // only when the pass skipped exact zeros alone: a request for probabilities runs such a pass
     // (excitation_skip_arg bounds the cut absolutely, not relatively, for tiny probabilities).
     void evaluate_single(bool grad, double *probs_host) {
         if (!have_params_) throw NotInitialized("parameters are not set (setParameters)");
         exact_s_ = probs_host != nullptr || (flags & ALL_PAIRS);
+        bool incremental = false;
         try {
-            run(params_, 1, grad, probs_host, true);
+            incremental = !exact_s_ && !force_natural_ && evaluate_incremental(grad);
+            if (!incremental) {
+                // a full evaluation; its per-event moments are kept for later ones
+                const bool keep = embeddingDimension <= 3 && !exact_s_;
+                double *sums = nullptr;
+                if (keep) {
+                    d_sums_[sums_cur_].ensure(sizeof(double) * 6 * (size_t)npad_);
+                    sums = d_sums_[sums_cur_].as<double>();
+                }
+                cache_valid_ = false;
+                run(params_, 1, grad, probs_host, true, sums);
+                last_eval_ = HPH_EVAL_FULL;
+                if (keep) {
+                    cache_valid_ = true;
+                    cache_grad_ = grad;
+                    std::memcpy(cache_params_, params_, sizeof(params_));
+                    cache_xs_ = kern::excitation_skip_arg(params_, embeddingDimension, locationCount);
+                }
+            }
         } catch (...) {
             exact_s_ = false;
             throw;
         }
         probs_valid_ = exact_s_;
         exact_s_ = false;
-        // the location gradient takes 1/lambda from natural-form FP64 passes only: its components
-        // cancel between neighbouring events, so it needs rates to about 1e-15
-        w_valid_ = !fp32_ && !last_fast_;
+        // the location gradient takes 1/lambda from natural-form FP64 full passes only: its
+        // components cancel between neighbouring events, so it needs rates to about 1e-15
+        w_valid_ = !fp32_ && !last_fast_ && !incremental;
         const double *r = h_result_.as<double>();
         ll_ = r[0];
         ll_valid_ = true;
@@ -1127,6 +1233,14 @@ class HipHawkes : public AbstractHawkes {
     // culled order of the split passes (built on first use after a data change)
     bool culled_valid_ = false;
     DevBuf d_xc_, d_tc_, d_cperm_, d_gstat_, d_bsum_;
+    // per-event moments of the last single-vector evaluation (two buffers: one read, one written)
+    DevBuf d_sums_[2];
+    DevBuf d_sband_; // excitation sums of a re-evaluation ([3][npad])
+    int sums_cur_ = 0;
+    bool cache_valid_ = false, cache_grad_ = false;
+    double cache_params_[6] = {0, 0, 0, 0, 0, 0};
+    double cache_xs_ = 0.0; // excitation skip point the cached excitation moments were cut at
+    int last_eval_ = HPH_EVAL_FULL;
 };
 
 
@@ -1539,6 +1653,15 @@ int hph_set_pair_kernel(hph_engine *e, int kind) {
 int hph_get_timing(hph_engine *e, double *out, size_t length) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->getTiming(out, length); });
+}
+
+int hph_get_last_evaluation(hph_engine *e, int *kind) {
+    HPH_REQUIRE_ENGINE(e);
+    if (kind == nullptr) {
+        g_last_error = "kind pointer is null";
+        return HPH_ERR_INVALID_ARGUMENT;
+    }
+    return guarded([&] { *kind = e->impl->getLastEvaluation(); });
 }
 
 struct hph_batch {

@@ -97,9 +97,12 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
 
 // Per-row log-likelihood terms and gradient terms, reduced per block in double-double.
 // block_out layout: [K][row_blocks][7][2]. probs (may be null) receives p_i for k == 0 in the
-// caller's order, inv_lam (may be null) 1 / lambda_i for k == 0 in sorted order.
+// caller's order, inv_lam (may be null) 1 / lambda_i for k == 0 in sorted order, sums (may be
+// null) the six moments per event of k == 0 ([6][npad], natural units; without the gradient
+// moments 0 and 3).
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
-                     const double *part, double *block_out, double *probs, double *inv_lam, hipStream_t s);
+                     const double *part, double *block_out, double *probs, double *inv_lam, double *sums,
+                     hipStream_t s);
 
 // Sums the per-block partials (nblocks finalize blocks of 256 rows) per k in a fixed order:
 // result layout [K][7] (only [k][0] without grad).
@@ -107,6 +110,11 @@ void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double
                    hipStream_t s);
 
 // ---- Location gradient (hph_locgrad.hip) ----
+// Excitation sums per event of vector params (dimensions 1..3), natural units, out [3 or 1][npad]
+// (S, S r^2, S dt; without the gradient S only): earlier events within the likelihood passes'
+// excitation cut of each 256-event block, per-pair dt > 0 mask.
+void launch_excitation_band(const Events &ev, const double *params, bool grad, double *out, hipStream_t s);
+
 // w[i] = 1 / lambda_i (sorted order, npad entries) from a rate pass (launch_pair without
 // gradient, K = 1, grid g) whose partials are rate_part.
 void launch_inv_rate(const Events &ev, const double *params, const PairGrid &g, const double *rate_part, double *w,
@@ -201,17 +209,20 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 // Finalize (per row, then per 256-row block in double-double) and reduction (per dataset and
 // vector: result[(res0 + k) * 7 + q]); probs and inv_lam (may be null) receive dataset 0,
 // vector 0 (as launch_finalize).
-// bsum (may be null): background sums per (k, moment, time-sorted event), written by
-// launch_sym_bsums; the finalize then takes the background moments from it and the excitation
-// moments from the row partials of a kPassExcitation pass.
+// bsum (may be null): background sums per (k, moment, time-sorted event) in natural units
+// (launch_sym_bsums, or a sums array of an earlier evaluation); the finalize then takes the
+// background moments from it and the excitation moments from the row partials of a
+// kPassExcitation pass, or, with scache (may be null; needs bsum), from per-event excitation sums
+// in natural units ([3 or 1][npad]). sums (may be null): receives the six moments per event of
+// dataset 0, vector 0 in natural units ([6][npad]; without the gradient moments 0 and 3).
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
-                         const double *colpart, const double *bsum, double *block_out, double *probs,
-                         double *inv_lam, hipStream_t s);
+                         const double *colpart, const double *bsum, const double *scache, double *sums,
+                         double *block_out, double *probs, double *inv_lam, hipStream_t s);
 // Background row sums of a kPassCulledB pass over dataset 0 (row partials + column partials),
-// scattered to time-sorted positions: bsum[(k * NB + q) * npad + sorted index], NB = 3 with the
-// gradient (sum, r^2 and dt^2 moments), else 1.
-void launch_sym_bsums(const SymLaunch &L, bool grad, const double *rowpart, const double *colpart, double *bsum,
-                      hipStream_t s);
+// scattered to time-sorted positions in natural units: bsum[(k * NB + q) * npad + sorted index],
+// NB = 3 with the gradient (sum, r^2 and dt^2 moments), else 1.
+void launch_sym_bsums(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
+                      const double *colpart, double *bsum, hipStream_t s);
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
 // Zeroes the work-queue counter (a one-thread kernel, so that the sequence can be graph-captured).
 void launch_queue_reset(const SymLaunch &L, hipStream_t s);

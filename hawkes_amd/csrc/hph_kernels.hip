@@ -232,7 +232,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
                                                                  PairGrid g, const double *__restrict__ part,
                                                                  double *__restrict__ block_out,
                                                                  double *__restrict__ probs,
-                                                                 double *__restrict__ inv_lam) {
+                                                                 double *__restrict__ inv_lam,
+                                                                 double *__restrict__ sums) {
     constexpr int NACC = GRAD ? 6 : 2;
     __shared__ double2 sh[kResultWidth * kRowsPerBlock];
     const int k = blockIdx.y;
@@ -257,6 +258,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
         } else {
             s[0] = a6[0];
             s[3] = a6[NACC - 1];
+        }
+        if (sums != nullptr && k == 0) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (GRAD || q == 0 || q == 3) sums[(size_t)q * npad + i] = s[q];
         }
         double lam;
         const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v, lam);
@@ -290,6 +296,86 @@ __global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nb
         a[0] = dd_add(a[0], make_double2(c, 0.0));
 #pragma unroll
         for (int q = 0; q < W; ++q) result[(size_t)k * kResultWidth + q] = a[q].x + a[q].y;
+    }
+}
+
+// Excitation sums per event alone, for re-evaluations whose background sums are cached: one
+// workgroup per 64 events (one per lane); the earlier events' 256-event tiles are visited from the
+// events' own tile backwards, each split over the four wavefronts (64 columns each), until every
+// remaining excitation term is past the likelihood passes' cut (excitation_skip_arg: omega times
+// the smallest time gap at or past X_S). Natural form, FP64, per-pair dt > 0 mask (ties and later
+// events contribute nothing); the wavefronts' sums are added in a fixed order. out: [3 or 1][npad]
+// in natural units (S, S r^2, S dt).
+template <int D, bool GRAD>
+__global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events ev, const double *__restrict__ params,
+                                                                        double *__restrict__ out) {
+    constexpr int ST = TileStride<D>::value;
+    constexpr int NQ = GRAD ? 3 : 1;
+    __shared__ double tile[kTile * ST];
+    __shared__ double red[3][NQ][kNarrow];
+    const int rg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n = ev.n, npad = ev.npad;
+    const int i = rg * kNarrow + lane;
+    const int ic = min(i, n - 1);
+    const double sigma = params[0], omega = params[3];
+    const double cSx = -0.5 * sigma * sigma;
+    const double xs = excitation_skip_arg(params, D, n);
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = ev.x[(size_t)d * npad + ic];
+    const double ti = ev.t[ic];
+    const double t_first = ev.t[rg * kNarrow]; // the group's smallest time (sorted)
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int t_own = rg * kNarrow / kTile;    // the tile holding the group's rows
+    for (int tj = t_own; tj >= 0; --tj) {
+        const int j0 = tj * kTile;
+        if (tj < t_own && omega >= 0.0 && omega * (t_first - ev.t[j0 + kTile - 1]) >= xs) break; // uniform
+        const int cnt_all = min(kTile, n - j0);
+        __syncthreads();
+        if (tid < cnt_all) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) tile[tid * ST + d] = ev.x[(size_t)d * npad + j0 + tid];
+            tile[tid * ST + D] = ev.t[j0 + tid];
+        }
+        __syncthreads();
+        const int cnt = min(kNarrow, cnt_all - w * kNarrow); // this wavefront's columns (may be <= 0)
+        const double *sub = tile + w * kNarrow * ST;
+#pragma unroll 2
+        for (int j = 0; j < cnt; ++j) {
+            const double *e = sub + j * ST;
+            double r2;
+            {
+                const double d0 = xi[0] - e[0];
+                r2 = d0 * d0;
+            }
+#pragma unroll
+            for (int d = 1; d < D; ++d) {
+                const double dd = xi[d] - e[d];
+                r2 = __builtin_fma(dd, dd, r2);
+            }
+            const double dt = ti - e[D];
+            double es = exp_fp64(__builtin_fma(cSx, r2, -omega * dt));
+            es = dt > 0.0 ? es : 0.0;
+            acc[0] += es;
+            if (GRAD) {
+                acc[1] = __builtin_fma(es, r2, acc[1]);
+                acc[2] = __builtin_fma(es, dt, acc[2]);
+            }
+        }
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) red[w - 1][q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (w == 0 && i < n) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double v = acc[q];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) v += red[u][q][lane];
+            out[(size_t)q * npad + i] = v;
+        }
     }
 }
 
@@ -367,15 +453,29 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
         launch_pair_p<false>(ev, params, K, g, grad, part, s);
 }
 
+void launch_excitation_band(const Events &ev, const double *params, bool grad, double *out, hipStream_t s) {
+    const dim3 grid((unsigned)((ev.n + kNarrow - 1) / kNarrow));
+    switch (ev.dim * 2 + (grad ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((excitation_band_kernel<1, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 3: hipLaunchKernelGGL((excitation_band_kernel<1, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 4: hipLaunchKernelGGL((excitation_band_kernel<2, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 5: hipLaunchKernelGGL((excitation_band_kernel<2, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 6: hipLaunchKernelGGL((excitation_band_kernel<3, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 7: hipLaunchKernelGGL((excitation_band_kernel<3, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    default: throw std::invalid_argument("the excitation-only pass is built for dimensions 1..3");
+    }
+}
+
 void launch_finalize(const Events &ev, const double *params, int K, const PairGrid &g, bool grad,
-                     const double *part, double *block_out, double *probs, double *inv_lam, hipStream_t s) {
+                     const double *part, double *block_out, double *probs, double *inv_lam, double *sums,
+                     hipStream_t s) {
     dim3 grid((unsigned)g.row_blocks, (unsigned)K);
     if (grad)
         hipLaunchKernelGGL((finalize_kernel<true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
-                           block_out, probs, inv_lam);
+                           block_out, probs, inv_lam, sums);
     else
         hipLaunchKernelGGL((finalize_kernel<false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, g, part,
-                           block_out, probs, inv_lam);
+                           block_out, probs, inv_lam, sums);
 }
 
 void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double *block_out, double *result,

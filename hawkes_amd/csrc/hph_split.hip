@@ -34,7 +34,8 @@ void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool 
 // row-side chunks, its diagonal block and the column side of every later block, in a fixed
 // order, written to its time-sorted position perm[r]. One block of 256 positions per workgroup.
 template <bool GRAD>
-__global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, const double *__restrict__ rowpart,
+__global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, const double *__restrict__ params,
+                                                                 const double *__restrict__ rowpart,
                                                                  const double *__restrict__ colpart,
                                                                  double *__restrict__ bsum) {
     constexpr int NR = GRAD ? 6 : 2;
@@ -81,18 +82,25 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     add_remaining_columns<NC>(cbase, ftile, R, Pw, ds.nb, r, live, cs, qlist, &qcount);
     if (!live) return;
     const int i = ds.perm[r];
+    double b[QS];
 #pragma unroll
-    for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = a[q] + cs[q];
+    for (int q = 0; q < QS; ++q) b[q] = a[q] + cs[q];
+    if (GRAD && ds.fast) { // the base-2 pass summed e dt'^2 with dt' = st dt
+        const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
+        b[QS - 1] /= st * st;
+    }
+#pragma unroll
+    for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = b[q];
 }
 
-void launch_sym_bsums(const SymLaunch &L, bool grad, const double *rowpart, const double *colpart, double *bsum,
-                      hipStream_t s) {
+void launch_sym_bsums(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
+                      const double *colpart, double *bsum, hipStream_t s) {
     if (grad)
-        hipLaunchKernelGGL((sym_bsum_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, rowpart,
-                           colpart, bsum);
+        hipLaunchKernelGGL((sym_bsum_kernel<true>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, params,
+                           rowpart, colpart, bsum);
     else
-        hipLaunchKernelGGL((sym_bsum_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, rowpart,
-                           colpart, bsum);
+        hipLaunchKernelGGL((sym_bsum_kernel<false>), dim3((unsigned)L.fin_items), dim3(kRowsPerBlock), 0, s, L, params,
+                           rowpart, colpart, bsum);
 }
 
 } // namespace kern

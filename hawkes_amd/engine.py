@@ -132,6 +132,13 @@ class Engine:
     def set_timing(self, enabled=True):
         check(_lib.lib().hph_set_timing(self._h, 1 if enabled else 0))
 
+    def last_evaluation(self):
+        """How the last single-vector call was formed: _lib.EVAL_FULL, EVAL_EXCITATION (cached
+        background sums + excitation pass) or EVAL_CACHED (cached sums only)."""
+        out = ctypes.c_int()
+        check(_lib.lib().hph_get_last_evaluation(self._h, ctypes.byref(out)))
+        return out.value
+
     def timing(self):
         """(pair-pass ms, finalize + reduction ms) of the last evaluation."""
         out = np.zeros(2)

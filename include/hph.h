@@ -117,6 +117,20 @@ int hph_get_log_likelihood_batch(hph_engine *engine, double *loglik, double *gra
 int hph_set_timing(hph_engine *engine, int enabled);
 int hph_get_timing(hph_engine *engine, double *out, size_t length);
 
+/* ---- Re-evaluation from cached per-event sums (extension; no counterpart in the reference) ----
+ * A single-vector log-likelihood (or gradient) call keeps the per-event background and
+ * excitation sums of its parameters. A later call whose parameters differ only in sigmaXprec,
+ * omega, theta and mu0 (the parameters the reference's random-scan sampler proposes,
+ * R/Untitled.R:254-422) reuses the background sums and runs an excitation-only pass, or, when
+ * only theta and mu0 changed, reuses both (O(N)). Data changes, tauXprec or tauTprec changes,
+ * probability requests, batches and HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3.
+ * hph_get_last_evaluation: *kind = HPH_EVAL_FULL, HPH_EVAL_EXCITATION (cached background sums +
+ * excitation pass) or HPH_EVAL_CACHED (cached sums only) for the last single-vector call. */
+#define HPH_EVAL_FULL 0
+#define HPH_EVAL_EXCITATION 1
+#define HPH_EVAL_CACHED 2
+int hph_get_last_evaluation(hph_engine *engine, int *kind);
+
 /* ---- Pair-pass selection (extension) ----
  * AUTO: the tuned or built-in choice among the others (row-wise below ~12k events or above
  * dimension 3; symmetric above; split for dimensions 1..3, at least 65,536 events and locations
