@@ -50,7 +50,8 @@ class AbstractHawkes {
     virtual void autotune(int repeats, double *best_ms) = 0;
     virtual void getTunedConfig(int *out, size_t length) = 0;
     virtual void getTiming(double *out, size_t length) = 0;
-    // HPH_EVAL_FULL / HPH_EVAL_EXCITATION / HPH_EVAL_CACHED for the last single-vector call.
+    // How the last single-vector call was formed: 0 full pass, 1 excitation pass on cached
+    // background sums, 2 cached sums only (HPH_EVAL_* in hph.h).
     virtual int getLastEvaluation() = 0;
     // d ll / d locations, N x D row-major in the caller's order; the reference declares a location
     // gradient for Java callers (src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96) without an

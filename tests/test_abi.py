@@ -86,3 +86,13 @@ def test_create_without_device_reports_device_error():
     assert L.hph_create(2, 10, _lib.OPENCL, 0, 0, ctypes.byref(h)) == _lib.HPH_ERR_DEVICE
     assert b"device" in L.hph_last_error().lower()
     assert not h.value
+
+
+def test_cpp_caller_compiles_against_the_engine_header(tmp_path):
+    """A C++ caller of the reference's interface (factory, setTimesData, updateLocations,
+    setParameters, getSumOfLikContribs, getProbsSelfExcite) builds against include/hph_engine.hpp
+    and links against libhph.so with the host compiler alone."""
+    assert os.path.exists(LIB), "libhph.so not built"
+    out = str(tmp_path / "dropin")
+    subprocess.run(["bash", os.path.join(ROOT, "tests", "cpp", "build.sh"), out], check=True)
+    assert os.path.exists(out)
