@@ -330,7 +330,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
     for (int tj = t_own; tj >= 0; --tj) {
         const int j0 = tj * kTile;
         if (tj < t_own && omega >= 0.0 && omega * (t_first - ev.t[j0 + kTile - 1]) >= xs) break; // uniform
-        const int cnt_all = min(kTile, n - j0);
+        // columns up to the group's last row: later events excite none of its rows
+        const int cnt_all = min(min(kTile, n - j0), (rg + 1) * kNarrow - j0);
         __syncthreads();
         if (tid < cnt_all) {
 #pragma unroll
