@@ -545,8 +545,8 @@ class HipHawkes : public AbstractHawkes {
         evaluate_single(false, result);
     }
 
-    // w = 1/lambda from the last single-vector evaluation of the current state (else one is run,
-    // which also caches the log-likelihood, gradient and probabilities; FP32-bulk engines run a
+    // w = 1/lambda from the last single-vector evaluation of the current state (else a natural-form
+    // log-likelihood pass is run, which also caches the log-likelihood; FP32-bulk engines run a
     // row-wise FP64 rate pass instead), then the location-gradient pass of the engine's pass
     // configuration (symmetric for D <= 3, else row-wise); the result is downloaded in the
     // caller's event order. Timing: [0] = the rate pass run by this call (0
@@ -563,7 +563,7 @@ class HipHawkes : public AbstractHawkes {
         if (!w_valid_ && !fp32_) {
             force_natural_ = true;
             try {
-                evaluate_single(true, nullptr);
+                evaluate_single(false, nullptr);
             } catch (...) {
                 force_natural_ = false;
                 throw;
