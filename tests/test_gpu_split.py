@@ -127,3 +127,29 @@ def test_fp32_bulk_and_batches(wide):
     assert rel_err(ll32, ll64) <= 1e-6
     assert np.all(np.abs(g32 - g64) <= 1e-6 * np.maximum(np.abs(g64), 1e-3 * np.abs(g64).max()))
     assert np.max(np.abs(p32 - p64)) <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["one_location", "one_time", "two_clusters"])
+def test_degenerate_boxes_against_oracle(case):
+    """Boxes that collapse along an axis (every event at one location, or at one time) or split
+    the events into two far groups: the skips still hold only negligible terms."""
+    rng = np.random.default_rng(77)
+    n = 3000
+    if case == "one_location":
+        locs = np.zeros((n, 2)) + 3.0
+        times = np.sort(rng.uniform(0.0, 20_000.0, n))
+    elif case == "one_time":
+        locs = rng.uniform(0.0, 300.0, (n, 2))
+        times = np.full(n, 5.0)
+    else:
+        locs = np.concatenate([rng.normal(0.0, 1.0, (n // 2, 2)), rng.normal(500.0, 1.0, (n - n // 2, 2))])
+        times = np.sort(rng.uniform(0.0, 5_000.0, n))
+    p = synth.P0
+    ref = oracle.total(locs, times, p, probs=True)
+    e = engine(locs, times, p)
+    ll, g = e.log_likelihood_and_gradient()
+    assert rel_err(ll, ref["ll"]) <= 1e-12, case
+    scale = ref["scale"][1:]
+    assert np.all(np.abs(g - ref["grad"]) <= 1e-12 * np.maximum(np.abs(ref["grad"]), 1e-3 * scale)), case
+    np.testing.assert_allclose(e.probs_self_excite(), ref["probs"], rtol=1e-12, atol=1e-300, err_msg=case)
+    e.close()
