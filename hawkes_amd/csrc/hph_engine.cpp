@@ -187,13 +187,13 @@ class SymRunner {
 
     // The split passes over one dataset: the background pass on the culled order (culled: that
     // order's events, perm = culled position -> time-sorted position, gstat = its group boxes),
-    // its per-event sums into bsum (device, K x (grad ? 3 : 1) x npad), the excitation pass on the
-    // time-sorted order, then the finalize (background moments from bsum) and the reduction.
-    // Arguments otherwise as enqueue; ev1 is recorded after both passes.
+    // its per-event sums into bsum (device, K x (grad ? 3 : 1) x npad), the excitation sums per
+    // event of the time-sorted order into sband (same shape), then the finalize and the
+    // reduction. Arguments otherwise as enqueue; ev1 is recorded after both passes.
     void enqueue_split(const DataRef &sorted, const DataRef &culled, const double *gstat, int K, int dim,
                        const double *params, bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
-                       int cut_s, bool fast, double *bsum, double *probs_dev, double *inv_lam_dev, double *result,
-                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, double *sums) {
+                       int cut_s, bool fast, double *bsum, double *sband, double *probs_dev, double *inv_lam_dev,
+                       double *result, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, double *sums) {
         kern::SymLaunch L =
             prepare({sorted}, {K}, dim, grad, fp32, rows_per_lane, target_items, skip_zero, {fast ? (char)1 : (char)0});
         // the background pass's table: the sorted table with the culled order's arrays
@@ -211,21 +211,20 @@ class SymRunner {
         kern::SymLaunch Lb = L;
         Lb.data = d_data_b_.as<kern::SymData>();
         Lb.pm = kern::kPassCulledB;
-        kern::SymLaunch Ls = L;
-        Ls.pm = kern::kPassExcitation;
-        Ls.cut_s = skip_zero && cut_s ? 1 : 0;
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(Lb, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "background pass launch");
         kern::launch_sym_bsums(Lb, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, s);
         check(hipGetLastError(), "background sums launch");
-        kern::launch_sym(Ls, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
+        // the excitation terms of the time-sorted events, per event (FP64 in both precision modes)
+        kern::Events ev{sorted.x, sorted.t, sorted.perm, sorted.n, sorted.npad, dim, sorted.t_last};
+        kern::launch_excitation_band(ev, params, K, grad, !skip_zero ? 2 : (cut_s ? 0 : 1), sband, s);
         check(hipGetLastError(), "excitation pass launch");
         if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
-        kern::launch_sym_finalize(Ls, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, nullptr, sums,
+        kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, sband, sums,
                                   d_block_.as<double>(), probs_dev, inv_lam_dev, s);
         check(hipGetLastError(), "finalize kernel launch");
-        kern::launch_sym_reduce(Ls, grad, d_block_.as<double>(), result, s);
+        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
         check(hipGetLastError(), "reduce kernel launch");
         if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
     }
@@ -1014,11 +1013,13 @@ class HipHawkes : public AbstractHawkes {
             if (split) {
                 if (!culled_valid_) build_culled();
                 d_bsum_.ensure(sizeof(double) * (size_t)K * (grad ? 3 : 1) * npad_);
+                d_sband_.ensure(sizeof(double) * (size_t)K * 3 * npad_);
                 SymRunner::DataRef culled{d_xc_.as<double>(), d_tc_.as<double>(), d_cperm_.as<int>(), ev.n, ev.npad,
                                           ev.t_last};
                 sym_.enqueue_split(ref, culled, d_gstat_.as<double>(), K, embeddingDimension, d_params_.as<double>(),
                                    grad, fp32_, cfg.rows_per_lane, cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
-                                   last_fast_, d_bsum_.as<double>(), probs_dev, w_dev, d_result_.as<double>(), stream_,
+                                   last_fast_, d_bsum_.as<double>(), d_sband_.as<double>(), probs_dev, w_dev,
+                                   d_result_.as<double>(), stream_,
                                    timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
                                    timing_ ? ev_[2] : nullptr, sums);
             } else {
@@ -1112,7 +1113,7 @@ class HipHawkes : public AbstractHawkes {
         if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
         if (!reuse_s) {
             d_sband_.ensure(sizeof(double) * 3 * (size_t)npad_);
-            kern::launch_excitation_band(ev, d_params_.as<double>(), grad, d_sband_.as<double>(), stream_);
+            kern::launch_excitation_band(ev, d_params_.as<double>(), 1, grad, 0, d_sband_.as<double>(), stream_);
             check(hipGetLastError(), "excitation pass launch");
             scache = d_sband_.as<double>();
         }

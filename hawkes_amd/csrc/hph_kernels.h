@@ -110,10 +110,12 @@ void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double
                    hipStream_t s);
 
 // ---- Location gradient (hph_locgrad.hip) ----
-// Excitation sums per event of vector params (dimensions 1..3), natural units, out [3 or 1][npad]
-// (S, S r^2, S dt; without the gradient S only): earlier events within the likelihood passes'
-// excitation cut of each 256-event block, per-pair dt > 0 mask.
-void launch_excitation_band(const Events &ev, const double *params, bool grad, double *out, hipStream_t s);
+// Excitation sums per event of K vectors (dimensions 1..3), natural units, out [K][3 or 1][npad]
+// (S, S r^2, S dt; without the gradient S only) from the earlier events of each 64-event group,
+// per-pair dt > 0 mask. cut 0: within the likelihood passes' excitation cut; 1: within the
+// exact-zero point (probability requests); 2: every earlier event.
+void launch_excitation_band(const Events &ev, const double *params, int K, bool grad, int cut, double *out,
+                            hipStream_t s);
 
 // w[i] = 1 / lambda_i (sorted order, npad entries) from a rate pass (launch_pair without
 // gradient, K = 1, grid g) whose partials are rate_part.
@@ -173,7 +175,7 @@ struct SymLaunch {
     int fin_items;
     const int2 *res_item;   // (dataset, k)
     int res_items;
-    int pm;                 // pass mode: kPassBoth, kPassCulledB or kPassExcitation
+    int pm;                 // pass mode: kPassBoth or kPassCulledB
     unsigned char *colflag; // per (earlier column tile t, block P), at flag0 + k flag_kstride +
                             // t nb + P: 1 when the pass wrote the tile's column partials, 0 when it
                             // skipped the tile (its partials are then zero and left unwritten);
@@ -186,10 +188,9 @@ struct SymLaunch {
 // windows of consecutive events, each window ordered along a Morton curve of its locations, so
 // that groups of 64 events are compact in space and time) and skips every (wave, 64-column group)
 // and (block, tile) pair whose bounding boxes put every background exponent past the negligible
-// point; kPassExcitation forms the excitation terms of the time-sorted events alone.
+// point; the excitation terms then come from launch_excitation_band over the time-sorted events.
 constexpr int kPassBoth = 0;
 constexpr int kPassCulledB = 1;
-constexpr int kPassExcitation = 2;
 constexpr int kGroup = 64; // events per bounding-box group of the culled order
 
 // Host-side plan of a symmetric pass over datasets of sizes n[d] with K[d] vectors each.
@@ -209,12 +210,11 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 // Finalize (per row, then per 256-row block in double-double) and reduction (per dataset and
 // vector: result[(res0 + k) * 7 + q]); probs and inv_lam (may be null) receive dataset 0,
 // vector 0 (as launch_finalize).
-// bsum (may be null): background sums per (k, moment, time-sorted event) in natural units
-// (launch_sym_bsums, or a sums array of an earlier evaluation); the finalize then takes the
-// background moments from it and the excitation moments from the row partials of a
-// kPassExcitation pass, or, with scache (may be null; needs bsum), from per-event excitation sums
-// in natural units ([3 or 1][npad]). sums (may be null): receives the six moments per event of
-// dataset 0, vector 0 in natural units ([6][npad]; without the gradient moments 0 and 3).
+// bsum, scache (both null, or both set): per-event background and excitation sums in natural units
+// ([k][3 or 1][npad]; launch_sym_bsums / launch_excitation_band, or the sums of an earlier
+// evaluation); when set, the finalize takes the moments from them instead of the pass's partials.
+// sums (may be null): receives the six moments per event of dataset 0, vector 0 in natural units
+// ([6][npad]; without the gradient moments 0 and 3).
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                          const double *colpart, const double *bsum, const double *scache, double *sums,
                          double *block_out, double *probs, double *inv_lam, hipStream_t s);

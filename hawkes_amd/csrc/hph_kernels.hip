@@ -305,21 +305,25 @@ __global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nb
 // remaining excitation term is past the likelihood passes' cut (excitation_skip_arg: omega times
 // the smallest time gap at or past X_S). Natural form, FP64, per-pair dt > 0 mask (ties and later
 // events contribute nothing); the wavefronts' sums are added in a fixed order. out: [3 or 1][npad]
-// in natural units (S, S r^2, S dt).
+// in natural units (S, S r^2, S dt), per vector k (grid y): out + k NQ npad. cut: 0 the
+// likelihood passes' point X_S, 1 exactly zero terms only (omega gap >= 746, probability
+// requests), 2 none (every earlier event).
 template <int D, bool GRAD>
 __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events ev, const double *__restrict__ params,
-                                                                        double *__restrict__ out) {
+                                                                        int cut, double *__restrict__ out) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NQ = GRAD ? 3 : 1;
     __shared__ double tile[kTile * ST];
     __shared__ double red[3][NQ][kNarrow];
-    const int rg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int rg = blockIdx.x, k = blockIdx.y, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = ev.n, npad = ev.npad;
     const int i = rg * kNarrow + lane;
     const int ic = min(i, n - 1);
-    const double sigma = params[0], omega = params[3];
+    const double *pk = params + 6 * (size_t)k;
+    const double sigma = pk[0], omega = pk[3];
     const double cSx = -0.5 * sigma * sigma;
-    const double xs = excitation_skip_arg(params, D, n);
+    const double xs = cut == 0 ? excitation_skip_arg(pk, D, n) : (cut == 1 ? kZeroExpArg : HUGE_VAL);
+    out += (size_t)k * NQ * npad;
     double xi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = ev.x[(size_t)d * npad + ic];
@@ -454,15 +458,19 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
         launch_pair_p<false>(ev, params, K, g, grad, part, s);
 }
 
-void launch_excitation_band(const Events &ev, const double *params, bool grad, double *out, hipStream_t s) {
-    const dim3 grid((unsigned)((ev.n + kNarrow - 1) / kNarrow));
+void launch_excitation_band(const Events &ev, const double *params, int K, bool grad, int cut, double *out,
+                            hipStream_t s) {
+    if (K < 1 || K > 65535) throw std::invalid_argument("excitation pass: 1..65535 vectors");
+    if (cut < 0 || cut > 2) throw std::invalid_argument("excitation pass: cut mode 0, 1 or 2");
+    const dim3 grid((unsigned)((ev.n + kNarrow - 1) / kNarrow), (unsigned)K);
+    const int ex = cut;
     switch (ev.dim * 2 + (grad ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((excitation_band_kernel<1, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
-    case 3: hipLaunchKernelGGL((excitation_band_kernel<1, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
-    case 4: hipLaunchKernelGGL((excitation_band_kernel<2, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
-    case 5: hipLaunchKernelGGL((excitation_band_kernel<2, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
-    case 6: hipLaunchKernelGGL((excitation_band_kernel<3, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
-    case 7: hipLaunchKernelGGL((excitation_band_kernel<3, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, out); break;
+    case 2: hipLaunchKernelGGL((excitation_band_kernel<1, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
+    case 3: hipLaunchKernelGGL((excitation_band_kernel<1, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
+    case 4: hipLaunchKernelGGL((excitation_band_kernel<2, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
+    case 5: hipLaunchKernelGGL((excitation_band_kernel<2, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
+    case 6: hipLaunchKernelGGL((excitation_band_kernel<3, false>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
+    case 7: hipLaunchKernelGGL((excitation_band_kernel<3, true>), grid, dim3(kRowsPerBlock), 0, s, ev, params, ex, out); break;
     default: throw std::invalid_argument("the excitation-only pass is built for dimensions 1..3");
     }
 }

@@ -1,6 +1,6 @@
-// Split symmetric passes for spatially extended data: the background terms on the culled order
-// (bounding-box skips) and the excitation terms on the time-sorted order, as two passes, and
-// the kernel that turns the background pass's partials into per-event sums.
+// Split passes for spatially extended data: the background pass on the culled order (bounding-box
+// skips) and the kernel that turns its partials into per-event sums; the excitation sums come from
+// excitation_band_kernel (hph_kernels.hip).
 #include "hph_sym_device.h"
 
 #include <stdexcept>
@@ -18,12 +18,6 @@ void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool 
         case 1: launch_sym_p<false, 1, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
         case 2: launch_sym_p<true, 0, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
         default: launch_sym_p<true, 1, kPassCulledB>(L, params, grad, rowpart, colpart, s); break;
-        }
-    } else if (L.pm == kPassExcitation) {
-        // the excitation terms are FP64 in both precision modes
-        switch (L.fast) {
-        case 0: launch_sym_p<false, 0, kPassExcitation>(L, params, grad, rowpart, colpart, s); break;
-        default: launch_sym_p<false, 1, kPassExcitation>(L, params, grad, rowpart, colpart, s); break;
         }
     } else {
         throw std::invalid_argument("unknown pass mode");

@@ -15,12 +15,11 @@ __global__ void queue_reset_kernel(int *counter) { *counter = 0; }
 // Row sums = row-side chunks + diagonal block + column side of every later block. One block of
 // 256 rows of one (dataset, vector) per workgroup.
 // Moment sources: SPLIT false: all six from the pass's row and column partials. SPLIT true: the
-// background moments from bsum (per-event sums in natural units, [k][3 or 1][npad]) and no column
-// partials read; the excitation moments from the row partials of an excitation pass (SCACHE
-// false) or from scache (per-event sums in natural units, [3 or 1][npad], vector 0; SCACHE true).
+// background moments from bsum and the excitation moments from scache, per-event sums in natural
+// units ([k][3 or 1][npad] each); no partials are read.
 // sums (may be null): the six moments of dataset 0, vector 0 per event in natural units
 // ([6][npad]; without the gradient only moments 0 and 3), for later evaluations that reuse them.
-template <bool GRAD, bool SPLIT, bool SCACHE>
+template <bool GRAD, bool SPLIT>
 __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
     SymLaunch L, const double *__restrict__ params, const double *__restrict__ rowpart,
     const double *__restrict__ colpart, const double *__restrict__ bsum, const double *__restrict__ scache,
@@ -53,36 +52,32 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
 #pragma unroll
     for (int q = 0; q < NR; ++q) a6[q] = 0.0;
     if (i < n) {
-        if (!SCACHE) {
+        if (!SPLIT) {
             const int P = i / R;
             const int nch = (P * R + ds.L - 1) / ds.L;
             const size_t rpad = (size_t)ds.nb * R;
             const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
-            // moments read from the row partials: all, or the excitation ones after split passes
-            constexpr int q0 = SPLIT ? NR - QS : 0;
             // unrolled so that several slots' loads are in flight (one grid of 256-row blocks is
             // too small to cover HBM latency with one load per thread)
 #pragma unroll 4
             for (int c = 0; c < nch; ++c) {
                 const double *rp = rbase + (size_t)c * NR * rpad;
 #pragma unroll
-                for (int q = q0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
+                for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
             }
             {
                 const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
 #pragma unroll
-                for (int q = q0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
+                for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
             }
-        } else {
-#pragma unroll
-            for (int q = 0; q < QS; ++q) a6[NR - QS + q] = scache[(size_t)q * ds.npad + i];
-        }
-        if (SPLIT) {
-            // background sums per event, already complete
-#pragma unroll
-            for (int q = 0; q < QS; ++q) a6[q] = bsum[((size_t)k * QS + q) * ds.npad + i];
-        } else {
             add_listed_columns<NC>(cbase, R, i, qlist, cnt0, cs);
+        } else {
+            // per-event sums, already complete
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                a6[q] = bsum[((size_t)k * QS + q) * ds.npad + i];
+                a6[NR - QS + q] = scache[((size_t)k * QS + q) * ds.npad + i];
+            }
         }
     }
     if (!SPLIT) add_remaining_columns<NC>(cbase, ftile, R, Pw, ds.nb, i, i < n, cs, qlist, &qcount);
@@ -95,11 +90,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
             s[3] = a6[3];
             s[4] = a6[4];
             s[5] = a6[5];
-            if (ds.fast && !SCACHE) {
-                // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt (the background sums of
-                // split passes arrive in natural units)
+            if (ds.fast && !SPLIT) { // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt
                 const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
-                if (!SPLIT) s[2] /= st * st;
+                s[2] /= st * st;
                 s[5] /= st;
             }
         } else {
@@ -280,29 +273,28 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     }
 }
 
-template <bool SPLIT, bool SCACHE>
+template <bool SPLIT>
 static void launch_fin(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                        const double *colpart, const double *bsum, const double *scache, double *sums,
                        double *block_out, double *probs, double *inv_lam, hipStream_t s) {
     const dim3 grid((unsigned)L.fin_items), block(kRowsPerBlock);
     if (grad)
-        hipLaunchKernelGGL((sym_finalize_kernel<true, SPLIT, SCACHE>), grid, block, 0, s, L, params, rowpart, colpart,
+        hipLaunchKernelGGL((sym_finalize_kernel<true, SPLIT>), grid, block, 0, s, L, params, rowpart, colpart,
                            bsum, scache, sums, block_out, probs, inv_lam);
     else
-        hipLaunchKernelGGL((sym_finalize_kernel<false, SPLIT, SCACHE>), grid, block, 0, s, L, params, rowpart,
+        hipLaunchKernelGGL((sym_finalize_kernel<false, SPLIT>), grid, block, 0, s, L, params, rowpart,
                            colpart, bsum, scache, sums, block_out, probs, inv_lam);
 }
 
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                          const double *colpart, const double *bsum, const double *scache, double *sums,
                          double *block_out, double *probs, double *inv_lam, hipStream_t s) {
-    if (scache != nullptr && bsum == nullptr) throw std::invalid_argument("cached excitation sums need background sums");
+    if ((bsum == nullptr) != (scache == nullptr))
+        throw std::invalid_argument("per-event background and excitation sums come together");
     if (bsum == nullptr)
-        launch_fin<false, false>(L, params, grad, rowpart, colpart, bsum, scache, sums, block_out, probs, inv_lam, s);
-    else if (scache == nullptr)
-        launch_fin<true, false>(L, params, grad, rowpart, colpart, bsum, scache, sums, block_out, probs, inv_lam, s);
+        launch_fin<false>(L, params, grad, rowpart, colpart, bsum, scache, sums, block_out, probs, inv_lam, s);
     else
-        launch_fin<true, true>(L, params, grad, rowpart, colpart, bsum, scache, sums, block_out, probs, inv_lam, s);
+        launch_fin<true>(L, params, grad, rowpart, colpart, bsum, scache, sums, block_out, probs, inv_lam, s);
 }
 
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s) {

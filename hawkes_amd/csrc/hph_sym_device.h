@@ -121,8 +121,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 }
 
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
-// the dt > 0 mask. PM: the terms of the pass mode (kPassBoth, kPassCulledB: background only,
-// kPassExcitation: excitation only).
+// the dt > 0 mask. PM: the terms of the pass mode (kPassBoth, or kPassCulledB: background only).
 template <int D, bool GRAD, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
                                           const PairConst &c, double (&acc)[6]) {
@@ -141,7 +140,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
             r2 = __builtin_fma(dd, dd, r2);
         }
         const double dt = ti - e[D];
-        if (PM != kPassExcitation) {
+        {
             double dt2;
             const double eb = background_term<F32, FAST>(c, r2, dt, dt2);
             acc[0] += eb;
@@ -162,7 +161,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     }
 }
 
-// mode = 4 * (background formed) + excitation mode; only the modes of pass mode PM are built.
+// mode = 4 * (background formed) + excitation mode; the culled background pass builds only its own.
 template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
@@ -173,16 +172,13 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
     }
     switch (mode) {
     case 4 + kSNone:
-        if constexpr (PM == kPassBoth)
-            sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case 4 + kSFull:
-        if constexpr (PM == kPassBoth)
-            sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case 4 + kSMixed:
-        if constexpr (PM == kPassBoth)
-            sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
         break;
     case kSFull:
         sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
@@ -288,8 +284,7 @@ __device__ __forceinline__ double box_background_exponent(const double (&alo)[D 
 // One work item of the symmetric pass: row block P of dataset it.x, vector it.w, against a chunk
 // of earlier columns (it.z >= 0) or its own diagonal block (it.z < 0). FAST: the base-2 form.
 // PM: pass mode (kPassBoth; kPassCulledB: background terms of the culled order, bounding-box
-// skips, events of a chunk need not be earlier than the block's; kPassExcitation: excitation
-// terms only, no column partials).
+// skips, events of a chunk need not be earlier than the block's).
 template <int D, bool GRAD, int A, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
                                          double *__restrict__ rowpart, double *__restrict__ colpart,
@@ -399,11 +394,6 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                     continue;
                 }
             }
-            if (PM == kPassExcitation && L.skip_zero) {
-                const double tl = FAST ? (et[j0 + kTile - 1] - t0) * st : et[j0 + kTile - 1];
-                const double gap = t_blk - tl;
-                if (tl < t_blk && gs >= 0.0 && gs * gap >= zs) continue;
-            }
             if (PM == kPassBoth && L.skip_zero) {
                 // Workgroup-wide skip, decided as the per-wave test below decides it for the
                 // first wave (whose gap is the smallest): when neither term survives there, it
@@ -437,7 +427,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 // need the mask
                 const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
                 int smode = PM == kPassCulledB ? kSNone : (tl < t_lo ? kSFull : kSMixed);
-                int dob = PM == kPassExcitation ? 0 : 1;
+                int dob = 1;
                 if (PM != kPassCulledB && L.skip_zero) {
                     const double gap = tl < t_lo ? t_lo - tl : 0.0;
                     if (gb * gap * gap >= zb) dob = 0;
@@ -460,28 +450,24 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                         sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, ca);
                     else
                         sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, ca);
-                    if (PM != kPassExcitation) {
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
-                    }
+                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
                 }
-            } else if (PM != kPassExcitation) {
+            } else {
 #pragma unroll
                 for (int sb = 0; sb < 4; ++sb)
 #pragma unroll
                     for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = 0.0;
             }
-            if (PM != kPassExcitation) {
-                __syncthreads();
+            __syncthreads();
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    double v = colbuf[(0 * NC + c) * kTile + tid];
+            for (int c = 0; c < NC; ++c) {
+                double v = colbuf[(0 * NC + c) * kTile + tid];
 #pragma unroll
-                    for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
-                    cp[(size_t)c * P0 + j0 + tid] = v;
-                }
-                if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 1;
+                for (int ww = 1; ww < kSymWaves; ++ww) v += colbuf[(ww * NC + c) * kTile + tid];
+                cp[(size_t)c * P0 + j0 + tid] = v;
             }
+            if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 1;
         }
     }
     const size_t rpad = (size_t)ds.nb * R;
@@ -554,7 +540,7 @@ static void launch_sym_p(const SymLaunch &L, const double *params, bool grad, do
     }
 }
 
-// The split passes (pass modes kPassCulledB and kPassExcitation; FM 0 or 1), built in hph_split.hip.
+// The culled background pass (pass mode kPassCulledB; FM 0 or 1), built in hph_split.hip.
 void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart,
                       double *colpart, hipStream_t s);
 
