@@ -8,7 +8,11 @@ ln2 is split as LN2_HI + LN2_LO with LN2_HI holding 32 significant bits, so k*LN
 for |k| < 2^21.
 exp2 (for exponents already scaled by log2 e):
     y = k + f,   k = nearest integer to y,   f = y - k exact,   2^y = 2^k * q(f),   q ~ 2^f
-on [-1/2, 1/2]. Both polynomials have value 1 at 0 exactly and the other coefficients minimise
+on [-1/2, 1/2].
+exp2 of a non-negative exponent (background terms of the base-2 pair form, 2^-y with y >= 0):
+    y = i + f,   i = trunc(y),   f = fract(y) in [0, 1) exact,   2^-y = 2^-i * s(f),   s ~ 2^-f
+on [0, 1] (the same relative error as q: 2^-f on [0, 1] is 2^-1/2 times 2^g on [-1/2, 1/2]).
+All three polynomials have value 1 at 0 exactly and the other coefficients minimise
 the weighted (relative) least-squares error on 3000 Chebyshev nodes at 40 digits, which lands
 close to the minimax polynomial. The script then measures the max relative error of the
 double-precision Horner evaluation with correctly rounded fma at 20000 points.
@@ -25,9 +29,11 @@ mp.mp.dps = 40
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def fit(degree, A, scale=1):
-    """p(r) = 1 + sum c_k r^k ~ e^(scale r) on [-A, A]."""
-    nodes = [A * mp.cos(mp.pi * (j + mp.mpf(0.5)) / 3000) for j in range(3000)]
+def fit(degree, A, scale=1, lo=None):
+    """p(r) = 1 + sum c_k r^k ~ e^(scale r) on [-A, A] (on [lo, A] when lo is given)."""
+    lo = -A if lo is None else lo
+    mid, half = (A + lo) / 2, (A - lo) / 2
+    nodes = [mid + half * mp.cos(mp.pi * (j + mp.mpf(0.5)) / 3000) for j in range(3000)]
     # unknowns c1..c_degree; minimise sum ((p(r) - e^(scale r)) / e^(scale r))^2
     rows, rhs = [], []
     for r in nodes:
@@ -76,10 +82,19 @@ def main():
         k = round(y)
         v = mp.ldexp(mp.mpf(horner(coef2_d, y - k)), k)
         worst2 = max(worst2, abs(float((v - mp.power(2, y)) / mp.power(2, y))))
+    coef3 = fit(degree, mp.mpf(1), -ln2, mp.mpf(0))
+    coef3_d = [float(c) for c in coef3]
+    worst3 = 0.0
+    for _ in range(20000):
+        y = random.choice((random.uniform(0.0, 4.0), random.uniform(0.0, 1070.0)))
+        i = int(y)
+        v = mp.ldexp(mp.mpf(horner(coef3_d, y - i)), -i)
+        worst3 = max(worst3, abs(float((v - mp.power(2, -y)) / mp.power(2, -y))))
     path = os.path.join(HERE, "hph_exp_coeffs.h")
     with open(path, "w") as f:
         f.write("// Generated by gen_exp_poly.py (degree %d); do not edit.\n" % degree)
-        f.write("// Measured max relative error of the FP64 evaluation: exp %.3e, exp2 %.3e\n" % (worst, worst2))
+        f.write("// Measured max relative error of the FP64 evaluation: exp %.3e, exp2 %.3e, exp2 of -y %.3e\n"
+                % (worst, worst2, worst3))
         f.write("#pragma once\n")
         f.write("#define HPH_EXP_DEGREE %d\n" % degree)
         f.write("#define HPH_LOG2E %r\n" % log2e)
@@ -89,7 +104,9 @@ def main():
             f.write("#define HPH_EXP_C%d %r\n" % (k, c))
         for k, c in enumerate(coef2_d):
             f.write("#define HPH_EXP2_C%d %r\n" % (k, c))
-    print("degree", degree, "max rel err exp %.3e exp2 %.3e" % (worst, worst2), "->", path)
+        for k, c in enumerate(coef3_d):
+            f.write("#define HPH_EXP2N_C%d %r\n" % (k, c))
+    print("degree", degree, "max rel err exp %.3e exp2 %.3e exp2 of -y %.3e" % (worst, worst2, worst3), "->", path)
 
 
 if __name__ == "__main__":

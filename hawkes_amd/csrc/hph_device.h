@@ -67,6 +67,28 @@ __device__ __forceinline__ double exp2_fp64(double y) {
     return __builtin_amdgcn_ldexp(p, ki);
 }
 
+// 2^-y for 0 <= y < 2^31 (the background exponents of the base-2 form, bounded below 1e9 by the
+// host as for exp2_fp64): y = i + f with i = trunc(y) (v_cvt_i32_f64 of -y gives -i) and
+// f = fract(y) in [0, 1) (v_fract_f64, exact for y >= 0); s(f) is the degree-10 polynomial of
+// gen_exp_poly.py for 2^-f on [0, 1] and v_ldexp_f64 applies 2^-i. Two operations for the
+// reduction where exp2_fp64 takes three; max relative error 5.0e-16; exactly 0 for y > 1075.
+__device__ __forceinline__ double exp2_neg_fp64(double y) {
+    const int mi = (int)(-y);
+    const double f = __builtin_amdgcn_fract(y);
+    double p = HPH_EXP2N_C10;
+    p = __builtin_fma(p, f, HPH_EXP2N_C9);
+    p = __builtin_fma(p, f, HPH_EXP2N_C8);
+    p = __builtin_fma(p, f, HPH_EXP2N_C7);
+    p = __builtin_fma(p, f, HPH_EXP2N_C6);
+    p = __builtin_fma(p, f, HPH_EXP2N_C5);
+    p = __builtin_fma(p, f, HPH_EXP2N_C4);
+    p = __builtin_fma(p, f, HPH_EXP2N_C3);
+    p = __builtin_fma(p, f, HPH_EXP2N_C2);
+    p = __builtin_fma(p, f, HPH_EXP2N_C1);
+    p = __builtin_fma(p, f, HPH_EXP2N_C0);
+    return __builtin_amdgcn_ldexp(p, mi);
+}
+
 // Scaled time of the base-2 pair form: t' = (t - t0) * st with st = |tauT| sqrt(log2(e) / 2), so
 // that the background exponent tauT^2 dt^2 / 2 becomes dt'^2 / log2(e) and 2^-(dt'^2) =
 // e^-(tauT^2 dt^2 / 2). Kernels and finalize compute st with this one expression.

@@ -48,7 +48,7 @@ __device__ __forceinline__ double background_term(const PairConst &c, double r2,
     if (FAST) {
         const double y = __builtin_fma(c.cB0, r2, dt2);
         if (F32) return (double)exp2_f32((float)(-y));
-        return exp2_fp64(-y);
+        return exp2_neg_fp64(y); // y >= 0: cB0, r^2 and dt'^2 are non-negative
     }
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
 }
@@ -69,8 +69,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                                             double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
     constexpr int NC = GRAD ? 3 : 1;
     const int src = ((lane + 1) & 63) << 2;
-#pragma unroll 2
-    for (int s = 0; s < 64; ++s) {
+    auto step = [&](int s) {
         // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]
         double q[D + 1];
 #pragma unroll
@@ -117,6 +116,15 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 #pragma unroll
             for (int c = 0; c < NC; ++c) ca[c] = lane_shift_down(ca[c], src);
         }
+    };
+    // the background-only loop (the bulk of a likelihood pass) is unrolled further: its
+    // per-iteration address and constant moves are then shared by more pairs
+    if constexpr (DOB && SMODE == kSNone) {
+#pragma unroll 8
+        for (int s = 0; s < 64; ++s) step(s);
+    } else {
+#pragma unroll 2
+        for (int s = 0; s < 64; ++s) step(s);
     }
 }
 
