@@ -12,11 +12,14 @@
 // The block's own columns (the diagonal block) are paired in both orders with the per-pair
 // dt > 0 mask, as in src/NewHawkes.hpp:520-547.
 //
-// Column sums inside a wavefront use a rotation: over 64 steps lane l pairs its rows with column
-// (l + s) mod 64 of a 64-column subtile and adds into a 3-double accumulator that moves one lane
-// down after every step (ds_bpermute), so after 64 steps lane l holds the sums of column l.
-// The subtile is stored twice in LDS, as structure of arrays, so the per-lane address (l + s)
-// needs no wrap and consecutive lanes read consecutive doubles (no bank conflicts).
+// Column sums inside a wavefront: over 64 steps lane l pairs its rows with column (l + s) mod 64
+// of a 64-column subtile, sums its rows' contributions to that column and adds them to slot
+// l + s of 128 LDS slots (ds_add_f64: no result to wait for; in one step the 64 lanes add to 64
+// different slots, so every slot receives its additions in step order). Column u's sum is then
+// slot u (steps s <= u) plus slot u + 64 (the steps after the wrap). Each wave has its own slots,
+// so the sums do not depend on the timing of the waves. The subtile is stored twice in LDS, as
+// structure of arrays, so the per-lane read address (l + s) needs no wrap either and
+// consecutive lanes read consecutive doubles (no bank conflicts).
 #pragma once
 
 #include "hph_device.h"
@@ -59,21 +62,35 @@ __device__ __forceinline__ double excitation_term(const PairConst &c, double r2,
     return FAST ? exp2_fp64(y) : exp_fp64(y);
 }
 
-// One 64-column subtile against this lane's A rows, rotating the column accumulators ca.
+// Adds v to an LDS double (ds_add_f64; the result is not read).
+__device__ __forceinline__ void lds_add(double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One 64-column subtile against this lane's A rows; the column contributions are added to the
+// wave's LDS slots cs (moment c, slot u in [0, 128) at cs[c * 128 + u]).
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
 // has rows past the last event, whose column contributions are masked by vm; F32: FP32-bulk
 // background exponential; FAST: the base-2 form of PairConst.
 template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&vm)[A], const PairConst &c,
-                                            double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
+                                            double (&acc)[A][6], double *cs) {
     constexpr int NC = GRAD ? 3 : 1;
-    const int src = ((lane + 1) & 63) << 2;
+    // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]; the next step's
+    // entry is read before this step's slot additions are issued (entry lane + 64 of the last
+    // step's read is in the stored copy)
+    double qn[D + 1];
+#pragma unroll
+    for (int d = 0; d <= D; ++d) qn[d] = sub[d * 128 + lane];
     auto step = [&](int s) {
-        // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]
         double q[D + 1];
 #pragma unroll
-        for (int d = 0; d <= D; ++d) q[d] = sub[d * 128 + lane + s];
+        for (int d = 0; d <= D; ++d) {
+            q[d] = qn[d];
+            qn[d] = sub[d * 128 + lane + s + 1];
+        }
+        double v[NC]; // this step's column contributions of the A rows
 #pragma unroll
         for (int a = 0; a < A; ++a) {
             double r2;
@@ -96,10 +113,18 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                     acc[a][2] = __builtin_fma(eb, dt2, acc[a][2]);
                 }
                 const double ec = TAIL ? eb * vm[a] : eb;
-                ca[0] += ec;
-                if (GRAD) {
-                    ca[1] = __builtin_fma(ec, r2, ca[1]);
-                    ca[2] = __builtin_fma(ec, dt2, ca[2]);
+                if (a == 0) {
+                    v[0] = ec;
+                    if (GRAD) {
+                        v[1] = ec * r2;
+                        v[2] = ec * dt2;
+                    }
+                } else {
+                    v[0] += ec;
+                    if (GRAD) {
+                        v[1] = __builtin_fma(ec, r2, v[1]);
+                        v[2] = __builtin_fma(ec, dt2, v[2]);
+                    }
                 }
             }
             if (SMODE != kSNone) {
@@ -114,7 +139,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
         }
         if (DOB) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) ca[c] = lane_shift_down(ca[c], src);
+            for (int c = 0; c < NC; ++c) lds_add(cs + c * 128 + lane + s, v[c]);
         }
     };
     // the background-only loop (the bulk of a likelihood pass) is unrolled further: its
@@ -173,26 +198,26 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
 template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
-                                             const PairConst &c, double (&acc)[A][6], double (&ca)[GRAD ? 3 : 1]) {
+                                             const PairConst &c, double (&acc)[A][6], double *cs) {
     if constexpr (PM == kPassCulledB) {
-        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         return;
     }
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, ca);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
         break;
     default: break; // every term of the subtile is exactly zero
     }
@@ -296,8 +321,8 @@ __device__ __forceinline__ double box_background_exponent(const double (&alo)[D 
 template <int D, bool GRAD, int A, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
                                          double *__restrict__ rowpart, double *__restrict__ colpart,
-                                         double *__restrict__ tile, double *__restrict__ colbuf, int tid, int lane,
-                                         int w) {
+                                         double *__restrict__ tile, double *__restrict__ colbuf, double *cs, int tid,
+                                         int lane, int w) {
     constexpr int ST = TileStride<D>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
@@ -450,16 +475,20 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                         group_box<D>(ds.gstat, (j0 + sb * 64) / kGroup, 1, slo, shi);
                         if (box_background_exponent<D>(wlo, whi, slo, shi, hx, ht) >= ds.zero_arg_b) mode = 0;
                     }
-                    double ca[NC];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) ca[c] = 0.0;
                     const double *sub = tile + sb * (D + 1) * 128;
                     if (tail)
-                        sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, ca);
+                        sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, cs);
                     else
-                        sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, ca);
+                        sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, cs);
+                    // column lane of the subtile: its slots before and after the wrap; the slots
+                    // are zeroed again for the next subtile (the wave's LDS operations take effect
+                    // in program order)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) colbuf[(w * NC + c) * kTile + sb * 64 + lane] = ca[c];
+                    for (int c = 0; c < NC; ++c) {
+                        colbuf[(w * NC + c) * kTile + sb * 64 + lane] = cs[c * 128 + lane] + cs[c * 128 + lane + 64];
+                        cs[c * 128 + lane] = 0.0;
+                        cs[c * 128 + lane + 64] = 0.0;
+                    }
                 }
             } else {
 #pragma unroll
@@ -507,9 +536,17 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x (D + 1) x 128 doubles
     __shared__ double tile[(2 * kTile * (D + 1) > kTile * ST) ? 2 * kTile * (D + 1) : kTile * ST];
     __shared__ double colbuf[kSymWaves * NC * kTile];
+    __shared__ double colslot[kSymWaves * NC * 128];
     __shared__ int s_item;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // this wave's column slots (sym_subtile), zero between subtiles
+    double *cs = colslot + w * NC * 128;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        cs[c * 128 + lane] = 0.0;
+        cs[c * 128 + lane + 64] = 0.0;
+    }
     for (;;) {
         if (tid == 0) s_item = atomicAdd(L.counter, 1);
         __syncthreads();
@@ -518,9 +555,9 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         if (idx >= L.items) break;
         const int4 it = L.item[idx];
         if (FM == 1 || (FM == 2 && L.data[it.x].fast))
-            sym_item<D, GRAD, A, F32, true, PM>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
+            sym_item<D, GRAD, A, F32, true, PM>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
         else
-            sym_item<D, GRAD, A, F32, false, PM>(L, it, params, rowpart, colpart, tile, colbuf, tid, lane, w);
+            sym_item<D, GRAD, A, F32, false, PM>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
     }
 }
 
