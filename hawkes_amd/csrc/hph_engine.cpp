@@ -119,17 +119,22 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
     return e;
 }
 
-// True when the pass may use the base-2 form (scaled times, clamp-free exp2): every background
-// (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt + sigma^2 r^2/2) exponent of the K
-// vectors is below 1e9 in size, the time scale st = |tauT| sqrt(log2(e)/2) lies in [1e-100, 1e100]
-// with omega log2(e) / st finite, and distinct times are at least 8 ulp(max |t|) apart: then
-// (t - t0) st, rounded, keeps every strict order and every tie of the times, so the dt > 0 mask
-// of the excitation term is decided as on the original times. Accuracy: the scaled times carry
-// rounding of order eps |t - t0| st, t0 = the block's first event, so an excitation exponent
-// omega dt is off by up to ~2 eps log2(e) |omega| (block time span); the form is used only while
-// that stays below 1e-13 (|omega| x span <= 300), i.e. excitation terms within 1e-13 relative.
+// True when the pass may use the base-2 form (scaled coordinates and times, clamp-free exp2):
+// every background (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt + sigma^2 r^2/2)
+// exponent of the K vectors is below 1e9 in size, the scales st = |tauT| sqrt(log2(e)/2) and
+// sx = |tauX| sqrt(log2(e)/2) lie in [1e-100, 1e100] with omega log2(e) / st and sigma^2 / tauX^2
+// finite, and distinct times are at least 8 ulp(max |t|) apart: then (t - t0) st, rounded, keeps
+// every strict order and every tie of the times, so the dt > 0 mask of the excitation term is
+// decided as on the original times. Accuracy: the scaled times carry rounding of order
+// eps |t - t0| st, t0 = the block's first event, so an excitation exponent omega dt is off by up
+// to ~2 eps log2(e) |omega| (block time span); the form is used only while that stays below
+// 1e-13 (|omega| x span <= 300), i.e. excitation terms within 1e-13 relative. Likewise the
+// scaled coordinates (x - x0) sx carry rounding of order eps |x - x0| sx, x0 = the block's first
+// event: a term whose spatial exponent is of order one is off by ~eps x (scale x extent) relative,
+// kept below 1e-13 by sx x diagonal <= 300 and sigma sqrt(log2(e)/2) x diagonal <= 300.
 static bool fast_ok(const Extents &e, const double *params, int K) {
     if (!e.gaps_ok) return false;
+    const double diag = std::sqrt(e.r2max);
     for (int k = 0; k < K; ++k) {
         const double *p = params + 6 * (size_t)k;
         if (!(std::fabs(p[3]) * e.block_span <= 300.0)) return false;
@@ -137,8 +142,11 @@ static bool fast_ok(const Extents &e, const double *params, int K) {
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
         if (!(aB < 1e9 && aS < 1e9)) return false;
         const double st = std::fabs(p[2]) * 0.84932180028801904272;
-        if (!(st >= 1e-100 && st <= 1e100)) return false;
+        const double sx = std::fabs(p[1]) * 0.84932180028801904272;
+        if (!(st >= 1e-100 && st <= 1e100 && sx >= 1e-100 && sx <= 1e100)) return false;
         if (!std::isfinite(p[3] * 1.4426950408889634 / st)) return false;
+        if (!std::isfinite(p[0] * p[0] / (p[1] * p[1]))) return false;
+        if (!(sx * diag <= 300.0 && std::fabs(p[0]) * 0.84932180028801904272 * diag <= 300.0)) return false;
     }
     return true;
 }

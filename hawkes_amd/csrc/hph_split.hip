@@ -79,9 +79,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     double b[QS];
 #pragma unroll
     for (int q = 0; q < QS; ++q) b[q] = a[q] + cs[q];
-    if (GRAD && ds.fast) { // the base-2 pass summed e dt'^2 with dt' = st dt
-        const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
-        b[QS - 1] /= st * st;
+    if (GRAD && ds.fast) { // the base-2 pass summed e y (y = r'^2 + dt'^2) and e dt'^2, r' = sx r, dt' = st dt
+        const double *pk = params + 6 * (size_t)(ds.param0 + k);
+        const double st = time_scale(pk[2]), sx = time_scale(pk[1]);
+        b[1] = (b[1] - b[2]) / (sx * sx);
+        b[2] /= st * st;
     }
 #pragma unroll
     for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = b[q];

@@ -90,9 +90,14 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
             s[3] = a6[3];
             s[4] = a6[4];
             s[5] = a6[5];
-            if (ds.fast && !SPLIT) { // the base-2 pass summed e dt'^2 and e dt' with dt' = st dt
-                const double st = time_scale(params[6 * (size_t)(ds.param0 + k) + 2]);
+            if (ds.fast && !SPLIT) {
+                // the base-2 pass summed e y (y = r'^2 + dt'^2), e dt'^2, e r'^2 and e dt' with
+                // r' = sx r, dt' = st dt
+                const double *pk = params + 6 * (size_t)(ds.param0 + k);
+                const double st = time_scale(pk[2]), sx = time_scale(pk[1]);
+                s[1] = (s[1] - s[2]) / (sx * sx);
                 s[2] /= st * st;
+                s[4] /= sx * sx;
                 s[5] /= st;
             }
         } else {

@@ -33,25 +33,56 @@ namespace kern {
 
 
 // Per-vector constants of the pair loops, in one of two forms.
-//  natural (FAST false): times as given; exponents in e-units with the clamped exp_fp64:
-//      background cB0 r^2 + cB1 dt^2 (cB0 = -tauX^2/2, cB1 = -tauT^2/2; times log2 e for the
-//      FP32-bulk v_exp_f32), excitation cS0 r^2 + cS1 dt (cS0 = -sigma^2/2, cS1 = -omega).
-//  base 2 (FAST true): times arrive as dt' = st dt (time_scale); exponents in 2-units with the
-//      clamp-free exp2_fp64: background -(cB0 r^2 + dt'^2) with cB0 = tauX^2 log2(e)/2, excitation
-//      cS0 r^2 + cS1 dt' with cS0 = -sigma^2 log2(e)/2 and cS1 = -omega log2(e)/st. The row sums
-//      of e dt^2 and e dt then carry st^2 and st, which the finalize kernel divides out.
+//  natural (FAST false): coordinates and times as given; exponents in e-units with the clamped
+//      exp_fp64: background cB0 r^2 + cB1 dt^2 (cB0 = -tauX^2/2, cB1 = -tauT^2/2; times log2 e
+//      for the FP32-bulk v_exp_f32), excitation cS0 r^2 + cS1 dt (cS0 = -sigma^2/2, cS1 = -omega).
+//  base 2 (FAST true): coordinates arrive as x' = sx (x - x0) and times as t' = st (t - t0)
+//      (time_scale of tauX and tauT; x0, t0 = the row block's first event), so that the
+//      background exponent in 2-units is the plain sum of squares y = r'^2 + dt'^2 (clamp-free
+//      exp2_neg_fp64); excitation cS0 r'^2 + cS1 dt' with cS0 = -sigma^2 / tauX^2 and
+//      cS1 = -omega log2(e) / st (exp2_fp64). The background moments are summed as e y and
+//      e dt'^2, the excitation ones as e r'^2 and e dt'; the finalize kernels turn them into
+//      e r^2 = (e y - e dt'^2) / sx^2, e dt^2, e r^2 and e dt.
 struct PairConst {
     double cB0, cB1, cS0, cS1;
 };
 
-// Background term of one pair and its dt^2 (scaled by st^2 in the base-2 form).
-template <bool F32, bool FAST>
-__device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt, double &dt2) {
+// Pair geometry from the row's coordinates xi, ti and the column's q (D coordinates, then the
+// time): dt, dt^2 and, in the base-2 form, the background exponent y = r'^2 + dt'^2. r^2 is formed
+// where the natural form or the excitation term needs it (R2); else y takes the squares directly.
+template <int D, bool FAST, bool R2>
+__device__ __forceinline__ void pair_geometry(const double (&xi)[D], double ti, const double (&q)[D + 1], double &r2,
+                                              double &dt, double &dt2, double &y) {
+    dt = ti - q[D];
     dt2 = dt * dt;
+    if (FAST && !R2) {
+        y = dt2;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const double dd = xi[d] - q[d];
+            y = __builtin_fma(dd, dd, y);
+        }
+        r2 = 0.0;
+        return;
+    }
+    {
+        const double d0 = xi[0] - q[0];
+        r2 = d0 * d0;
+    }
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+        const double dd = xi[d] - q[d];
+        r2 = __builtin_fma(dd, dd, r2);
+    }
+    y = FAST ? r2 + dt2 : 0.0;
+}
+
+// Background term of one pair (geometry from pair_geometry).
+template <bool F32, bool FAST>
+__device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt2, double y) {
     if (FAST) {
-        const double y = __builtin_fma(c.cB0, r2, dt2);
         if (F32) return (double)exp2_f32((float)(-y));
-        return exp2_neg_fp64(y); // y >= 0: cB0, r^2 and dt'^2 are non-negative
+        return exp2_neg_fp64(y); // y >= 0: a sum of squares
     }
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
 }
@@ -93,36 +124,27 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
         double v[NC]; // this step's column contributions of the A rows
 #pragma unroll
         for (int a = 0; a < A; ++a) {
-            double r2;
-            {
-                const double d0 = xi[a][0] - q[0];
-                r2 = d0 * d0;
-            }
-#pragma unroll
-            for (int d = 1; d < D; ++d) {
-                const double dd = xi[a][d] - q[d];
-                r2 = __builtin_fma(dd, dd, r2);
-            }
-            const double dt = ti[a] - q[D];
+            double r2, dt, dt2, y;
+            pair_geometry<D, FAST, !FAST || SMODE != kSNone>(xi[a], ti[a], q, r2, dt, dt2, y);
+            const double m1 = FAST ? y : r2; // first background moment: e y (base 2) or e r^2
             if (DOB) {
-                double dt2;
-                const double eb = background_term<F32, FAST>(c, r2, dt, dt2);
+                const double eb = background_term<F32, FAST>(c, r2, dt2, y);
                 acc[a][0] += eb;
                 if (GRAD) {
-                    acc[a][1] = __builtin_fma(eb, r2, acc[a][1]);
+                    acc[a][1] = __builtin_fma(eb, m1, acc[a][1]);
                     acc[a][2] = __builtin_fma(eb, dt2, acc[a][2]);
                 }
                 const double ec = TAIL ? eb * vm[a] : eb;
                 if (a == 0) {
                     v[0] = ec;
                     if (GRAD) {
-                        v[1] = ec * r2;
+                        v[1] = ec * m1;
                         v[2] = ec * dt2;
                     }
                 } else {
                     v[0] += ec;
                     if (GRAD) {
-                        v[1] = __builtin_fma(ec, r2, v[1]);
+                        v[1] = __builtin_fma(ec, m1, v[1]);
                         v[2] = __builtin_fma(ec, dt2, v[2]);
                     }
                 }
@@ -161,24 +183,16 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
     constexpr int ST = TileStride<D>::value;
 #pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
-        const double *e = tile + j * ST;
-        double r2;
-        {
-            const double d0 = xi[0] - e[0];
-            r2 = d0 * d0;
-        }
+        double q[D + 1];
 #pragma unroll
-        for (int d = 1; d < D; ++d) {
-            const double dd = xi[d] - e[d];
-            r2 = __builtin_fma(dd, dd, r2);
-        }
-        const double dt = ti - e[D];
+        for (int d = 0; d <= D; ++d) q[d] = tile[j * ST + d];
+        double r2, dt, dt2, y;
+        pair_geometry<D, FAST, !FAST || PM != kPassCulledB>(xi, ti, q, r2, dt, dt2, y);
         {
-            double dt2;
-            const double eb = background_term<F32, FAST>(c, r2, dt, dt2);
+            const double eb = background_term<F32, FAST>(c, r2, dt2, y);
             acc[0] += eb;
             if (GRAD) {
-                acc[1] = __builtin_fma(eb, r2, acc[1]);
+                acc[1] = __builtin_fma(eb, FAST ? y : r2, acc[1]);
                 acc[2] = __builtin_fma(eb, dt2, acc[2]);
             }
         }
@@ -336,14 +350,18 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     const int n = ds.n, npad = ds.npad;
     const int P0 = P * R;
 
-    // base-2 form: times relative to the block's first event, scaled by st
+    // base-2 form: coordinates and times relative to the block's first event, scaled by sx and st
     const double st = FAST ? time_scale(tauT) : 1.0;
+    const double sx = FAST ? time_scale(tauX) : 1.0;
     const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
+    double x0[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x0[d] = FAST ? ex[(size_t)d * npad + min(P0, n - 1)] : 0.0;
     PairConst c;
     if (FAST) {
-        c.cB0 = tauX * tauX * (0.5 * kLog2e);
+        c.cB0 = 1.0;
         c.cB1 = 0.0;
-        c.cS0 = -sigma * sigma * (0.5 * kLog2e);
+        c.cS0 = -(sigma * sigma) / (tauX * tauX);
         c.cS1 = -omega * kLog2e / st;
     } else {
         c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
@@ -366,7 +384,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         const int r = wrow0 + a * 64 + lane;
         const int rc = min(r, n - 1);
 #pragma unroll
-        for (int d = 0; d < D; ++d) xi[a][d] = ex[(size_t)d * npad + rc];
+        for (int d = 0; d < D; ++d) xi[a][d] = FAST ? (ex[(size_t)d * npad + rc] - x0[d]) * sx : ex[(size_t)d * npad + rc];
         ti[a] = FAST ? (et[rc] - t0) * st : et[rc];
         vm[a] = r < n ? 1.0 : 0.0;
     }
@@ -388,7 +406,10 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             __syncthreads();
             if (tid < cnt) {
 #pragma unroll
-                for (int d = 0; d < D; ++d) tile[tid * ST + d] = ex[(size_t)d * npad + j0 + tid];
+                for (int d = 0; d < D; ++d) {
+                    const double v = ex[(size_t)d * npad + j0 + tid];
+                    tile[tid * ST + d] = FAST ? (v - x0[d]) * sx : v;
+                }
                 tile[tid * ST + D] = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
             }
             __syncthreads();
@@ -446,7 +467,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const double v = ex[(size_t)d * npad + j0 + tid];
+                    const double u = ex[(size_t)d * npad + j0 + tid];
+                    const double v = FAST ? (u - x0[d]) * sx : u;
                     tile[e0 + d * 128] = v;
                     tile[e0 + d * 128 + 64] = v;
                 }
