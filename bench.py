@@ -232,8 +232,10 @@ def cpu_config1(seconds):
 
 def dataset_batch(count):
     """Uneven multi-dataset batch: `count` cluster-process datasets (N log-uniform in 2e3..2e5),
-    8 jittered parameter vectors each, LL + gradient for all in one graph-replayed launch, FP64
-    and FP32-bulk; FP32 error stated per dataset against the FP64 result."""
+    8 jittered parameter vectors each, LL + gradient for all in one graph-replayed launch sequence
+    (one time-sorted pass over the datasets, the split passes for those spread over several
+    negligible-background distances: split_datasets), FP64 and FP32-bulk; FP32 error stated per
+    dataset against the FP64 result."""
     from hawkes_amd import _lib, synth
     from hawkes_amd.batch import Batch
     data = synth.uneven_datasets(count)
@@ -255,7 +257,8 @@ def dataset_batch(count):
             ll, g = b.log_likelihood()
         wall = (time.perf_counter() - t0) / reps
         res[name] = (ll, g)
-        out[name] = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing()}
+        out[name] = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing(),
+                     "split_datasets": [{"dataset": d, "N": len(data[d][1])} for d in range(count) if b.last_split(d)]}
         b.close()
     ll64, g64 = res["fp64"]
     ll32, g32 = res["fp32_bulk"]

@@ -66,6 +66,7 @@ SIGNATURES = {
     "hph_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hph_batch_get_timing": (ctypes.c_int, [_vp, _dp]),
+    "hph_batch_last_split": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }
 
 _lib = None

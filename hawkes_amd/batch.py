@@ -76,3 +76,9 @@ class Batch:
         out = ctypes.c_double()
         check(_lib.lib().hph_batch_get_timing(self._h, ctypes.byref(out)))
         return out.value
+
+    def last_split(self, dataset):
+        """True when the dataset took the split passes in the last evaluation."""
+        out = ctypes.c_int()
+        check(_lib.lib().hph_batch_last_split(self._h, int(dataset), ctypes.byref(out)))
+        return bool(out.value)

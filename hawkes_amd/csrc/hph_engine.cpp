@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <numeric>
@@ -149,6 +150,108 @@ static bool fast_ok(const Extents &e, const double *params, int K) {
         if (!(sx * diag <= 300.0 && std::fabs(p[0]) * 0.84932180028801904272 * diag <= 300.0)) return false;
     }
     return true;
+}
+
+// Events per window of the culled order: N / 8 rounded down to a power of two, in
+// [1024, 32768] (longer windows make groups smaller in space but longer in time).
+static int cull_window(int n) {
+    int w = 1024;
+    while (w < 32768 && 2 * w <= n / 8) w *= 2;
+    return w;
+}
+
+// Split passes (chosen automatically for likelihood passes): from kSplitMinEvents events on, when
+// the locations' diagonal is at least kSplitMinRatio times the negligible-background distance
+// (counted on cluster processes with windows of 32,768: ratio 3.6 forms 0.54x the background
+// pairs of the time-sorted pass, ratio 7.9 0.23x, ratio 2.6 about as many).
+constexpr int kSplitMinEvents = 65536;
+constexpr double kSplitMinRatio = 3.0;
+
+// Whether the split passes are expected to pay for a dataset of n events in D dimensions and its
+// K vectors: the bounding box's diagonal spans at least kSplitMinRatio times the distance at which
+// every background term of every vector is negligible (tauX r / sqrt(2 skip point)), so that
+// groups far apart in space are skipped.
+static bool split_pays(const Extents &e, const double *params, int K, int n, int D, bool fp32) {
+    if (D > 3 || n < kSplitMinEvents) return false;
+    const double arg = kern::background_skip_arg(n, fp32);
+    for (int k = 0; k < K; ++k) {
+        const double tx = std::fabs(params[6 * (size_t)k + 1]);
+        if (!(tx * std::sqrt(e.r2max) >= kSplitMinRatio * std::sqrt(2.0 * arg))) return false;
+    }
+    return true;
+}
+
+// The culled order of the split passes, on the host: the time-sorted events cut into windows of
+// cull_window(n) events, each window ordered along a Morton curve of its locations (21 bits per
+// axis of the bounding box), so that every group of kern::kGroup consecutive events spans a small
+// box in space and a window in time. x: D planes of npad, t: npad, perm: culled position ->
+// time-sorted position (padding: 0), gstat: the groups' boxes (2D + 2 doubles each, for every
+// group of the 512-event row blocks; empty groups get an inverted box that every test skips).
+// locs: n x D in the caller's order; sorted: time-sorted position -> caller's index; times: in
+// the caller's order.
+struct CulledHost {
+    std::vector<double> x, t, gstat;
+    std::vector<int> perm;
+};
+static CulledHost make_culled(const std::vector<double> &locs, const std::vector<int> &sorted,
+                              const std::vector<double> &times, int n, int D, int npad) {
+    std::vector<double> lo(D), hi(D);
+    for (int d = 0; d < D; ++d) {
+        lo[d] = hi[d] = locs[d];
+        for (int i = 1; i < n; ++i) {
+            lo[d] = std::min(lo[d], locs[(size_t)i * D + d]);
+            hi[d] = std::max(hi[d], locs[(size_t)i * D + d]);
+        }
+    }
+    std::vector<unsigned long long> code(n);
+    for (int s = 0; s < n; ++s) {
+        const int u = sorted[s];
+        unsigned long long c = 0;
+        for (int d = 0; d < D; ++d) {
+            const double w = hi[d] > lo[d] ? (locs[(size_t)u * D + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
+            const unsigned long long q =
+                (unsigned long long)std::min(2097151.0, std::max(0.0, std::floor(w * 2097152.0)));
+            for (int b = 0; b < 21; ++b) c |= ((q >> b) & 1ULL) << (b * D + d);
+        }
+        code[s] = c;
+    }
+    std::vector<int> cperm(n); // culled position -> sorted position
+    std::iota(cperm.begin(), cperm.end(), 0);
+    const int W = cull_window(n);
+    for (int w0 = 0; w0 < n; w0 += W) {
+        const int w1 = std::min(n, w0 + W);
+        std::stable_sort(cperm.begin() + w0, cperm.begin() + w1, [&](int a, int b) { return code[a] < code[b]; });
+    }
+    CulledHost c;
+    c.x.resize((size_t)D * npad);
+    c.t.resize(npad);
+    c.perm.resize(npad);
+    for (int r = 0; r < npad; ++r) {
+        const int sp = cperm[r < n ? r : n - 1];
+        const int u = sorted[sp];
+        for (int d = 0; d < D; ++d) c.x[(size_t)d * npad + r] = locs[(size_t)u * D + d];
+        c.t[r] = times[u];
+        c.perm[r] = r < n ? sp : 0;
+    }
+    const int groups = (n + 511) / 512 * (512 / kern::kGroup);
+    const int GS = 2 * D + 2;
+    c.gstat.resize((size_t)groups * GS);
+    for (int g = 0; g < groups; ++g) {
+        double *e = c.gstat.data() + (size_t)g * GS;
+        for (int d = 0; d <= D; ++d) {
+            e[d] = 1e300;
+            e[D + 1 + d] = -1e300;
+        }
+        for (int r = g * kern::kGroup; r < std::min(n, (g + 1) * kern::kGroup); ++r) {
+            for (int d = 0; d < D; ++d) {
+                e[d] = std::min(e[d], c.x[(size_t)d * npad + r]);
+                e[D + 1 + d] = std::max(e[D + 1 + d], c.x[(size_t)d * npad + r]);
+            }
+            e[D] = std::min(e[D], c.t[r]);
+            e[2 * D + 1] = std::max(e[2 * D + 1], c.t[r]);
+        }
+    }
+    return c;
 }
 
 // Drives the symmetric pass over a set of device-resident datasets: plans work items, keeps the
@@ -812,98 +915,25 @@ class HipHawkes : public AbstractHawkes {
         cache_valid_ = false;
     }
 
-    // The culled order of the split passes: the time-sorted events cut into windows of
-    // cull_window() events, each window ordered along a Morton curve of its locations (21 bits per axis of
-    // the bounding box), so that every group of kern::kGroup consecutive events spans a small box
-    // in space and a window in time. Uploads that order's locations and times, its map to sorted
-    // positions and the groups' boxes (empty groups get an inverted box that every test skips).
+    // The culled order of the split passes (make_culled), uploaded with its group boxes.
     void build_culled() {
         const int N = locationCount, D = embeddingDimension;
-        std::vector<double> lo(D), hi(D);
-        for (int d = 0; d < D; ++d) {
-            lo[d] = hi[d] = locs_[d];
-            for (int i = 1; i < N; ++i) {
-                lo[d] = std::min(lo[d], locs_[(size_t)i * D + d]);
-                hi[d] = std::max(hi[d], locs_[(size_t)i * D + d]);
-            }
-        }
-        std::vector<unsigned long long> code(N);
-        for (int s = 0; s < N; ++s) {
-            const int u = perm_[s];
-            unsigned long long c = 0;
-            for (int d = 0; d < D; ++d) {
-                const double w = hi[d] > lo[d] ? (locs_[(size_t)u * D + d] - lo[d]) / (hi[d] - lo[d]) : 0.0;
-                const unsigned long long q =
-                    (unsigned long long)std::min(2097151.0, std::max(0.0, std::floor(w * 2097152.0)));
-                for (int b = 0; b < 21; ++b) c |= ((q >> b) & 1ULL) << (b * D + d);
-            }
-            code[s] = c;
-        }
-        std::vector<int> cperm(N); // culled position -> sorted position
-        std::iota(cperm.begin(), cperm.end(), 0);
-        const int W = cull_window();
-        for (int w0 = 0; w0 < N; w0 += W) {
-            const int w1 = std::min(N, w0 + W);
-            std::stable_sort(cperm.begin() + w0, cperm.begin() + w1,
-                             [&](int a, int b) { return code[a] < code[b]; });
-        }
-        const size_t bytes = sizeof(double) * (size_t)(D + 1) * npad_ + sizeof(int) * (size_t)npad_;
-        h_stage_.ensure(bytes);
-        double *hx = h_stage_.as<double>();
-        double *ht = hx + (size_t)D * npad_;
-        int *hp = reinterpret_cast<int *>(ht + npad_);
-        for (int r = 0; r < npad_; ++r) {
-            const int sp = cperm[r < N ? r : N - 1];
-            const int u = perm_[sp];
-            for (int d = 0; d < D; ++d) hx[(size_t)d * npad_ + r] = locs_[(size_t)u * D + d];
-            ht[r] = times_[u];
-            hp[r] = r < N ? sp : 0;
-        }
-        // boxes for every row of the largest row blocks (512 events), past npad
-        const int groups = (N + 511) / 512 * (512 / kern::kGroup);
-        const int GS = 2 * D + 2;
-        std::vector<double> gs((size_t)groups * GS);
-        for (int g = 0; g < groups; ++g) {
-            double *e = gs.data() + (size_t)g * GS;
-            for (int d = 0; d <= D; ++d) {
-                e[d] = 1e300;
-                e[D + 1 + d] = -1e300;
-            }
-            for (int r = g * kern::kGroup; r < std::min(N, (g + 1) * kern::kGroup); ++r) {
-                for (int d = 0; d < D; ++d) {
-                    e[d] = std::min(e[d], hx[(size_t)d * npad_ + r]);
-                    e[D + 1 + d] = std::max(e[D + 1 + d], hx[(size_t)d * npad_ + r]);
-                }
-                e[D] = std::min(e[D], ht[r]);
-                e[2 * D + 1] = std::max(e[2 * D + 1], ht[r]);
-            }
-        }
+        const CulledHost c = make_culled(locs_, perm_, times_, N, D, npad_);
         d_xc_.ensure(sizeof(double) * (size_t)D * npad_);
         d_tc_.ensure(sizeof(double) * (size_t)npad_);
         d_cperm_.ensure(sizeof(int) * (size_t)npad_);
-        d_gstat_.ensure(sizeof(double) * gs.size());
-        check(hipMemcpyAsync(d_xc_.ptr, hx, sizeof(double) * (size_t)D * npad_, hipMemcpyHostToDevice, stream_),
-              "upload culled x");
-        check(hipMemcpyAsync(d_tc_.ptr, ht, sizeof(double) * npad_, hipMemcpyHostToDevice, stream_), "upload culled t");
-        check(hipMemcpyAsync(d_cperm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_),
+        d_gstat_.ensure(sizeof(double) * c.gstat.size());
+        check(hipMemcpy(d_xc_.ptr, c.x.data(), sizeof(double) * c.x.size(), hipMemcpyHostToDevice), "upload culled x");
+        check(hipMemcpy(d_tc_.ptr, c.t.data(), sizeof(double) * c.t.size(), hipMemcpyHostToDevice), "upload culled t");
+        check(hipMemcpy(d_cperm_.ptr, c.perm.data(), sizeof(int) * c.perm.size(), hipMemcpyHostToDevice),
               "upload culled order");
-        check(hipStreamSynchronize(stream_), "culled order sync");
-        check(hipMemcpy(d_gstat_.ptr, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice),
+        check(hipMemcpy(d_gstat_.ptr, c.gstat.data(), sizeof(double) * c.gstat.size(), hipMemcpyHostToDevice),
               "upload group boxes");
         culled_valid_ = true;
     }
 
-    // Whether the split passes are expected to pay: the bounding box's diagonal spans at least
-    // kSplitMinRatio times the distance at which every background term of every vector is
-    // negligible (tauX r / sqrt(2 skip point)), so that groups far apart in space are skipped.
     bool split_pays(const double *params, int K) const {
-        if (embeddingDimension > 3 || locationCount < kSplitMinEvents) return false;
-        const double arg = kern::background_skip_arg(locationCount, fp32_);
-        for (int k = 0; k < K; ++k) {
-            const double tx = std::fabs(params[6 * (size_t)k + 1]);
-            if (!(tx * std::sqrt(extents_.r2max) >= kSplitMinRatio * std::sqrt(2.0 * arg))) return false;
-        }
-        return true;
+        return ::hph::split_pays(extents_, params, K, locationCount, embeddingDimension, fp32_);
     }
 
     kern::Events events() const {
@@ -1192,20 +1222,6 @@ class HipHawkes : public AbstractHawkes {
     // below 24,576 (229 us at 1.6e4), symmetric with 2 rows per lane above (551 us at 3.3e4).
     static constexpr int kNarrowMaxEvents = 12288;
     static constexpr int kSymA1MaxEvents = 24576;
-    // Split passes (AUTO): chosen from kSplitMinEvents events on when the locations' diagonal is
-    // at least kSplitMinRatio times the negligible-background distance (counted on cluster
-    // processes with windows of 32,768: ratio 3.6 forms 0.54x the background pairs of the
-    // time-sorted pass, ratio 7.9 0.23x, ratio 2.6 about as many).
-    static constexpr int kSplitMinEvents = 65536;
-    static constexpr double kSplitMinRatio = 3.0;
-
-    // Events per window of the culled order: N / 8 rounded down to a power of two, in
-    // [1024, 32768] (longer windows make groups smaller in space but longer in time).
-    int cull_window() const {
-        int w = 1024;
-        while (w < 32768 && 2 * w <= locationCount / 8) w *= 2;
-        return w;
-    }
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
     Extents extents_;
@@ -1281,6 +1297,8 @@ class HipHawkesBatch {
         }
         locs_.resize(nd);
         times_.resize(nd);
+        sorted_.resize(nd);
+        split_.resize(nd);
         ext_.resize(nd);
         have_.assign(nd, false);
         d_x_.ensure(sizeof(double) * (size_t)D * off_[nd]);
@@ -1294,6 +1312,10 @@ class HipHawkesBatch {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graph();
         for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_params_, &d_result_}) b->release();
+        for (auto &sd : split_)
+            if (sd)
+                for (DevBuf *b : {&sd->x, &sd->t, &sd->perm, &sd->gstat, &sd->bsum, &sd->sband}) b->release();
+        split_.clear();
         h_params_.release();
         h_result_.release();
         for (auto &e : ev_)
@@ -1327,6 +1349,8 @@ class HipHawkesBatch {
             hp[s] = u;
         }
         ext_[d] = extents_of(locs_[d], ht.data(), (size_t)n, D_);
+        sorted_[d] = perm;
+        if (split_[d]) split_[d]->valid = false;
         // x planes of dataset d live at d_x_ + D * off_[d]: plane q at + q * npad_[d]
         check(hipMemcpy(d_x_.as<double>() + (size_t)D_ * off_[d], hx.data(), sizeof(double) * hx.size(),
                         hipMemcpyHostToDevice),
@@ -1354,8 +1378,7 @@ class HipHawkesBatch {
         if (K != K_) drop_graph();
         K_ = K;
         total_ = (int)total;
-        h_params_.ensure(sizeof(double) * count);
-        std::memcpy(h_params_.ptr, params, sizeof(double) * count);
+        params_.assign(params, params + count);
         have_params_ = true;
     }
 
@@ -1385,15 +1408,37 @@ class HipHawkesBatch {
         d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
+        h_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         const std::vector<char> fast = fast_flags();
-        if (graph_exec_ && fast != graph_fast_) drop_graph(); // the graph holds one variant and table
+        const std::vector<char> split = split_flags();
+        // the graph holds one variant, table and split layout
+        if (graph_exec_ && (fast != graph_fast_ || split != graph_split_)) drop_graph();
+        // vector slots: the datasets of the time-sorted pass first (dataset order), then each
+        // dataset of the split passes; parameters are staged and results read in that order
+        const int nd = (int)counts_.size();
+        slot_.assign(nd, 0);
+        std::vector<int> first(nd, 0);
+        for (int d = 1; d < nd; ++d) first[d] = first[d - 1] + K_[d - 1];
+        int next = 0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int d = 0; d < nd; ++d)
+                if ((split[d] != 0) == (pass == 1)) {
+                    slot_[d] = next;
+                    next += K_[d];
+                }
+        for (int d = 0; d < nd; ++d)
+            std::memcpy(h_params_.as<double>() + 6 * (size_t)slot_[d], params_.data() + 6 * (size_t)first[d],
+                        sizeof(double) * 6 * (size_t)K_[d]);
+        for (int d = 0; d < nd; ++d)
+            if (split[d]) ensure_split(d); // host work and uploads, outside any capture
         if (graph_mode_) {
             if (!graph_exec_) {
-                enqueue(false, fast); // plans, allocates and uploads the tables outside the capture
+                enqueue(false, fast, split); // plans, allocates and uploads the tables outside the capture
                 check(hipStreamSynchronize(stream_), "stream sync");
                 check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                enqueue(false, fast);
+                enqueue(false, fast, split);
                 graph_fast_ = fast;
+                graph_split_ = split;
                 hipGraph_t g = nullptr;
                 check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
                 graph_ = g;
@@ -1403,18 +1448,28 @@ class HipHawkesBatch {
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
             check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         } else {
-            enqueue(true, fast);
+            enqueue(true, fast, split);
         }
         check(hipStreamSynchronize(stream_), "stream sync");
         float a = 0.f;
         check(hipEventElapsedTime(&a, ev_[0], ev_[2]), "hipEventElapsedTime");
         last_ms_ = a;
+        last_split_ = split;
         const double *r = h_result_.as<double>();
-        for (int k = 0; k < total_; ++k) {
-            loglik[k] = r[(size_t)k * kern::kResultWidth];
-            if (want_grad)
-                for (int q = 0; q < 6; ++q) grad[(size_t)k * 6 + q] = r[(size_t)k * kern::kResultWidth + 1 + q];
-        }
+        for (int d = 0; d < nd; ++d)
+            for (int k = 0; k < K_[d]; ++k) {
+                const double *rk = r + (size_t)(slot_[d] + k) * kern::kResultWidth;
+                const size_t o = (size_t)first[d] + k;
+                loglik[o] = rk[0];
+                if (want_grad)
+                    for (int q = 0; q < 6; ++q) grad[o * 6 + q] = rk[1 + q];
+            }
+    }
+
+    // Whether dataset d took the split passes in the last evaluation.
+    bool lastSplit(int d) const {
+        if (d < 0 || d >= (int)counts_.size()) throw std::invalid_argument("dataset index out of range");
+        return d < (int)last_split_.size() && last_split_[d] != 0;
     }
 
     double lastMs() const { return last_ms_; }
@@ -1423,28 +1478,92 @@ class HipHawkesBatch {
     // Per dataset: whether all its vectors of the current parameters allow the base-2 form.
     std::vector<char> fast_flags() const {
         std::vector<char> fast(counts_.size());
-        const double *hp = h_params_.as<double>();
+        const double *hp = params_.data();
         for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
             fast[d] = fast_ok(ext_[d], hp + 6 * k0, K_[d]) ? 1 : 0;
         return fast;
     }
 
-    // Parameter upload, pass, finalize, reduction and result download on the batch's stream.
-    void enqueue(bool events, const std::vector<char> &fast) {
+    // Per dataset: whether it takes the split passes (likelihood passes with skipping, data spread
+    // over several negligible-background distances for every one of its vectors; as for a single
+    // engine's built-in choice).
+    std::vector<char> split_flags() const {
+        std::vector<char> split(counts_.size(), 0);
+        if (flags_ & ALL_PAIRS) return split;
+        const double *hp = params_.data();
+        for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
+            split[d] = split_pays(ext_[d], hp + 6 * k0, K_[d], counts_[d], D_, fp32_) ? 1 : 0;
+        return split;
+    }
+
+    // The culled order of dataset d (make_culled) on the device, its runner and its per-event
+    // sum buffers for the current vectors.
+    void ensure_split(int d) {
+        if (!split_[d]) split_[d] = std::make_unique<SplitData>();
+        SplitData &sd = *split_[d];
+        const int np = npad_[d];
+        if (!sd.valid) {
+            const CulledHost c = make_culled(locs_[d], sorted_[d], times_[d], counts_[d], D_, np);
+            sd.x.ensure(sizeof(double) * c.x.size());
+            sd.t.ensure(sizeof(double) * c.t.size());
+            sd.perm.ensure(sizeof(int) * c.perm.size());
+            sd.gstat.ensure(sizeof(double) * c.gstat.size());
+            check(hipMemcpy(sd.x.ptr, c.x.data(), sizeof(double) * c.x.size(), hipMemcpyHostToDevice), "upload culled x");
+            check(hipMemcpy(sd.t.ptr, c.t.data(), sizeof(double) * c.t.size(), hipMemcpyHostToDevice), "upload culled t");
+            check(hipMemcpy(sd.perm.ptr, c.perm.data(), sizeof(int) * c.perm.size(), hipMemcpyHostToDevice),
+                  "upload culled order");
+            check(hipMemcpy(sd.gstat.ptr, c.gstat.data(), sizeof(double) * c.gstat.size(), hipMemcpyHostToDevice),
+                  "upload group boxes");
+            sd.valid = true;
+            drop_graph();
+        }
+        sd.bsum.ensure(sizeof(double) * (size_t)K_[d] * 3 * np);
+        sd.sband.ensure(sizeof(double) * (size_t)K_[d] * 3 * np);
+    }
+
+    // Parameter upload, passes, finalize, reduction and result download on the batch's stream:
+    // one time-sorted symmetric pass over the datasets not split, then each split dataset's
+    // passes (background on its culled order, per-event sums, excitation band, finalize),
+    // parameters and results in the slot order of evaluate.
+    void enqueue(bool events, const std::vector<char> &fast, const std::vector<char> &split) {
         check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)total_, hipMemcpyHostToDevice,
                              stream_),
               "upload params");
-        std::vector<SymRunner::DataRef> refs(counts_.size());
-        for (size_t d = 0; d < counts_.size(); ++d)
-            refs[d] = SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
-                                         d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
         const int skip_zero = (flags_ & ALL_PAIRS) ? 0 : 1;
-        runner_.enqueue(refs, K_, D_, d_params_.as<double>(), grad_, fp32_, A_, target_, skip_zero, 1, fast, nullptr, nullptr,
-                        d_result_.as<double>(), stream_, events ? ev_[0] : nullptr, events ? ev_[1] : nullptr,
-                        events ? ev_[2] : nullptr);
+        double *params = d_params_.as<double>(), *result = d_result_.as<double>();
+        if (events) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        std::vector<SymRunner::DataRef> refs;
+        std::vector<int> K;
+        std::vector<char> f;
+        for (size_t d = 0; d < counts_.size(); ++d)
+            if (!split[d]) {
+                refs.push_back(ref(d));
+                K.push_back(K_[d]);
+                f.push_back(fast[d]);
+            }
+        if (!refs.empty())
+            runner_.enqueue(refs, K, D_, params, grad_, fp32_, A_, target_, skip_zero, 1, f, nullptr, nullptr, result,
+                            stream_);
+        for (size_t d = 0; d < counts_.size(); ++d)
+            if (split[d]) {
+                SplitData &sd = *split_[d];
+                const SymRunner::DataRef culled{sd.x.as<double>(), sd.t.as<double>(), sd.perm.as<int>(), counts_[d],
+                                                npad_[d], times_[d].back()};
+                sd.runner.enqueue_split(ref(d), culled, sd.gstat.as<double>(), K_[d], D_, params + 6 * (size_t)slot_[d],
+                                        grad_, fp32_, A_, target_, skip_zero, 1, fast[d] != 0, sd.bsum.as<double>(),
+                                        sd.sband.as<double>(), nullptr, nullptr,
+                                        result + kern::kResultWidth * (size_t)slot_[d], stream_, nullptr, nullptr,
+                                        nullptr, nullptr);
+            }
+        if (events) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,
                              hipMemcpyDeviceToHost, stream_),
               "download result");
+    }
+
+    SymRunner::DataRef ref(size_t d) const {
+        return SymRunner::DataRef{d_x_.as<double>() + (size_t)D_ * off_[d], d_t_.as<double>() + off_[d],
+                                  d_perm_.as<int>() + off_[d], counts_[d], npad_[d], times_[d].back()};
     }
 
     void drop_graph() {
@@ -1463,12 +1582,22 @@ class HipHawkesBatch {
     std::vector<long long> off_; // start of dataset d in the concatenated device arrays
     std::vector<int> npad_;
     std::vector<std::vector<double>> locs_, times_;
+    std::vector<std::vector<int>> sorted_; // per dataset: time-sorted position -> caller's index
     std::vector<Extents> ext_;
     std::vector<bool> have_;
     std::vector<int> K_;
+    std::vector<double> params_; // the caller's vectors, dataset order
+    std::vector<int> slot_;      // result and parameter slot of dataset d's first vector
     int total_ = 0;
     bool have_params_ = false, grad_ = true, graph_mode_ = false;
-    std::vector<char> graph_fast_;
+    std::vector<char> graph_fast_, graph_split_, last_split_;
+    // a dataset of the split passes: its culled order, runner and per-event sums
+    struct SplitData {
+        DevBuf x, t, perm, gstat, bsum, sband;
+        bool valid = false;
+        SymRunner runner;
+    };
+    std::vector<std::unique_ptr<SplitData>> split_;
     int A_ = 2, target_ = 8192;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
@@ -1742,6 +1871,15 @@ int hph_batch_get_timing(hph_batch *b, double *out) {
         return HPH_ERR_INVALID_ARGUMENT;
     }
     return guarded([&] { *out = b->impl->lastMs(); });
+}
+
+int hph_batch_last_split(hph_batch *b, int dataset, int *out) {
+    HPH_REQUIRE_BATCH(b);
+    if (out == nullptr) {
+        g_last_error = "output pointer is null";
+        return HPH_ERR_INVALID_ARGUMENT;
+    }
+    return guarded([&] { *out = b->impl->lastSplit(dataset) ? 1 : 0; });
 }
 
 } // extern "C"

@@ -183,6 +183,10 @@ int hph_batch_set_graph(hph_batch *batch, int enabled);
 int hph_batch_set_config(hph_batch *batch, int rows_per_lane, int target_items);
 /* Device time (ms) of the last evaluation's launch sequence. */
 int hph_batch_get_timing(hph_batch *batch, double *out);
+/* *out = 1 when dataset d took the split passes in the last evaluation (data spread over several
+ * negligible-background distances for each of its vectors, at least 65,536 events, as for an
+ * engine's built-in choice; the other datasets share one time-sorted pass), else 0. */
+int hph_batch_last_split(hph_batch *batch, int dataset, int *out);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,46 @@ def test_batch_matches_single_engines():
         e.close()
 
 
+def test_split_datasets_match_single_engines():
+    """Datasets of at least 65,536 events spread over 60 km take the split passes inside the batch
+    (as a single engine's built-in choice does); the others share the time-sorted pass. Each
+    vector's result equals a single engine's (same passes, other work decomposition: rounding)
+    in eager and graph mode, and graph replays equal the eager evaluation bitwise."""
+    data = [synth.cluster_process(47000, side=60.0, seed=51), synth.cluster_process(1400, side=4.0, seed=52),
+            synth.cluster_process(48000, side=60.0, seed=53), synth.cluster_process(3000, side=6.0, seed=54)]
+    params = [synth.jitter_params(2, sd=0.05, seed=960 + d) for d in range(len(data))]
+    b = make_batch(data, params)
+    ll, g = b.log_likelihood()
+    assert [b.last_split(d) for d in range(len(data))] == [True, False, True, False]
+    lls, gs = b.split(ll), b.split(g)
+    for d, (locs, times) in enumerate(data):
+        e = Engine(2, len(times))
+        e.update_locations(locs)
+        e.set_times(times)
+        for k in range(2):
+            e.set_parameters(params[d][k])
+            l1, g1 = e.log_likelihood_and_gradient()
+            print(d, k, lls[d][k], l1, flush=True)
+            assert rel_err(lls[d][k], l1) <= 1e-13, (d, k)
+            assert np.all(np.abs(gs[d][k] - g1) <= 1e-12 * np.maximum(np.abs(g1), 1e-3 * np.abs(g1).max())), (d, k)
+        e.close()
+    b.set_graph(True)
+    for rep in range(3):
+        got = b.log_likelihood()
+        assert np.array_equal(got[0], ll) and np.array_equal(got[1], g), rep
+    # vectors that put a large dataset back on the time-sorted pass (a wide background kernel)
+    wide = [p.copy() for p in params]
+    wide[0][:, 1] = 0.05
+    b.set_parameters(wide)
+    got = b.log_likelihood()
+    assert [b.last_split(d) for d in range(len(data))] == [False, False, True, False]
+    b.set_graph(False)
+    eager = b.log_likelihood()
+    assert np.array_equal(got[0], eager[0]) and np.array_equal(got[1], eager[1])
+    ref = oracle.total(data[1][0], data[1][1], wide[1][0])
+    assert rel_err(got[0][2], ref["ll"]) <= 1e-12
+
+
 def test_graph_replays_match_eager():
     data = uneven_datasets(4, 500, 8000, seed=31)
     params = [synth.jitter_params(2, seed=900 + d) for d in range(len(data))]
