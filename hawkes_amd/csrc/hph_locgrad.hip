@@ -179,7 +179,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void locgrad_finalize_kernel(Events 
 // gradient (plan_sym(..., grad = true): 6 row and 3 column accumulators, of which the first D
 // are used). A row block is paired with earlier columns only; for a row i and an earlier column
 // j, f = fB eB_ij (w_i + w_j) + fS eS_ij w_i (eS_ji = 0), and d ll / d x_i gets -f (x_i - x_j),
-// d ll / d x_j gets +f (x_i - x_j) (rotating column accumulators, as in the likelihood pass).
+// d ll / d x_j gets +f (x_i - x_j) (column accumulators that move one lane per step, ds_bpermute;
+// this pass is bound by its natural-form exp, and the likelihood pass's LDS column slots measured
+// no faster here).
 // The diagonal block is paired in both orders, each order adding to its row only.
 template <int D, int A, bool DOB, int SMODE, bool TAIL>
 __device__ __forceinline__ void loc_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
