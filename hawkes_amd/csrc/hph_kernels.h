@@ -154,6 +154,8 @@ struct SymData {
     long long row0, row_kstride; // row partials: doubles offset of k = 0 and stride per k
     long long col0, col_kstride; // column partials: the same
     long long flag0, flag_kstride; // column-tile flags (bytes): the same
+    long long rflag0, rflag_kstride; // row-slot flags (bytes, in the same buffer): per (k, block P,
+                                     // chunk c) at rflag0 + k rflag_kstride + P max_chunks + c
     long long blk0;         // index of its (k = 0, block 0) block partial
     int res0;               // index of its (k = 0) result
 };
@@ -179,7 +181,9 @@ struct SymLaunch {
     unsigned char *colflag; // per (earlier column tile t, block P), at flag0 + k flag_kstride +
                             // t nb + P: 1 when the pass wrote the tile's column partials, 0 when it
                             // skipped the tile (its partials are then zero and left unwritten);
-                            // readers skip unflagged tiles
+                            // readers skip unflagged tiles. After them, the row-slot flags
+                            // (SymData::rflag0): 1 when a chunk item wrote its row partials, 0
+                            // when the pass skipped every tile of the item (partials unwritten)
 };
 
 // Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted

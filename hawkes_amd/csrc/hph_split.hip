@@ -35,8 +35,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int QS = GRAD ? 3 : 1;
-    __shared__ int qlist[kQChunk];
-    __shared__ int qcount;
+    __shared__ int qlist[kQChunk], rlist[kQChunk];
+    __shared__ int qcount, rcount;
     const int4 it = L.fin_item[blockIdx.x];
     if (it.x != 0) return; // uniform: the whole workgroup
     const SymData &ds = L.data[0];
@@ -55,24 +55,15 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     for (int q = 0; q < QS; ++q) a[q] = 0.0;
 #pragma unroll
     for (int c = 0; c < NC; ++c) cs[c] = 0.0;
-    if (live) {
-        const int P = r / R;
+    {
+        // row side: the chunk slots the pass wrote (the workgroup's 256 positions share one block)
+        const int P = it.z * kRowsPerBlock / R;
         const int nch = (P * R + ds.L - 1) / ds.L;
-        const size_t rpad = (size_t)ds.nb * R;
-        const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
-#pragma unroll 4
-        for (int c = 0; c < nch; ++c) {
-            const double *rp = rbase + (size_t)c * NR * rpad;
-#pragma unroll
-            for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
-        }
-        {
-            const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
-#pragma unroll
-            for (int q = 0; q < QS; ++q) a[q] += rp[(size_t)q * rpad + r];
-        }
-        add_listed_columns<NC>(cbase, R, r, qlist, cnt0, cs);
+        const unsigned char *rf = L.colflag + ds.rflag0 + (size_t)k * ds.rflag_kstride + (size_t)P * ds.max_chunks;
+        add_row_slots<NR, QS>(rowpart + ds.row0 + (size_t)k * ds.row_kstride, rf, nch, ds.max_chunks,
+                              (size_t)ds.nb * R, r, live, a, rlist, &rcount);
     }
+    if (live) add_listed_columns<NC>(cbase, R, r, qlist, cnt0, cs);
     add_remaining_columns<NC>(cbase, ftile, R, Pw, ds.nb, r, live, cs, qlist, &qcount);
     if (!live) return;
     const int i = ds.perm[r];

@@ -29,8 +29,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int QS = GRAD ? 3 : 1;
     __shared__ double2 sh[kResultWidth * kRowsPerBlock];
-    __shared__ int qlist[kQChunk];
-    __shared__ int qcount;
+    __shared__ int qlist[kQChunk], rlist[kQChunk];
+    __shared__ int qcount, rcount;
     const int4 it = L.fin_item[blockIdx.x];
     const SymData &ds = L.data[it.x];
     const int k = it.y, fb = it.z;
@@ -51,25 +51,16 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
     double a6[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) a6[q] = 0.0;
+    if (!SPLIT) {
+        // row side: the chunk slots the pass wrote (the block's 256 rows share one row block P)
+        const int P = fb * kRowsPerBlock / R;
+        const int nch = (P * R + ds.L - 1) / ds.L;
+        const unsigned char *rf = L.colflag + ds.rflag0 + (size_t)k * ds.rflag_kstride + (size_t)P * ds.max_chunks;
+        add_row_slots<NR, NR>(rowpart + ds.row0 + (size_t)k * ds.row_kstride, rf, nch, ds.max_chunks,
+                              (size_t)ds.nb * R, i, i < n, a6, rlist, &rcount);
+    }
     if (i < n) {
         if (!SPLIT) {
-            const int P = i / R;
-            const int nch = (P * R + ds.L - 1) / ds.L;
-            const size_t rpad = (size_t)ds.nb * R;
-            const double *rbase = rowpart + ds.row0 + (size_t)k * ds.row_kstride;
-            // unrolled so that several slots' loads are in flight (one grid of 256-row blocks is
-            // too small to cover HBM latency with one load per thread)
-#pragma unroll 4
-            for (int c = 0; c < nch; ++c) {
-                const double *rp = rbase + (size_t)c * NR * rpad;
-#pragma unroll
-                for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
-            }
-            {
-                const double *rp = rbase + (size_t)ds.max_chunks * NR * rpad;
-#pragma unroll
-                for (int q = 0; q < NR; ++q) a6[q] += rp[(size_t)q * rpad + i];
-            }
             add_listed_columns<NC>(cbase, R, i, qlist, cnt0, cs);
         } else {
             // per-event sums, already complete
@@ -224,6 +215,7 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
         row0 += (size_t)K[d] * s.row_kstride;
         col0 += (size_t)K[d] * s.col_kstride;
         flag0 += (size_t)K[d] * s.flag_kstride;
+        s.rflag_kstride = (long long)((size_t)nb[d] * s.max_chunks);
         blk0 += (size_t)K[d] * s.fin_blocks;
         res0 += K[d];
         param0 += K[d];
@@ -248,6 +240,11 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     for (const Cost &c : items) plan.items.push_back(c.item);
     plan.rowpart_doubles = row0;
     plan.colpart_doubles = col0;
+    // row-slot flags after every dataset's column flags
+    for (SymData &s : plan.data) {
+        s.rflag0 = (long long)flag0;
+        flag0 += (size_t)s.K * s.rflag_kstride;
+    }
     plan.colflag_bytes = flag0;
     plan.block_partials = blk0;
     plan.results = res0;

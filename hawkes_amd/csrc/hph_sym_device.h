@@ -292,6 +292,35 @@ __device__ __forceinline__ void add_remaining_columns(const double *__restrict__
     }
 }
 
+// Row side of the symmetric pass, read back for row r of block P: adds the first NQ moments of
+// the row partials of the chunk slots c < nch whose flag rf[c] is set (listed kQChunk at a time,
+// ascending) and of the diagonal slot max_chunks into a. NR moments per slot, slots rpad rows
+// apart. Every thread of the block calls it (rows past n: live false, barriers only); list and
+// count are LDS scratch of the caller's own.
+template <int NR, int NQ>
+__device__ __forceinline__ void add_row_slots(const double *__restrict__ rbase, const unsigned char *__restrict__ rf,
+                                              int nch, int max_chunks, size_t rpad, int r, bool live,
+                                              double (&a)[NQ], int *list, int *count) {
+    for (int c0 = 0; c0 < nch; c0 += kQChunk) {
+        if (c0 > 0) __syncthreads(); // every thread has read the previous list
+        const int cnt = flagged_blocks(rf, c0, min(nch, c0 + kQChunk), list, count);
+        if (live) {
+            // unrolled so that several slots' loads are in flight
+#pragma unroll 4
+            for (int u = 0; u < cnt; ++u) {
+                const double *rp = rbase + (size_t)list[u] * NR * rpad;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a[q] += rp[(size_t)q * rpad + r];
+            }
+        }
+    }
+    if (live) {
+        const double *rp = rbase + (size_t)max_chunks * NR * rpad;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) a[q] += rp[(size_t)q * rpad + r];
+    }
+}
+
 // Bounding box of `count` consecutive groups of the culled order, starting at group g:
 // lo/hi[0..D-1] coordinates, lo/hi[D] time (gstat layout of SymData).
 template <int D>
@@ -436,6 +465,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             group_box<D>(ds.gstat, P0 / kGroup, R / kGroup, blo, bhi);
             group_box<D>(ds.gstat, wrow0 / kGroup, A, wlo, whi);
         }
+        bool wrote = false; // a tile of the item passed the workgroup-level skip tests (uniform)
         for (int j0 = c0; j0 < c1; j0 += kTile) {
             // c1 - c0 and P0 are multiples of kTile: every tile here is full.
             if (PM == kPassCulledB && L.skip_zero) {
@@ -527,7 +557,13 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 cp[(size_t)c * P0 + j0 + tid] = v;
             }
             if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 1;
+            wrote = true;
         }
+        // the row-slot flag: an item whose every tile was skipped leaves its (zero) row partials
+        // unwritten
+        unsigned char *rf = L.colflag + ds.rflag0 + (size_t)k * ds.rflag_kstride + (size_t)P * ds.max_chunks + chunk;
+        if (tid == 0) *rf = wrote ? 1 : 0;
+        if (!wrote) return;
     }
     const size_t rpad = (size_t)ds.nb * R;
     double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;

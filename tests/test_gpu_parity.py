@@ -193,6 +193,26 @@ def test_huge_exponents_use_the_clamped_exp(kernel):
     assert_matches_oracle(locs, times, p, name="huge", kernel=kernel)
 
 
+@pytest.mark.parametrize("kernel", ["symmetric", "split"])
+def test_wide_region_far_offset_locations(kernel):
+    """Locations over 3000 km, offset by 5e6 (projected coordinates), with a 1.6 km background
+    kernel: tauX x diagonal is past the bound of the base-2 form's scaled coordinates, so the
+    natural form must run; every pair's distance is formed from the original coordinates."""
+    rng = np.random.default_rng(79)
+    n = 3000
+    locs = 5e6 + rng.uniform(0, 3000.0, (n, 2))
+    locs[: n // 3] = locs[n // 3: 2 * (n // 3)] + rng.normal(0, 0.5, (n // 3, 2))
+    times = np.sort(rng.uniform(0, 8760.0, n))
+    assert_matches_oracle(locs, times, synth.P0, probs=False, name="wide", kernel=kernel)
+    # isolated events have subnormal probabilities (excitation terms near e^-745): those agree to
+    # the subnormal spacing, every other one to PROB_TOL
+    e = make_engine(locs, times, synth.P0, kernel=kernel)
+    p = e.probs_self_excite()
+    e.close()
+    p_ref = oracle.total(locs, times, synth.P0, probs=True)["probs"]
+    np.testing.assert_allclose(p, p_ref, rtol=PROB_TOL, atol=1e-300)
+
+
 @pytest.mark.parametrize("kernel", ["symmetric", "symmetric_a1", "rowwise", "split"])
 def test_near_tied_times_keep_the_strict_mask(kernel):
     """Pairs of events one ulp apart in time excite each other (dt > 0), exact ties do not. Scaled
