@@ -104,6 +104,27 @@ def test_split_datasets_match_single_engines():
     assert rel_err(got[0][2], ref["ll"]) <= 1e-12
 
 
+def test_split_dataset_fp32_bulk_error():
+    """FP32-bulk batch with a dataset on the split passes: every vector within 1e-6 of the FP64
+    single-engine result (and the split dataset's FP32 error of the same order as the others)."""
+    data = [synth.cluster_process(47000, side=60.0, seed=55), synth.cluster_process(2000, side=4.5, seed=56)]
+    params = [synth.jitter_params(2, sd=0.05, seed=970 + d) for d in range(len(data))]
+    b = make_batch(data, params, flags=_lib.OPENCL | _lib.FLOAT)
+    ll, g = b.log_likelihood()
+    assert b.last_split(0) and not b.last_split(1)
+    lls = b.split(ll)
+    for d, (locs, times) in enumerate(data):
+        e = Engine(2, len(times))
+        e.update_locations(locs)
+        e.set_times(times)
+        for k in range(2):
+            e.set_parameters(params[d][k])
+            l64 = e.log_likelihood()
+            print(d, k, lls[d][k], l64, rel_err(lls[d][k], l64), flush=True)
+            assert rel_err(lls[d][k], l64) <= 1e-6, (d, k)
+        e.close()
+
+
 def test_graph_replays_match_eager():
     data = uneven_datasets(4, 500, 8000, seed=31)
     params = [synth.jitter_params(2, seed=900 + d) for d in range(len(data))]
@@ -174,3 +195,7 @@ def test_batch_errors():
     assert ex.value.code == _lib.HPH_ERR_NOT_INITIALIZED
     with pytest.raises(_lib.HphError):
         Batch(4, [10])
+    with pytest.raises(_lib.HphError) as ex:
+        b.last_split(2)
+    assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
+    assert not b.last_split(0)  # nothing evaluated yet
