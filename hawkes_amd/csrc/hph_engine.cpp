@@ -233,9 +233,11 @@ static CulledHost make_culled(const std::vector<double> &locs, const std::vector
         c.t[r] = times[u];
         c.perm[r] = r < n ? sp : 0;
     }
-    const int groups = (n + 511) / 512 * (512 / kern::kGroup);
+    const int groups = kern::culled_groups(n);
+    const int tiles = groups * kern::kGroup / kern::kTile;
+    const int supers = (tiles * kern::kTile + kern::kSuperTile - 1) / kern::kSuperTile;
     const int GS = 2 * D + 2;
-    c.gstat.resize((size_t)groups * GS);
+    c.gstat.resize((size_t)(groups + tiles + supers) * GS);
     for (int g = 0; g < groups; ++g) {
         double *e = c.gstat.data() + (size_t)g * GS;
         for (int d = 0; d <= D; ++d) {
@@ -251,6 +253,24 @@ static CulledHost make_culled(const std::vector<double> &locs, const std::vector
             e[2 * D + 1] = std::max(e[2 * D + 1], c.t[r]);
         }
     }
+    // tile and supertile boxes: unions of their groups' boxes
+    auto merge = [&](int box, int g0, int g1) {
+        double *e = c.gstat.data() + (size_t)box * GS;
+        for (int d = 0; d <= D; ++d) {
+            e[d] = 1e300;
+            e[D + 1 + d] = -1e300;
+        }
+        for (int g = g0; g < g1; ++g) {
+            const double *f = c.gstat.data() + (size_t)g * GS;
+            for (int d = 0; d <= D; ++d) {
+                e[d] = std::min(e[d], f[d]);
+                e[D + 1 + d] = std::max(e[D + 1 + d], f[D + 1 + d]);
+            }
+        }
+    };
+    const int gpt = kern::kTile / kern::kGroup, gps = kern::kSuperTile / kern::kGroup;
+    for (int u = 0; u < tiles; ++u) merge(groups + u, u * gpt, (u + 1) * gpt);
+    for (int u = 0; u < supers; ++u) merge(groups + tiles + u, u * gps, std::min(groups, (u + 1) * gps));
     return c;
 }
 

@@ -196,6 +196,12 @@ struct SymLaunch {
 constexpr int kPassBoth = 0;
 constexpr int kPassCulledB = 1;
 constexpr int kGroup = 64; // events per bounding-box group of the culled order
+// Box table of the culled order (SymData::gstat): one box per group of kGroup events for
+// culled_groups(n) groups (every group of the 512-event row blocks), then one per tile of kTile
+// events (culled_groups(n) kGroup / kTile of them), then one per supertile of kSuperTile events
+// (the last one partial), so that the background pass tests a far supertile once.
+constexpr int kSuperTile = 4096;
+__host__ __device__ inline int culled_groups(int n) { return (n + 511) / 512 * (512 / kGroup); }
 
 // Host-side plan of a symmetric pass over datasets of sizes n[d] with K[d] vectors each.
 struct SymPlan {

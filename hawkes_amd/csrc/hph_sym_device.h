@@ -466,13 +466,23 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             group_box<D>(ds.gstat, wrow0 / kGroup, A, wlo, whi);
         }
         bool wrote = false; // a tile of the item passed the workgroup-level skip tests (uniform)
+        const int box_tile0 = culled_groups(n), box_super0 = box_tile0 + culled_groups(n) * kGroup / kTile;
         for (int j0 = c0; j0 < c1; j0 += kTile) {
             // c1 - c0 and P0 are multiples of kTile: every tile here is full.
             if (PM == kPassCulledB && L.skip_zero) {
-                // every background exponent between the block and the tile past the negligible
-                // point: the tile's column partials are zero
+                // every background exponent between the block and a supertile (when one starts
+                // here and lies inside the chunk) or the tile past the negligible point: their
+                // column partials are zero
                 double tlo[D + 1], thi[D + 1];
-                group_box<D>(ds.gstat, j0 / kGroup, kTile / kGroup, tlo, thi);
+                if (j0 % kSuperTile == 0 && j0 + kSuperTile <= c1) {
+                    group_box<D>(ds.gstat, box_super0 + j0 / kSuperTile, 1, tlo, thi);
+                    if (box_background_exponent<D>(blo, bhi, tlo, thi, hx, ht) >= ds.zero_arg_b) {
+                        if (tid < kSuperTile / kTile) cf[(size_t)(j0 / kTile + tid) * ds.nb] = 0;
+                        j0 += kSuperTile - kTile;
+                        continue;
+                    }
+                }
+                group_box<D>(ds.gstat, box_tile0 + j0 / kTile, 1, tlo, thi);
                 if (box_background_exponent<D>(blo, bhi, tlo, thi, hx, ht) >= ds.zero_arg_b) {
                     if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 0;
                     continue;
