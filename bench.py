@@ -57,8 +57,11 @@ def excitation_skip_arg(params, n, D=2):
     50 + ln N + max(0, ln(|cS| / |cB|))), cB = mu0 tauX^D tauT / (2 pi), cS = theta omega sigma^D /
     sqrt(2 pi) (skipped terms change every event's rate by < 2e-22 relative)."""
     sigma, tauX, tauT, omega, theta, mu0 = params
-    cB = abs(mu0 * tauX ** D * tauT) / (2 * np.pi)
-    cS = abs(theta * omega * sigma ** D) / np.sqrt(2 * np.pi)
+    cBs, cSs = mu0 * tauX ** D * tauT, theta * omega * sigma ** D
+    if cSs != 0 and (cBs > 0) != (cSs > 0):
+        return ZERO_ARG
+    cB = abs(cBs) / (2 * np.pi)
+    cS = abs(cSs) / np.sqrt(2 * np.pi)
     if not cB > 0:
         return ZERO_ARG
     r = np.log(cS / cB) if cS > 0 else 0.0

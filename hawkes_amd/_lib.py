@@ -53,6 +53,7 @@ SIGNATURES = {
     "hph_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_get_timing": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_get_last_evaluation": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "hph_get_last_pass": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _sz]),
     "hph_set_pair_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_set_pair_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hph_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),

@@ -93,9 +93,18 @@ struct HostBuf {
 // the host picks the pass variant for given parameters.
 struct Extents {
     double r2max = 0.0, span = 0.0;
-    double block_span = 0.0; // largest time span of 512 consecutive sorted events
-    bool gaps_ok = true;     // consecutive distinct times differ by at least 8 ulp(max |t|)
+    double block_span = 0.0;  // largest time span of 512 consecutive sorted events
+    double window_span = 0.0; // largest time span of a window of the split passes' culled order
+    bool gaps_ok = true;      // consecutive distinct times differ by at least 8 ulp(max |t|)
 };
+
+// Events per window of the culled order: N / 8 rounded down to a power of two, in
+// [1024, 32768] (longer windows make groups smaller in space but longer in time).
+static int cull_window(int n) {
+    int w = 1024;
+    while (w < 32768 && 2 * w <= n / 8) w *= 2;
+    return w;
+}
 
 // locs: N x D in any order; sorted_t: the N times in ascending order.
 static Extents extents_of(const std::vector<double> &locs, const double *sorted_t, size_t n, int D) {
@@ -117,6 +126,8 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
         if (g > 0.0 && g < 8.0 * ulp) e.gaps_ok = false;
     }
     for (size_t b = 0; b < n; b += 512) e.block_span = std::max(e.block_span, sorted_t[std::min(b + 511, n - 1)] - sorted_t[b]);
+    const size_t W = (size_t)cull_window((int)n);
+    for (size_t b = 0; b < n; b += W) e.window_span = std::max(e.window_span, sorted_t[std::min(b + W - 1, n - 1)] - sorted_t[b]);
     return e;
 }
 
@@ -133,7 +144,11 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
 // scaled coordinates (x - x0) sx carry rounding of order eps |x - x0| sx, x0 = the block's first
 // event: a term whose spatial exponent is of order one is off by ~eps x (scale x extent) relative,
 // kept below 1e-13 by sx x diagonal <= 300 and sigma sqrt(log2(e)/2) x diagonal <= 300.
-static bool fast_ok(const Extents &e, const double *params, int K) {
+// The split passes' background pass runs on the culled order, whose row blocks are ordered along
+// a space-filling curve inside a window of consecutive sorted events: their times, and so t - t0,
+// span the whole window, not 512 events. split adds the condition st x (window time span) <= 300,
+// which keeps the scaled times' rounding as small as sx x diagonal <= 300 keeps the coordinates'.
+static bool fast_ok(const Extents &e, const double *params, int K, bool split = false) {
     if (!e.gaps_ok) return false;
     const double diag = std::sqrt(e.r2max);
     for (int k = 0; k < K; ++k) {
@@ -148,16 +163,9 @@ static bool fast_ok(const Extents &e, const double *params, int K) {
         if (!std::isfinite(p[3] * 1.4426950408889634 / st)) return false;
         if (!std::isfinite(p[0] * p[0] / (p[1] * p[1]))) return false;
         if (!(sx * diag <= 300.0 && std::fabs(p[0]) * 0.84932180028801904272 * diag <= 300.0)) return false;
+        if (split && !(st * e.window_span <= 300.0)) return false;
     }
     return true;
-}
-
-// Events per window of the culled order: N / 8 rounded down to a power of two, in
-// [1024, 32768] (longer windows make groups smaller in space but longer in time).
-static int cull_window(int n) {
-    int w = 1024;
-    while (w < 32768 && 2 * w <= n / 8) w *= 2;
-    return w;
 }
 
 // Split passes (chosen automatically for likelihood passes): from kSplitMinEvents events on, when
@@ -559,10 +567,10 @@ static TuneCache &tune_cache() {
     return cache;
 }
 
-class HipHawkes : public AbstractHawkes {
+class HipHawkes : public HawkesEngine {
   public:
     HipHawkes(int D, int N, long flags, int device)
-        : AbstractHawkes(D, N, flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
+        : HawkesEngine(D, N, flags), device_(device < 0 ? 0 : device), fp32_((flags & FLOAT) != 0) {
         if (D < 1 || D > kern::kMaxDim) throw std::invalid_argument("embeddingDimension must be in 1..8");
         if (N < 1) throw std::invalid_argument("locationCount must be >= 1");
         if (N > (1 << 30)) throw std::invalid_argument("locationCount must be at most 2^30");
@@ -890,6 +898,13 @@ class HipHawkes : public AbstractHawkes {
 
     int getLastEvaluation() override { return last_eval_; }
 
+    void getLastPass(int *out, size_t length) override {
+        if (out == nullptr || length < 2) throw std::invalid_argument("last-pass output needs 2 values");
+        if (last_kind_ == 0) throw NotInitialized("no pair pass has run yet");
+        out[0] = last_kind_;
+        out[1] = last_fast_ ? 1 : 0;
+    }
+
     void getTiming(double *out, size_t length) override {
         if (out == nullptr || length < 2) throw std::invalid_argument("timing output needs 2 values");
         out[0] = last_ms_[0];
@@ -1068,9 +1083,9 @@ class HipHawkes : public AbstractHawkes {
         }
         if (cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC || cfg.kind == HPH_PAIR_KERNEL_SPLIT) {
             SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
-            last_fast_ = !force_natural_ && fast_ok(extents_, params, K);
             const bool split = cfg.kind == HPH_PAIR_KERNEL_SPLIT ||
                                (kernel_ == HPH_PAIR_KERNEL_AUTO && skip_zero && split_pays(params, K));
+            last_fast_ = !force_natural_ && fast_ok(extents_, params, K, split);
             if (split) {
                 if (!culled_valid_) build_culled();
                 d_bsum_.ensure(sizeof(double) * (size_t)K * (grad ? 3 : 1) * npad_);
@@ -1090,9 +1105,11 @@ class HipHawkes : public AbstractHawkes {
                              timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr, sums);
             }
             last_split_ = split;
+            last_kind_ = split ? HPH_PAIR_KERNEL_SPLIT : HPH_PAIR_KERNEL_SYMMETRIC;
         } else {
             last_fast_ = false;
             last_split_ = false;
+            last_kind_ = HPH_PAIR_KERNEL_ROWWISE;
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
             g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
@@ -1264,6 +1281,7 @@ class HipHawkes : public AbstractHawkes {
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
     bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
     bool force_natural_ = false, last_fast_ = false, exact_s_ = false, last_split_ = false;
+    int last_kind_ = 0; // pass of the last full evaluation (HPH_PAIR_KERNEL_*), 0 before the first
     std::map<long long, bool> sym_fits_;
 
     bool has_stored_ = false, stored_have_params_ = false, stored_have_locs_ = false, stored_ll_valid_ = false;
@@ -1432,8 +1450,8 @@ class HipHawkesBatch {
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_params_.ensure(sizeof(double) * 6 * (size_t)total_);
-        const std::vector<char> fast = fast_flags();
         const std::vector<char> split = split_flags();
+        const std::vector<char> fast = fast_flags(split);
         // the graph holds one variant, table and split layout
         if (graph_exec_ && (fast != graph_fast_ || split != graph_split_)) drop_graph();
         // vector slots: the datasets of the time-sorted pass first (dataset order), then each
@@ -1498,12 +1516,13 @@ class HipHawkesBatch {
     double lastMs() const { return last_ms_; }
 
   private:
-    // Per dataset: whether all its vectors of the current parameters allow the base-2 form.
-    std::vector<char> fast_flags() const {
+    // Per dataset: whether all its vectors of the current parameters allow the base-2 form (in the
+    // split passes where split[d] is set).
+    std::vector<char> fast_flags(const std::vector<char> &split) const {
         std::vector<char> fast(counts_.size());
         const double *hp = params_.data();
         for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
-            fast[d] = fast_ok(ext_[d], hp + 6 * k0, K_[d]) ? 1 : 0;
+            fast[d] = fast_ok(ext_[d], hp + 6 * k0, K_[d], split[d] != 0) ? 1 : 0;
         return fast;
     }
 
@@ -1631,8 +1650,12 @@ class HipHawkesBatch {
     HostBuf h_params_, h_result_;
 };
 
-SharedPtr factory(int dim1, int dim2, long flags, int device, int /*threads*/) {
+EngineSharedPtr engineFactory(int dim1, int dim2, long flags, int device, int /*threads*/) {
     return std::make_shared<HipHawkes>(dim1, dim2, flags, device);
+}
+
+SharedPtr factory(int dim1, int dim2, long flags, int device, int threads) {
+    return engineFactory(dim1, dim2, flags, device, threads);
 }
 
 } // namespace hph
@@ -1642,7 +1665,7 @@ SharedPtr factory(int dim1, int dim2, long flags, int device, int /*threads*/) {
 // ---------------------------------------------------------------------------------------
 
 struct hph_engine {
-    hph::SharedPtr impl;
+    hph::EngineSharedPtr impl;
 };
 
 static thread_local std::string g_last_error;
@@ -1694,7 +1717,7 @@ int hph_create(int embedding_dimension, int location_count, long flags, int devi
     }
     *out = nullptr;
     return guarded([&] {
-        auto impl = hph::factory(embedding_dimension, location_count, flags, device, threads);
+        auto impl = hph::engineFactory(embedding_dimension, location_count, flags, device, threads);
         *out = new hph_engine{impl};
     });
 }
@@ -1814,6 +1837,11 @@ int hph_set_pair_kernel(hph_engine *e, int kind) {
 int hph_get_timing(hph_engine *e, double *out, size_t length) {
     HPH_REQUIRE_ENGINE(e);
     return guarded([&] { e->impl->getTiming(out, length); });
+}
+
+int hph_get_last_pass(hph_engine *e, int *out, size_t length) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->getLastPass(out, length); });
 }
 
 int hph_get_last_evaluation(hph_engine *e, int *kind) {

@@ -65,15 +65,19 @@ inline double background_skip_arg(long long n, bool fp32) {
 // excitation parts of the gradient terms, by at most |cS| n e^-X_S <= e^-50 |cB| = 2e-22 |cB|,
 // i.e. relatively by under 2e-22. The self-excitation probability S/(B+S) then moves by under
 // 2e-22 in absolute terms, which is not a relative bound for tiny probabilities: passes that
-// return probabilities skip exact zeros only. Without a background (cB = 0) nothing is cut.
+// return probabilities skip exact zeros only. Without a background (cB = 0) nothing is cut, and
+// neither when cB and cS differ in sign (theta, omega, mu0 or a precision negative: the rate is
+// then a difference and the floor lambda_i >= |cB| does not hold).
 __host__ __device__ inline double excitation_skip_arg(const double *pk, int D, long long n) {
     double sD = 1.0, xD = 1.0;
     for (int q = 0; q < D; ++q) {
         sD *= pk[0];
         xD *= pk[1];
     }
-    const double cB = fabs(pk[5] * xD * pk[2]) * 0.15915494309189533577;
-    const double cS = fabs(pk[4] * pk[3] * sD) * 0.39894228040143267794;
+    const double cBs = pk[5] * xD * pk[2], cSs = pk[4] * pk[3] * sD;
+    if (cSs != 0.0 && (cBs > 0.0) != (cSs > 0.0)) return kZeroExpArg;
+    const double cB = fabs(cBs) * 0.15915494309189533577;
+    const double cS = fabs(cSs) * 0.39894228040143267794;
     if (!(cB > 0.0) || !(cS < 1e300)) return kZeroExpArg;
     const double r = cS > 0.0 ? log(cS / cB) : 0.0;
     const double x = 50.0 + log((double)(n > 1 ? n : 1)) + (r > 0.0 ? r : 0.0);

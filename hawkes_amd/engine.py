@@ -139,6 +139,14 @@ class Engine:
         check(_lib.lib().hph_get_last_evaluation(self._h, ctypes.byref(out)))
         return out.value
 
+    def last_pass(self):
+        """(pair pass, base-2 form) of the last full evaluation: pass = _lib.PAIR_KERNEL_ROWWISE,
+        PAIR_KERNEL_SYMMETRIC or PAIR_KERNEL_SPLIT; base-2 form True when the pass ran on scaled
+        coordinates and times (2^-y of a sum of squares), False for the natural form."""
+        out = (ctypes.c_int * 2)()
+        check(_lib.lib().hph_get_last_pass(self._h, out, 2))
+        return int(out[0]), bool(out[1])
+
     def timing(self):
         """(pair-pass ms, finalize + reduction ms) of the last evaluation."""
         out = np.zeros(2)

@@ -27,9 +27,14 @@ extern "C" {
 #define HPH_FLAG_SSE (1L << 5)    /* accepted; CPU-only, no effect here */
 #define HPH_FLAG_AVX (1L << 6)    /* accepted; CPU-only, no effect here */
 #define HPH_FLAG_AVX512 (1L << 7) /* accepted; CPU-only, no effect here */
-/* Extension: evaluate every pair. By default, tiles of time-sorted events in which every exp
- * argument is <= -746 (so every term is exactly +0.0 in FP64, as in the reference's exp) are
- * skipped; the result is bitwise the same either way. */
+/* Extension: evaluate every pair. By default a likelihood or gradient pass skips tiles of
+ * time-sorted events whose background terms are all below e^-(50 + ln N) and whose excitation
+ * terms are all below e^-X_S, X_S = 50 + ln N + max(0, ln(|cS| / |cB|)) (cB = mu0 tauX^D tauT /
+ * 2 pi, cS = theta omega sigmaX^D / sqrt(2 pi)); the skipped terms move every event's rate and
+ * sums by under 5e-20 relative, so results agree with this flag's to FP64 rounding, not bitwise.
+ * The excitation cut is made only where cB and cS have the same sign. A probability request
+ * skips exactly zero excitation terms only (their cut is bounded relative to the rate, not to a
+ * tiny probability). */
 #define HPH_FLAG_ALL_PAIRS (1L << 16)
 
 /* Status codes. */
@@ -130,6 +135,13 @@ int hph_get_timing(hph_engine *engine, double *out, size_t length);
 #define HPH_EVAL_EXCITATION 1
 #define HPH_EVAL_CACHED 2
 int hph_get_last_evaluation(hph_engine *engine, int *kind);
+/* Pair pass of the last full evaluation (a call that re-evaluated from cached sums ran none):
+ * out[0] = HPH_PAIR_KERNEL_ROWWISE, HPH_PAIR_KERNEL_SYMMETRIC or HPH_PAIR_KERNEL_SPLIT; out[1] = 1
+ * when it ran the base-2 form (coordinates and times scaled so that the background term is 2^-y
+ * of a sum of squares; chosen only where the rounding of the scaled values stays below 1e-13
+ * relative on the terms), 0 for the natural form. length must be >= 2; HPH_ERR_NOT_INITIALIZED
+ * before the first evaluation. */
+int hph_get_last_pass(hph_engine *engine, int *out, size_t length);
 
 /* ---- Pair-pass selection (extension) ----
  * AUTO: the tuned or built-in choice among the others (row-wise below ~12k events or above

@@ -1,6 +1,7 @@
 // A C++ caller written against the reference's engine interface (src/AbstractHawkes.hpp:32-64,
 // the calls of src/benchmark.cpp:154-211): factory, setTimesData, updateLocations,
-// setParameters, getSumOfLikContribs, getProbsSelfExcite, plus getLogLikeGradient. Reads
+// setParameters, getSumOfLikContribs, getProbsSelfExcite, plus getLogLikeGradient of the
+// extended interface hph::HawkesEngine (reached from the factory's pointer by a cast). Reads
 // "D N" then N rows of D locations and a time, then 6 parameters from stdin; prints the
 // log-likelihood, the gradient and the probabilities with 17 significant digits.
 #include "hph_engine.hpp"
@@ -26,7 +27,12 @@ int main() {
         engine->setParameters(params.data(), params.size());
         const double ll = engine->getSumOfLikContribs();
         std::vector<double> grad(6), probs(N);
-        engine->getLogLikeGradient(grad.data(), grad.size());
+        auto ext = std::dynamic_pointer_cast<hph::HawkesEngine>(engine);
+        if (!ext) {
+            std::fprintf(stderr, "error: the factory's engine does not implement hph::HawkesEngine\n");
+            return 1;
+        }
+        ext->getLogLikeGradient(grad.data(), grad.size());
         engine->getProbsSelfExcite(probs.data(), probs.size());
         std::printf("%.17g\n", ll);
         for (double g : grad) std::printf("%.17g ", g);

@@ -96,3 +96,28 @@ def test_cpp_caller_compiles_against_the_engine_header(tmp_path):
     out = str(tmp_path / "dropin")
     subprocess.run(["bash", os.path.join(ROOT, "tests", "cpp", "build.sh"), out], check=True)
     assert os.path.exists(out)
+
+
+def test_engine_subclass_of_the_reference_base_class_compiles(tmp_path):
+    """An engine subclass written against src/AbstractHawkes.hpp:32-60 (the nine pure virtuals and
+    the protected members) compiles against include/hph_engine.hpp unchanged and runs without a
+    GPU; observationCount = N (N - 1) / 2 is 64-bit (4,999,950,000 at N = 100,000)."""
+    out = str(tmp_path / "subclass")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "subclass_main.cpp"), "-o", out], check=True)
+    run = subprocess.run([out], capture_output=True, text=True, check=True)
+    assert run.stdout.split() == ["2", "100000", "4999950000", "16", "7"]
+
+
+def test_base_class_has_exactly_the_reference_virtuals():
+    """hph::AbstractHawkes declares the nine pure virtuals of src/AbstractHawkes.hpp:42-51 and no
+    others; the extensions live in hph::HawkesEngine."""
+    text = open(os.path.join(ROOT, "include", "hph_engine.hpp")).read()
+    body = text[text.index("class AbstractHawkes {"):text.index("typedef std::shared_ptr<hph::AbstractHawkes>")]
+    virtuals = re.findall(r"virtual\s+[\w\s\*]+?\b(\w+)\s*\(", body)
+    assert sorted(v for v in virtuals if v != "AbstractHawkes") == sorted([
+        "getSumOfLikContribs", "updateLocations", "getProbsSelfExcite", "storeState", "restoreState",
+        "acceptState", "setTimesData", "setParameters", "getInternalDimension"])
+    for member in ("int embeddingDimension;", "int locationCount;", "long long observationCount;", "long flags;",
+                   "int updatedLocation = -1;"):
+        assert member in body

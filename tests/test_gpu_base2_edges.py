@@ -6,8 +6,14 @@ bounding box); and its background exponent is a plain sum of squares, non-negati
 clamp-free 2^-y. The cases here: a large coordinate scale (sx x diagonal 23.5, against 15 for the
 20 km cluster workload) and a moderate one, and coincident events (groups of four sharing one
 location, two of each group also sharing a time: r = 0 and dt = 0 pairs, the strict dt > 0 mask).
-Same tolerances as the other FP64 parity tests (1e-12 relative; the gradient against
-max(|g|, 1e-3 x the sum of |per-row terms|)), and 1e-6 in FP32-bulk mode.
+Every case asserts, on the host, the conditions under which the engine selects the base-2 form
+(base2_gate, the same tests as the engine's) and that the engine reports having run it
+(Engine.last_pass). Same tolerances as the other FP64 parity tests (1e-12 relative; the gradient
+against max(|g|, 1e-3 x the sum of |per-row terms|)), and 1e-6 in FP32-bulk mode.
+
+The split passes' background pass runs on the culled order, whose blocks span a whole window of
+consecutive events in time; the base-2 form is then also gated on st x (window time span) <= 300.
+test_split_pass_wide_window_takes_the_natural_form builds data past that gate.
 """
 import numpy as np
 import pytest
@@ -19,15 +25,53 @@ from hawkes_amd import Engine, _lib, synth
 pytestmark = pytest.mark.gpu
 
 SX_PER_TAUX = 0.84932180028801904272  # sqrt(log2(e) / 2)
+HORIZON = 6000.0  # h: 16,384 events keep |omega| x (span of 512 sorted events) well below 300
+
+
+def cull_window(n):
+    """Events per window of the split passes' culled order (N / 8 as a power of two in [1024, 32768])."""
+    w = 1024
+    while w < 32768 and 2 * w <= n // 8:
+        w *= 2
+    return w
+
+
+def base2_gate(locs, times, p, split=False):
+    """The engine's conditions for the base-2 form (fast_ok in the engine), restated: distinct
+    times at least 8 ulp apart, |omega| x (largest span of 512 sorted events) <= 300, every
+    exponent below 1e9, sx x diagonal <= 300, sigma sqrt(log2(e)/2) x diagonal <= 300, and for the
+    split passes st x (largest window time span) <= 300."""
+    t = np.sort(np.asarray(times, dtype=float))
+    n = t.size
+    tabs = max(abs(t[0]), abs(t[-1]))
+    ulp = np.spacing(tabs)
+    g = np.diff(t)
+    if np.any((g > 0) & (g < 8 * ulp)):
+        return False
+    block_span = max(t[min(b + 511, n - 1)] - t[b] for b in range(0, n, 512))
+    W = cull_window(n)
+    window_span = max(t[min(b + W - 1, n - 1)] - t[b] for b in range(0, n, W))
+    r2max = float(np.sum((locs.max(axis=0) - locs.min(axis=0)) ** 2))
+    diag = np.sqrt(r2max)
+    span = t[-1] - t[0]
+    sigma, tauX, tauT, omega = p[0], p[1], p[2], p[3]
+    st, sx = abs(tauT) * SX_PER_TAUX, abs(tauX) * SX_PER_TAUX
+    ok = abs(omega) * block_span <= 300.0
+    ok &= 0.5 * tauX ** 2 * r2max + 0.5 * tauT ** 2 * span ** 2 < 1e9
+    ok &= abs(omega) * span + 0.5 * sigma ** 2 * r2max < 1e9
+    ok &= sx * diag <= 300.0 and abs(sigma) * SX_PER_TAUX * diag <= 300.0
+    if split:
+        ok &= st * window_span <= 300.0
+    return bool(ok)
 
 
 def events_at_scale(n, sx_diag, seed):
     """n events, uniform on a square whose diagonal times sx is sx_diag (params P0), times uniform
-    on [0, 8760] h and sorted."""
+    on [0, HORIZON) h and sorted."""
     rng = np.random.Generator(np.random.PCG64(seed))
     side = sx_diag / (synth.P0[1] * SX_PER_TAUX) / np.sqrt(2.0)
     locs = rng.uniform(0.0, side, size=(n, 2))
-    times = np.sort(rng.uniform(0.0, 8760.0, size=n))
+    times = np.sort(rng.uniform(0.0, HORIZON, size=n))
     return locs, times
 
 
@@ -37,25 +81,30 @@ def coincident_events(n, seed):
     m = n // 4
     base = rng.uniform(0.0, 20.0, size=(m, 2))
     locs = np.repeat(base, 4, axis=0)
-    t = rng.uniform(0.0, 8760.0, size=(m, 3))
+    t = rng.uniform(0.0, HORIZON, size=(m, 3))
     times = np.concatenate([t[:, :1], t], axis=1).reshape(-1)  # the first two of a group tie
     order = np.argsort(times, kind="stable")
     return locs[order], times[order]
 
 
-def check(locs, times, params, rows_per_lane, flags, tol_ll, tol_g, tol_p):
+def check(locs, times, params, rows_per_lane, flags, tol_ll, tol_g, tol_p, kind=_lib.PAIR_KERNEL_SYMMETRIC,
+          base2=True):
+    assert base2_gate(locs, times, params, split=kind == _lib.PAIR_KERNEL_SPLIT) == base2
     ref = oracle.total(locs, times, params, precision="double", probs=True)
     e = Engine(2, len(times), flags=flags)
-    e.set_pair_config(_lib.PAIR_KERNEL_SYMMETRIC, rows_per_lane, 4096)
+    e.set_pair_config(kind, rows_per_lane, 4096)
     e.update_locations(locs)
     e.set_times(times)
     e.set_parameters(params)
     ll, g = e.log_likelihood_and_gradient()
+    assert e.last_pass() == (kind, base2)
     gerr = np.abs(g - ref["grad"]) / np.maximum(np.abs(ref["grad"]), 1e-3 * ref["scale"][1:])
     p = e.probs_self_excite()
+    assert e.last_pass() == (kind, base2)
     perr = np.max(np.abs(p - ref["probs"]) / np.maximum(np.abs(ref["probs"]), 1e-300))
-    print("N=%d A=%d flags=%d: ll %.2e grad %.2e probs %.2e" % (len(times), rows_per_lane, flags,
-                                                                 rel_err(ll, ref["ll"]), gerr.max(), perr))
+    print("N=%d A=%d flags=%d kind=%d base2=%d: ll %.2e grad %.2e probs %.2e" % (
+        len(times), rows_per_lane, flags, kind, base2, rel_err(ll, ref["ll"]), gerr.max(), perr), flush=True)
+    e.close()
     assert rel_err(ll, ref["ll"]) <= tol_ll
     assert gerr.max() <= tol_g, gerr
     assert perr <= tol_p
@@ -78,3 +127,23 @@ def test_base2_form_with_coincident_events(rows_per_lane):
 def test_base2_form_fp32_bulk_at_large_coordinate_scale():
     locs, times = events_at_scale(16384, 23.5, seed=669)
     check(locs, times, synth.P0, 2, _lib.OPENCL | _lib.FLOAT, 1e-6, 1e-6, 1e-6)
+
+
+def test_base2_form_in_the_split_passes():
+    locs, times = events_at_scale(16384, 23.5, seed=670)
+    check(locs, times, synth.P0, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SPLIT)
+
+
+def test_split_pass_wide_window_takes_the_natural_form():
+    """tauT = 30 / h on 8,000 events over 1,000 h: the time-sorted blocks (512 events, ~64 h)
+    pass every gate of the base-2 form, but a window of the culled order (1,024 events, ~128 h)
+    has st x span ~ 3,300, so the split passes must take the natural form; the time-sorted pass
+    keeps the base-2 form."""
+    rng = np.random.Generator(np.random.PCG64(671))
+    n = 8000
+    locs = rng.uniform(0.0, 4.0, size=(n, 2))
+    times = np.sort(rng.uniform(0.0, 1000.0, size=n))
+    p = np.array([1.0, 1 / 1.6, 30.0, 1.0, 1.0, 1.0])
+    assert base2_gate(locs, times, p, split=False)
+    check(locs, times, p, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SPLIT, base2=False)
+    check(locs, times, p, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SYMMETRIC, base2=True)

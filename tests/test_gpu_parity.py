@@ -248,6 +248,24 @@ def test_boundary_parameters_and_negative_times(name, params, kernel):
     assert_matches_oracle(locs, times, params, name=name, kernel=kernel)
 
 
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
+def test_negative_theta_takes_no_excitation_cut(kernel):
+    """theta < 0 (accepted by setParameters, unphysical): lambda_i = cB sum B - |cS| sum S is a
+    difference, so the floor lambda_i >= |cB| that bounds the negligible-excitation cut does not
+    hold; with cB and cS of different signs the passes skip exactly zero terms only. Results
+    match the oracle (1e-12) and the every-pair evaluation to rounding (1e-15)."""
+    locs, times = synth.cluster_process(3000, side=np.sqrt(4300 / 100), seed=91)
+    p = np.array([1.0, 1 / 1.6, 1 / 336.0, 1.0, -3e-3, 1.0])
+    ref = assert_matches_oracle(locs, times, p, name="theta<0", kernel=kernel)
+    out = []
+    for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
+        e = make_engine(locs, times, p, kernel=kernel, flags=flags)
+        out.append(e.log_likelihood_and_gradient())
+        e.close()
+    assert rel_err(out[0][0], out[1][0]) <= 1e-15
+    assert np.all(np.abs(out[0][1] - out[1][1]) <= 1e-15 * ref["scale"][1:])
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
