@@ -9,10 +9,11 @@ vector each step (log-normal jitter, sd 0.1, around (1, 1/1.6, 1/336, 1, 1, 1)).
 resident in device memory before timing starts.
 
 Prints one JSON line. Options: --steps K --warmup W --gpus N (only 1 is supported)
---batch K64 (0 skips the batched leg) --cpu-seconds S (0 skips the CPU baseline) --datasets D
-(0 skips the uneven dataset batch) --large 0|1 (the N ~ 1e6 leg) --legs (extra legs: all_pairs,
-fp32, small, locgrad, mh = the adaptive MH sampler on re-evaluated proposals)
---config (a fixed pair-pass configuration instead of the autotuned one).
+--batch K64 (0 skips the batched legs: all six parameters jittered, and tauX, tauT shared)
+--cpu-seconds S (0 skips the CPU baseline) --datasets D (0 skips the uneven dataset batch)
+--large 0|1 (the N ~ 1e6 leg) --legs (extra legs: all_pairs, fp32, small, locgrad, mh = the
+adaptive MH sampler on re-evaluated proposals) --config (a fixed pair-pass configuration instead
+of the autotuned one) --only (run the listed legs alone, e.g. for a profile of one of them).
 
 value counts the N^2 ordered pairs the reference forms per call; the engine forms each
 background pair once for both events and skips tiles whose background terms are all below
@@ -237,8 +238,11 @@ def dataset_batch(count):
     """Uneven multi-dataset batch: `count` cluster-process datasets (N log-uniform in 2e3..2e5),
     8 jittered parameter vectors each, LL + gradient for all in one graph-replayed launch sequence
     (one time-sorted pass over the datasets, the split passes for those spread over several
-    negligible-background distances: split_datasets), FP64 and FP32-bulk; FP32 error stated per
-    dataset against the FP64 result."""
+    negligible-background distances: split_datasets), FP64 and FP32-bulk, each with the work
+    decomposition the batch autotuner picks for the table. Errors: vector 0 of every dataset
+    against the FP64 oracle (values committed in tests/golden/full_size_oracle.json by
+    tests/golden/make_full_size.py for these same inputs: long double up to N = 50,000, double
+    above), and every vector of the FP32-bulk batch against the FP64 batch."""
     from hawkes_amd import _lib, synth
     from hawkes_amd.batch import Batch
     data = synth.uneven_datasets(count)
@@ -246,12 +250,19 @@ def dataset_batch(count):
     pairs = sum(8 * len(t) ** 2 for _, t in data)
     out = {"datasets": count, "vectors_per_dataset": 8, "N_min": min(len(t) for _, t in data),
            "N_max": max(len(t) for _, t in data), "pairs_per_launch": float(pairs)}
+    refs = {}
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "full_size_oracle.json")) as f:
+            refs = {c["dataset"]: c for c in json.load(f)["uneven_datasets"]} if count == 32 else {}
+    except (OSError, ValueError, KeyError):
+        refs = {}
     res = {}
     for name, flags in (("fp64", _lib.OPENCL), ("fp32_bulk", _lib.OPENCL | _lib.FLOAT)):
         b = Batch(2, [len(t) for _, t in data], flags=flags)
         for d, (l, t) in enumerate(data):
             b.set_data(d, l, t)
         b.set_parameters(params)
+        tuned_ms = b.autotune(repeats=1)
         b.set_graph(True)
         b.log_likelihood()  # captures the graph
         t0 = time.perf_counter()
@@ -260,15 +271,30 @@ def dataset_batch(count):
             ll, g = b.log_likelihood()
         wall = (time.perf_counter() - t0) / reps
         res[name] = (ll, g)
-        out[name] = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing(),
-                     "split_datasets": [{"dataset": d, "N": len(data[d][1])} for d in range(count) if b.last_split(d)]}
+        A, target, source = b.config()
+        rec = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing(),
+               "config": {"rows_per_lane": A, "target_items": target, "tuned": source == 1,
+                          "autotune_best_device_ms": tuned_ms},
+               "split_datasets": [{"dataset": d, "N": len(data[d][1])} for d in range(count) if b.last_split(d)]}
+        if refs:
+            errs = {}
+            for d, c in sorted(refs.items()):
+                row = 8 * d
+                go, sc = np.asarray(c["grad"]), np.asarray(c["scale"][1:])
+                errs[str(d)] = {"N": c["N"], "ll": abs(ll[row] - c["ll"]) / abs(c["ll"]),
+                                "grad": float(np.max(np.abs(g[row] - go) / np.maximum(np.abs(go), 1e-3 * sc)))}
+            rec["rel_err_vs_fp64_oracle_vector0"] = errs
+            rec["rel_err_vs_fp64_oracle_max"] = {"ll": max(e["ll"] for e in errs.values()),
+                                                 "grad": max(e["grad"] for e in errs.values())}
+        out[name] = rec
         b.close()
     ll64, g64 = res["fp64"]
     ll32, g32 = res["fp32_bulk"]
     rel = np.abs(ll32 - ll64) / np.abs(ll64)
     starts = np.cumsum([0] + [8] * count)
-    out["fp32_bulk"]["rel_err_ll_per_dataset_max"] = [float(rel[starts[d]:starts[d + 1]].max()) for d in range(count)]
-    out["fp32_bulk"]["rel_err_ll_max"] = float(rel.max())
+    out["fp32_bulk"]["rel_err_ll_vs_fp64_batch_per_dataset_max"] = [
+        float(rel[starts[d]:starts[d + 1]].max()) for d in range(count)]
+    out["fp32_bulk"]["rel_err_ll_vs_fp64_batch_max"] = float(rel.max())
     return out
 
 
@@ -416,6 +442,72 @@ def small_n():
             "K64_plain_ms": 1e3 * t_plain, "K64_batched_ms": 1e3 * t_batch, "K64_speedup": t_plain / t_batch}
 
 
+def batched_leg(e, K, reps, n, plain_value):
+    """K = 64 jittered vectors (all six parameters, synth.jitter_params(K, seed=668)) evaluated in
+    one launch (LL + gradient), against the plain per-proposal path (one full evaluation per
+    vector: the headline's rate)."""
+    from hawkes_amd import synth
+    P = synth.jitter_params(K, seed=668)
+    e.set_parameters_batch(P)
+    e.log_likelihood_batch()
+    tb = time.perf_counter()
+    for _ in range(reps):
+        e.log_likelihood_batch()
+    bt = (time.perf_counter() - tb) / reps
+    bvalue = K * n * n / bt
+    out = {"K": K, "value": bvalue, "ms_per_batch": 1e3 * bt, "evaluation": e.last_evaluation()}
+    if plain_value:
+        out["speedup_vs_plain"] = bvalue / plain_value
+    return out
+
+
+def shared_tau_leg(e, K, reps, n, plain_value):
+    """K vectors that share tauX and tauT (the reference sampler's scan set: sigma, omega, theta,
+    mu0 jittered, R/Untitled.R:326; synth.jitter_params(K, seed=671) with tauX, tauT reset to P0):
+    the batch forms the background sums once and runs one excitation pass per vector. Reported
+    against the plain per-proposal path (one full evaluation per vector: the headline's rate) and
+    against K single calls in sequence (each of which re-evaluates from the previous call's cached
+    background sums)."""
+    from hawkes_amd import _lib, synth
+    P = synth.jitter_params(K, seed=671)
+    P[:, 1:3] = synth.P0[1:3]
+    e.set_parameters_batch(P)
+    e.log_likelihood_batch()
+    assert e.last_evaluation() == _lib.EVAL_EXCITATION
+    tb = time.perf_counter()
+    for _ in range(reps):
+        e.log_likelihood_batch()
+    bt = (time.perf_counter() - tb) / reps
+    e.set_parameters(P[-1])
+    e.log_likelihood_and_gradient()
+    ts = time.perf_counter()
+    for k in range(K):
+        e.set_parameters(P[k])
+        e.log_likelihood_and_gradient()
+    st = time.perf_counter() - ts
+    bvalue = K * n * n / bt
+    out = {"K": K, "value": bvalue, "ms_per_batch": 1e3 * bt, "ms_sequential_single_calls": 1e3 * st,
+           "speedup_vs_sequential_single_calls": st / bt}
+    if plain_value:
+        out["speedup_vs_plain"] = bvalue / plain_value
+    return out
+
+
+LEGS = ("headline", "all_pairs", "fp32", "batched", "shared_tau", "datasets", "large", "small", "locgrad", "mh",
+        "cpu")
+
+
+def formed_pairs(kind, rows_per_lane, times, params, skip):
+    """(background, excitation) pairs formed by the pass that ran, or None for the split passes
+    (their bounding-box skips are not modelled here, so no FLOP count is claimed for them)."""
+    from hawkes_amd import _lib
+    if kind == _lib.PAIR_KERNEL_SYMMETRIC:
+        return pairs_formed_symmetric(times, params, skip, rows_per_lane=rows_per_lane)
+    if kind == _lib.PAIR_KERNEL_ROWWISE:
+        return pairs_formed(times, params, skip)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,11 +521,21 @@ def main():
     ap.add_argument("--large", type=int, default=1, help="1: N ~ 1e6 FP32-bulk + HMC leg, 0: skip")
     ap.add_argument("--legs", default="all_pairs,fp32,small,locgrad,mh",
                     help="comma list of the extra legs to run (all_pairs, fp32, small, locgrad, mh) or 'none'")
+    ap.add_argument("--only", default="all",
+                    help="comma list of legs to run, for profiling one of them alone: " + ", ".join(LEGS) +
+                         " (default all: the headline and every leg the other options enable)")
     ap.add_argument("--config", default="auto",
                     help="pair-pass configuration: 'auto' (autotuned) or kind,rows_per_lane,target_grid with "
                          "kind sym or rowwise, e.g. sym,2,8192 (no autotuning; for profiling one kernel)")
     args = ap.parse_args()
     legs = set() if args.legs == "none" else set(args.legs.split(","))
+    only = None if args.only == "all" else set(args.only.split(","))
+    if only is not None and not only <= set(LEGS):
+        ap.error("--only takes legs among " + ", ".join(LEGS))
+
+    def want(leg, enabled=True):
+        return (only is None and enabled) or (only is not None and leg in only)
+
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
@@ -445,7 +547,7 @@ def main():
     causal = synth.causal_pairs(times)
     params = synth.jitter_params(max(args.steps, 3) + args.warmup)
 
-    def run(flags, steps):
+    def engine(flags):
         e = Engine(2, n, flags=flags)
         e.update_locations(locs)
         e.set_times(times)
@@ -458,10 +560,14 @@ def main():
         e.log_likelihood_and_gradient()  # uploads the data; first launch
         if args.config == "auto":
             e.autotune(repeats=2)        # picks the pair-pass configuration for this shape
+        return e
+
+    def run(flags, steps):
+        e = engine(flags)
         for s in range(args.warmup):
             e.set_parameters(params[s])
             e.log_likelihood_and_gradient()
-        pair_ms, fin_ms, formed, results, kinds = [], [], [], [], []
+        pair_ms, fin_ms, results, kinds, passes = [], [], [], [], []
         t0 = time.perf_counter()
         for s in range(steps):
             e.set_parameters(params[args.warmup + s])
@@ -470,84 +576,111 @@ def main():
             pair_ms.append(a)
             fin_ms.append(b)
             kinds.append(e.last_evaluation())
+            passes.append(e.last_pass())
         wall = time.perf_counter() - t0
         # every step jitters all six parameters: each one is a full pass, none a re-evaluation
         assert all(k == _lib.EVAL_FULL for k in kinds), kinds
+        assert len(set(passes)) == 1, passes
         run.results = results
-        kind, rows_per_lane, _ = e.tuned_config()
-        for s in range(steps):
-            skip = not (flags & _lib.ALL_PAIRS)
-            if kind == _lib.PAIR_KERNEL_SYMMETRIC:
-                formed.append(pairs_formed_symmetric(times, params[args.warmup + s], skip, rows_per_lane=rows_per_lane))
-            else:
-                formed.append(pairs_formed(times, params[args.warmup + s], skip))
+        kind, base2 = passes[0]
+        rows_per_lane = e.tuned_config()[1]
+        skip = not (flags & _lib.ALL_PAIRS)
+        formed = [formed_pairs(kind, rows_per_lane, times, params[args.warmup + s], skip) for s in range(steps)]
         return e, wall, pair_ms, fin_ms, formed
 
-    e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps)
-    fp64_results = run.results
-    ms_per_step = 1e3 * wall / args.steps
-    value = n * n / (wall / args.steps)
-
-    pair_avg_ms = float(np.mean(pair_ms))
-    flops = [flop_model(b, x) for b, x in formed]
-    achieved = float(np.mean([f / (ms * 1e-3) for f, ms in zip(flops, pair_ms)])) / 1e12
-    roofline = {
-        "bound": "valu",
-        "achieved": achieved,
-        "peak": FP64_VECTOR_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-        "traffic": load_traffic("sym" if e.tuned_config()[0] == _lib.PAIR_KERNEL_SYMMETRIC else "pair"),
-        "kernel": kernel_name(e.tuned_config()),
-        "kernel_ms": pair_avg_ms,
-        "finalize_reduce_ms": float(np.mean(fin_ms)),
-        "flop_per_launch": float(np.mean(flops)),
-        "pairs_formed": {"background": float(np.mean([b for b, _ in formed])),
-                         "excitation": float(np.mean([x for _, x in formed]))},
+    out = {
+        "metric": METRIC, "value": None, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Hawkes cluster process, 70k background events, seed 666",
+        "config": {"workload": args.workload, "N": n, "causal_pairs": causal, "pairs_per_step": n * n,
+                   "batch_K": args.batch},
     }
 
+    e = None
+    value = None
+    fp64_results = None
+    if want("headline"):
+        e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps)
+        fp64_results = run.results
+        out["ms_per_step"] = 1e3 * wall / args.steps
+        value = n * n / (wall / args.steps)
+        out["value"] = value
+        kind, base2 = e.last_pass()
+        pair_avg_ms = float(np.mean(pair_ms))
+        every_pair_flop = flop_model(n * (n + 1) / 2.0, causal)
+        roofline = {
+            "bound": "valu",
+            "achieved": None, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
+            "traffic": load_traffic("sym" if kind == _lib.PAIR_KERNEL_SYMMETRIC else "pair"),
+            "traffic_source": "not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
+                              "kernel and workload, calibrated by a plain copy, in profiles/pair_kernel_traffic.json",
+            "kernel": kernel_name(e.tuned_config()),
+            "pass": {"kind": kind, "base2_form": base2},
+            "kernel_ms": pair_avg_ms,
+            "kernel_ms_min_max": [float(np.min(pair_ms)), float(np.max(pair_ms))],
+            "finalize_reduce_ms": float(np.mean(fin_ms)),
+            "timing": "HIP events around the pair pass on the engine's stream, averaged over the timed steps",
+            "every_pair_flop": every_pair_flop,
+            "every_pair_tflops_effective": every_pair_flop / (pair_avg_ms * 1e-3) / 1e12,
+        }
+        if all(f is not None for f in formed):
+            flops = [flop_model(b, x) for b, x in formed]
+            achieved = float(np.mean([f / (ms * 1e-3) for f, ms in zip(flops, pair_ms)])) / 1e12
+            nb = float(np.mean([b for b, _ in formed]))
+            nx = float(np.mean([x for _, x in formed]))
+            roofline.update({
+                "achieved": achieved, "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+                "flop_per_launch": float(np.mean(flops)),
+                "flop_model": "35 per background pair + 28 per excitation pair formed (exp = 20 FLOP, FMA = 2); "
+                              "pairs counted from the pass's own tile decisions",
+                "pairs_formed": {"background": nb, "excitation": nx},
+                "formed_fraction": {"background_of_N(N+1)/2": nb / (n * (n + 1) / 2.0), "excitation_of_C": nx / causal},
+            })
+        else:
+            roofline["note"] = "split passes: formed pairs not modelled, no FLOP rate claimed"
+        out["roofline"] = roofline
+
     # Every pair formed (no tile skipping): the same results to rounding, more work.
-    all_pairs = None
-    if "all_pairs" in legs:
+    if want("all_pairs", "all_pairs" in legs):
         ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4))
         steps_a = len(pair_a)
-        flop_a = flop_model(*formed_a[0])
-        all_pairs = {
-            "value": n * n / (wall_a / steps_a),
-            "ms_per_step": 1e3 * wall_a / steps_a,
-            "kernel_ms": float(np.mean(pair_a)),
-            "roofline_frac": flop_a / (float(np.mean(pair_a)) * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS,
-        }
+        rec = {"value": n * n / (wall_a / steps_a), "ms_per_step": 1e3 * wall_a / steps_a,
+               "kernel_ms": float(np.mean(pair_a)), "kernel": kernel_name(ea.tuned_config())}
+        if formed_a[0] is not None:
+            rec["roofline_frac"] = flop_model(*formed_a[0]) / (float(np.mean(pair_a)) * 1e-3) / 1e12 / \
+                FP64_VECTOR_PEAK_TFLOPS
+        out["all_pairs"] = rec
         ea.close()
 
     # FP32-bulk mode: the background exponential in FP32, everything else FP64; error against the
-    # FP64 results above.
-    fp32 = None
-    if "fp32" in legs:
+    # FP64 results above (when they ran).
+    if want("fp32", "fp32" in legs):
         ef, wall_f, pair_f, _, formed_f = run(_lib.OPENCL | _lib.FLOAT, args.steps)
-        # same cost model; the FP32-bulk pass is mixed (background exp in FP32, the rest FP64), so
-        # its rate is stated against both vector peaks
-        ach_f = float(np.mean([flop_model(b, x) / (ms * 1e-3) for (b, x), ms in zip(formed_f, pair_f)])) / 1e12
-        ll_err = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(run.results, fp64_results))
-        g_err = max(float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-3 * np.max(np.abs(b[1])))))
-                    for a, b in zip(run.results, fp64_results))
-        fp32 = {
-            "value": n * n / (wall_f / args.steps),
-            "ms_per_step": 1e3 * wall_f / args.steps,
-            "kernel_ms": float(np.mean(pair_f)),
-            "achieved_tflops": ach_f,
-            "frac_fp64_peak": ach_f / FP64_VECTOR_PEAK_TFLOPS,
-            "frac_fp32_peak": ach_f / FP32_VECTOR_PEAK_TFLOPS,
-            "kernel": kernel_name(ef.tuned_config()),
-            "rel_err_ll_vs_fp64": ll_err,
-            "rel_err_grad_vs_fp64": g_err,
-        }
+        rec = {"value": n * n / (wall_f / args.steps), "ms_per_step": 1e3 * wall_f / args.steps,
+               "kernel_ms": float(np.mean(pair_f)), "kernel": kernel_name(ef.tuned_config())}
+        if formed_f[0] is not None:
+            # same cost model; the FP32-bulk pass is mixed (background exp in FP32, the rest FP64), so
+            # its rate is stated against both vector peaks
+            ach_f = float(np.mean([flop_model(b, x) / (ms * 1e-3) for (b, x), ms in zip(formed_f, pair_f)])) / 1e12
+            rec.update({"achieved_tflops": ach_f, "frac_fp64_peak": ach_f / FP64_VECTOR_PEAK_TFLOPS,
+                        "frac_fp32_peak": ach_f / FP32_VECTOR_PEAK_TFLOPS})
+        if fp64_results is not None:
+            rec["rel_err_ll_vs_fp64"] = max(abs(a[0] - b[0]) / abs(b[0]) for a, b in zip(run.results, fp64_results))
+            rec["rel_err_grad_vs_fp64"] = max(
+                float(np.max(np.abs(a[1] - b[1]) / np.maximum(np.abs(b[1]), 1e-3 * np.max(np.abs(b[1])))))
+                for a, b in zip(run.results, fp64_results))
+        out["fp32_bulk"] = rec
         ef.close()
+
+    need_e = want("locgrad", "locgrad" in legs) or want("batched", args.batch > 0) or \
+        want("shared_tau", args.batch > 0)
+    if e is None and need_e:
+        e = engine(_lib.OPENCL)
 
     # Location gradient d ll / d x (N x D): after a parameter change it runs a natural-form rate
     # pass (1/lambda) and the gradient pass ("cold"); at unchanged state the rates are reused.
-    locgrad = None
-    if "locgrad" in legs:
+    if want("locgrad", "locgrad" in legs):
         e.set_parameters(p0)
         e.location_gradient()
         reps = 3
@@ -563,59 +696,29 @@ def main():
             e.location_gradient()
         warm = (time.perf_counter() - t0) / reps
         lt = float(np.mean(cold))
-        locgrad = {"ms_per_call": 1e3 * lt, "rate_pass_ms": float(np.mean([a for a, _ in cold_ms])),
-                   "gradient_pass_ms": float(np.mean([b for _, b in cold_ms])),
-                   "ms_per_call_rates_reused": 1e3 * warm, "pair_evals_per_s": n * n / lt}
+        out["location_gradient"] = {
+            "ms_per_call": 1e3 * lt, "rate_pass_ms": float(np.mean([a for a, _ in cold_ms])),
+            "gradient_pass_ms": float(np.mean([b for _, b in cold_ms])),
+            "ms_per_call_rates_reused": 1e3 * warm, "pair_evals_per_s": n * n / lt}
 
-    batched = None
-    if args.batch > 0:
-        P = synth.jitter_params(args.batch, seed=668)
-        e.set_parameters_batch(P)
-        e.log_likelihood_batch()
-        tb = time.perf_counter()
-        for _ in range(args.batch_steps):
-            e.log_likelihood_batch()
-        bt = (time.perf_counter() - tb) / args.batch_steps
-        bvalue = args.batch * n * n / bt
-        batched = {"K": args.batch, "value": bvalue, "ms_per_batch": 1e3 * bt,
-                   "speedup_vs_plain": bvalue / value}
-    e.close()
+    if want("batched", args.batch > 0):
+        out["batched"] = batched_leg(e, args.batch, args.batch_steps, n, value)
+    if want("shared_tau", args.batch > 0):
+        out["shared_tau_batch"] = shared_tau_leg(e, args.batch, args.batch_steps, n, value)
+    if e is not None:
+        e.close()
 
-    datasets = dataset_batch(args.datasets) if args.datasets > 0 else None
-    large = large_dataset() if args.large else None
-    small = small_n() if "small" in legs else None
-    mh = mh_sampler() if "mh" in legs else None
-
-    cpu = cpu_baseline(locs, times, p0, args.cpu_seconds) if args.cpu_seconds > 0 else None
-    cpu1 = cpu_config1(args.cpu_seconds) if args.cpu_seconds > 0 else None
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "pair-evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: Hawkes cluster process, 70k background events, seed 666",
-        "config": {"workload": args.workload, "N": n, "causal_pairs": causal, "pairs_per_step": n * n,
-                   "batch_K": args.batch},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "batched": batched,
-        "all_pairs": all_pairs,
-        "fp32_bulk": fp32,
-        "dataset_batch": datasets,
-        "large_n1e6": large,
-        "small_n1000": small,
-        "location_gradient": locgrad,
-        "mh_sampler": mh,
-        "cpu_config1": cpu1,
-    }
+    if want("datasets", args.datasets > 0):
+        out["dataset_batch"] = dataset_batch(args.datasets)
+    if want("large", bool(args.large)):
+        out["large_n1e6"] = large_dataset()
+    if want("small", "small" in legs):
+        out["small_n1000"] = small_n()
+    if want("mh", "mh" in legs):
+        out["mh_sampler"] = mh_sampler()
+    if want("cpu", args.cpu_seconds > 0):
+        out["cpu_baseline"] = cpu_baseline(locs, times, p0, max(args.cpu_seconds, 1.0))
+        out["cpu_config1"] = cpu_config1(max(args.cpu_seconds, 1.0))
     print(json.dumps(out))
 
 

@@ -54,6 +54,7 @@ SIGNATURES = {
     "hph_get_timing": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_get_last_evaluation": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "hph_get_last_pass": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _sz]),
+    "hph_get_event_sums": (ctypes.c_int, [_vp, _dp, _sz]),
     "hph_set_pair_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_set_pair_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "hph_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
@@ -67,6 +68,8 @@ SIGNATURES = {
     "hph_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "hph_batch_set_config": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "hph_batch_get_timing": (ctypes.c_int, [_vp, _dp]),
+    "hph_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "hph_batch_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _sz]),
     "hph_batch_last_split": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }
 

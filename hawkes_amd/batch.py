@@ -72,6 +72,21 @@ class Batch:
     def set_config(self, rows_per_lane=2, target_items=8192):
         check(_lib.lib().hph_batch_set_config(self._h, int(rows_per_lane), int(target_items)))
 
+    def autotune(self, repeats=1):
+        """Times the work decompositions on the current datasets and vectors and keeps the fastest
+        for this table's shape (shared with every batch and engine of the process); returns the
+        winner's device time in ms."""
+        out = ctypes.c_double()
+        check(_lib.lib().hph_batch_autotune(self._h, int(repeats), ctypes.byref(out)))
+        return out.value
+
+    def config(self):
+        """(rows per lane, target items, source) of the next evaluation; source 0 built-in, 1 tuned,
+        2 set by set_config."""
+        out = (ctypes.c_int * 3)()
+        check(_lib.lib().hph_batch_get_config(self._h, out, 3))
+        return int(out[0]), int(out[1]), int(out[2])
+
     def timing(self):
         out = ctypes.c_double()
         check(_lib.lib().hph_batch_get_timing(self._h, ctypes.byref(out)))

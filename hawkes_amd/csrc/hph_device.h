@@ -95,6 +95,33 @@ __device__ __forceinline__ double exp2_neg_fp64(double y) {
 constexpr double kSqrtHalfLog2e = 0.84932180028801904272; // sqrt(log2(e) / 2)
 __device__ __forceinline__ double time_scale(double tauT) { return fabs(tauT) * kSqrtHalfLog2e; }
 
+// Double-double constants (hi + lo, |lo| <= ulp(hi) / 2; from a 50-digit evaluation, pinned by
+// tests/test_exp_poly.py): sqrt(log2(e) / 2), 2 ln 2 = 1 / (sqrt(log2(e) / 2))^2 and log2(e).
+constexpr double kSqrtHalfLog2eHi = 0.8493218002880191, kSqrtHalfLog2eLo = -2.5959464104597403e-17;
+constexpr double kTwoLn2Hi = 1.3862943611198906, kTwoLn2Lo = 4.638093627692599e-17;
+constexpr double kLog2eHi = 1.4426950408889634, kLog2eLo = 2.0355273740931033e-17;
+
+// The base-2 form's scales as double-doubles: a rounded scale st would make every scaled time
+// st (t - t0) carry the same relative error, i.e. the pass would evaluate the background kernel at
+// a slightly different tauT than the finalize's gradient formulas use; components of the gradient
+// that cancel between events amplify that mismatch (measured: 1e-15 of the summation scale). With
+// the scale carried to ~2^-104 and applied by one fma, only the random rounding of each scaled
+// value remains. s = (hi, lo) with hi + lo = |tau| sqrt(log2(e) / 2).
+__device__ __forceinline__ double2 time_scale_dd(double tau) {
+    const double a = fabs(tau);
+    const double hi = a * kSqrtHalfLog2eHi;
+    const double err = __builtin_fma(a, kSqrtHalfLog2eHi, -hi); // exact error of the product
+    return make_double2(hi, __builtin_fma(a, kSqrtHalfLog2eLo, err));
+}
+// v (s.hi + s.lo), rounded once.
+__device__ __forceinline__ double scale_by(double v, double2 s) { return __builtin_fma(v, s.x, v * s.y); }
+// A moment summed in the base-2 form's squared units (r'^2 = r^2 tau^2 log2(e) / 2) back to
+// natural units: m 2 ln2 / tau2 with tau2 = tau * tau as the finalize's gradient formulas round it,
+// so that tau2 times the result is m 2 ln 2 to one rounding.
+__device__ __forceinline__ double from_base2_sq(double m, double tau2) {
+    return __builtin_fma(m, kTwoLn2Hi, m * kTwoLn2Lo) / tau2;
+}
+
 // Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the
 // constants are then pre-scaled by log2 e).
 template <bool F32>

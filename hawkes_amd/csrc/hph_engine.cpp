@@ -384,6 +384,24 @@ class SymRunner {
         if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
     }
 
+    // Finalize and reduction of K vectors of one dataset that share tauX and tauT: one set of
+    // background moments for all of them (bsum, [3 or 1][npad], natural units) and per-vector
+    // excitation moments (scache, [K][3 or 1][npad]); result [K][7]. No partial sums are
+    // allocated. ev1 / ev2 (may be null) are recorded before the finalize and after the reduction.
+    void enqueue_shared_background(const DataRef &ref, int dim, const double *params, bool grad, bool fp32, int K,
+                                   const double *bsum, const double *scache, double *result, hipStream_t s,
+                                   hipEvent_t ev1, hipEvent_t ev2) {
+        kern::SymLaunch L = prepare({ref}, {K}, dim, grad, fp32, 2, 1, 1, {0}, false);
+        L.bsum_shared = 1;
+        if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
+        kern::launch_sym_finalize(L, params, grad, nullptr, nullptr, bsum, scache, nullptr, d_block_.as<double>(),
+                                  nullptr, nullptr, s);
+        check(hipGetLastError(), "finalize kernel launch");
+        kern::launch_sym_reduce(L, grad, d_block_.as<double>(), result, s);
+        check(hipGetLastError(), "reduce kernel launch");
+        if (ev2) check(hipEventRecord(ev2, s), "hipEventRecord");
+    }
+
     // Location-gradient pass over one dataset and one parameter vector (the plan of a K = 1
     // gradient evaluation, shared with it): w = 1/lambda (device, sorted), out = N x D (device).
     void enqueue_locgrad(const DataRef &ref, int dim, const double *params, const double *w, double *out,
@@ -400,7 +418,7 @@ class SymRunner {
     // fast[d]: dataset d runs the base-2 form.
     kern::SymLaunch prepare(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, bool grad,
                             bool fp32, int rows_per_lane, int target_items, int skip_zero,
-                            const std::vector<char> &fast) {
+                            const std::vector<char> &fast, bool partials = true) {
         std::vector<int> n(refs.size());
         for (size_t d = 0; d < refs.size(); ++d) n[d] = refs[d].n;
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
@@ -426,13 +444,16 @@ class SymRunner {
                 d_counter_.ensure(sizeof(int));
                 check(hipMemset(d_counter_.ptr, 0, sizeof(int)), "zero queue counter");
             }
-            d_row_.ensure(sizeof(double) * std::max<size_t>(plan_.rowpart_doubles, 1));
-            d_col_.ensure(sizeof(double) * std::max<size_t>(plan_.colpart_doubles, 1));
-            d_flag_.ensure(std::max<size_t>(plan_.colflag_bytes, 1));
             d_block_.ensure(sizeof(double) * std::max<size_t>(plan_.block_partials, 1) * kern::kResultWidth * 2);
             refs_.clear();
             valid_ = true;
             b_valid_ = false;
+        }
+        // the pass's partial sums (a finalize from per-event sums reads none)
+        if (partials) {
+            d_row_.ensure(sizeof(double) * std::max<size_t>(plan_.rowpart_doubles, 1));
+            d_col_.ensure(sizeof(double) * std::max<size_t>(plan_.colpart_doubles, 1));
+            d_flag_.ensure(std::max<size_t>(plan_.colflag_bytes, 1));
         }
         bool refs_same = refs_.size() == refs.size();
         for (size_t d = 0; refs_same && d < refs.size(); ++d)
@@ -503,17 +524,19 @@ struct TuneConfig {
     int target_blocks = 4096;           // grid size the work decomposition aims for
 };
 
-// Shape of a tuned problem: log2 of N, dimension, batch size, gradient, precision mode.
+// Shape of a tuned problem: log2 of N, dimension, batch size, gradient, precision mode; for a
+// dataset batch (datasets > 0) log2n is log2 of the launch's pair-evaluations sum_d K_d N_d^2 and K
+// the total number of vectors.
 struct TuneKey {
-    int log2n = 0, dim = 0, K = 0, grad = 0, fp32 = 0;
+    int log2n = 0, dim = 0, K = 0, grad = 0, fp32 = 0, datasets = 0;
     bool operator<(const TuneKey &o) const {
-        return std::tie(log2n, dim, K, grad, fp32) < std::tie(o.log2n, o.dim, o.K, o.grad, o.fp32);
+        return std::tie(log2n, dim, K, grad, fp32, datasets) < std::tie(o.log2n, o.dim, o.K, o.grad, o.fp32, o.datasets);
     }
 };
 
-// Process-wide tuning results, shared by all engines. When HPH_TUNE_CACHE names a file, results
-// are read from it at first use and appended to it after each tuning run (one line per shape:
-// "log2n dim K grad fp32 kind rows_per_lane target_blocks").
+// Process-wide tuning results, shared by all engines and dataset batches. When HPH_TUNE_CACHE names
+// a file, results are read from it at first use and appended to it after each tuning run (one line
+// per shape: "log2n dim K grad fp32 datasets kind rows_per_lane target_blocks").
 class TuneCache {
   public:
     bool lookup(const TuneKey &k, TuneConfig *out) {
@@ -531,8 +554,8 @@ class TuneCache {
         const char *path = std::getenv("HPH_TUNE_CACHE");
         if (path && *path) {
             if (FILE *f = std::fopen(path, "a")) {
-                std::fprintf(f, "%d %d %d %d %d %d %d %d\n", k.log2n, k.dim, k.K, k.grad, k.fp32, c.kind,
-                             c.rows_per_lane, c.target_blocks);
+                std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", k.log2n, k.dim, k.K, k.grad, k.fp32, k.datasets,
+                             c.kind, c.rows_per_lane, c.target_blocks);
                 std::fclose(f);
             }
         }
@@ -548,11 +571,12 @@ class TuneCache {
         if (!f) return;
         TuneKey k;
         TuneConfig c;
-        while (std::fscanf(f, "%d %d %d %d %d %d %d %d", &k.log2n, &k.dim, &k.K, &k.grad, &k.fp32, &c.kind,
-                           &c.rows_per_lane, &c.target_blocks) == 8) {
+        while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d", &k.log2n, &k.dim, &k.K, &k.grad, &k.fp32, &k.datasets,
+                           &c.kind, &c.rows_per_lane, &c.target_blocks) == 9) {
             const bool ok = (c.kind == HPH_PAIR_KERNEL_ROWWISE || c.kind == HPH_PAIR_KERNEL_SYMMETRIC) &&
                             (c.rows_per_lane == 1 || c.rows_per_lane == 2) && c.target_blocks >= 64 &&
-                            c.target_blocks <= (1 << 20) && (c.kind == HPH_PAIR_KERNEL_ROWWISE || k.dim <= 3);
+                            c.target_blocks <= (1 << 24) && (c.kind == HPH_PAIR_KERNEL_ROWWISE || k.dim <= 3) &&
+                            k.datasets >= 0;
             if (ok) map_[k] = c;
         }
         std::fclose(f);
@@ -593,7 +617,7 @@ class HipHawkes : public HawkesEngine {
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (DevBuf *b : {&d_x_, &d_t_, &d_perm_, &d_probs_, &d_params_, &d_part_, &d_block_, &d_result_, &d_w_,
                           &d_locgrad_, &d_xc_, &d_tc_, &d_cperm_, &d_gstat_, &d_bsum_, &d_sums_[0], &d_sums_[1],
-                          &d_sband_})
+                          &d_sband_, &d_sband_batch_})
             b->release();
         h_stage_.release();
         h_result_.release();
@@ -799,7 +823,12 @@ class HipHawkes : public HawkesEngine {
         if (K != batch_.size() / HPH_PARAMETER_COUNT)
             throw std::invalid_argument("K does not match the number of batch parameter vectors");
         const bool grad = gradient != nullptr;
-        run(batch_.data(), (int)K, grad, nullptr);
+        if (evaluate_batch_shared((int)K, grad)) {
+            last_eval_ = HPH_EVAL_EXCITATION;
+        } else {
+            run(batch_.data(), (int)K, grad, nullptr);
+            last_eval_ = HPH_EVAL_FULL;
+        }
         const double *r = h_result_.as<double>();
         for (size_t k = 0; k < K; ++k) {
             loglik[k] = r[k * kern::kResultWidth];
@@ -903,6 +932,22 @@ class HipHawkes : public HawkesEngine {
         if (last_kind_ == 0) throw NotInitialized("no pair pass has run yet");
         out[0] = last_kind_;
         out[1] = last_fast_ ? 1 : 0;
+    }
+
+    // The six per-event sums of the current parameters' last single-vector evaluation with the
+    // gradient (the moments every later re-evaluation starts from), natural units, caller's order.
+    void getEventSums(double *out, size_t length) override {
+        const int N = locationCount;
+        if (out == nullptr || length != (size_t)N * 6) throw std::invalid_argument("event sums need N x 6 values");
+        if (!cache_valid_ || !cache_grad_ || data_dirty_ || std::memcmp(cache_params_, params_, sizeof(params_)) != 0)
+            throw NotInitialized("no per-event sums of the current parameters: evaluate the log-likelihood and "
+                                 "gradient first (dimensions 1..3)");
+        check(hipSetDevice(device_), "hipSetDevice");
+        std::vector<double> host((size_t)6 * npad_);
+        check(hipMemcpy(host.data(), d_sums_[sums_cur_].ptr, sizeof(double) * host.size(), hipMemcpyDeviceToHost),
+              "download event sums");
+        for (int s = 0; s < N; ++s)
+            for (int q = 0; q < 6; ++q) out[(size_t)perm_[s] * 6 + q] = host[(size_t)q * npad_ + s];
     }
 
     void getTiming(double *out, size_t length) override {
@@ -1212,6 +1257,62 @@ class HipHawkes : public HawkesEngine {
         return true;
     }
 
+    // A batch whose K vectors share tauX and tauT (the reference sampler's random scan proposes
+    // sigma, omega, theta and mu0, R/Untitled.R:254-422; multiple-try proposals and parallel
+    // chains of that scan share them too): the background moments depend on the data, tauX and
+    // tauT alone, so they are formed once -- taken from the single-vector cache when it holds them
+    // for these tauX and tauT, else by a full single-vector pass for vector 0, which refills the
+    // cache -- and each vector gets its excitation moments from the band kernel (its own
+    // likelihood-pass cut X_S) and a finalize. Same results as K full passes to rounding.
+    // Dimensions 1..3, at least two vectors, not with HPH_FLAG_ALL_PAIRS; returns false when the
+    // batch does not qualify (the caller then runs the K-vector pass).
+    bool evaluate_batch_shared(int K, bool grad) {
+        if (K < 2 || embeddingDimension > 3 || (flags & ALL_PAIRS)) return false;
+        const double *P = batch_.data();
+        for (int k = 1; k < K; ++k)
+            if (P[6 * (size_t)k + 1] != P[1] || P[6 * (size_t)k + 2] != P[2]) return false;
+        require_ready();
+        check(hipSetDevice(device_), "hipSetDevice");
+        if (data_dirty_) upload();
+        const int N = locationCount, D = embeddingDimension;
+        const bool have = cache_valid_ && (!grad || cache_grad_) && cache_params_[1] == P[1] && cache_params_[2] == P[2];
+        if (!have) {
+            // a full pass for vector 0 whose per-event moments become the single-vector cache
+            d_sums_[sums_cur_].ensure(sizeof(double) * 6 * (size_t)npad_);
+            cache_valid_ = false;
+            run(P, 1, grad, nullptr, false, d_sums_[sums_cur_].as<double>());
+            cache_valid_ = true;
+            cache_grad_ = grad;
+            std::memcpy(cache_params_, P, sizeof(cache_params_));
+            cache_xs_ = kern::excitation_skip_arg(P, D, N);
+        }
+        const int NQ = grad ? 3 : 1;
+        d_params_.ensure(sizeof(double) * 6 * (size_t)K);
+        d_result_.ensure(sizeof(double) * (size_t)K * kern::kResultWidth);
+        h_result_.ensure(sizeof(double) * (size_t)K * kern::kResultWidth);
+        h_params_.ensure(sizeof(double) * 6 * (size_t)K);
+        d_sband_batch_.ensure(sizeof(double) * (size_t)K * NQ * npad_);
+        std::memcpy(h_params_.ptr, P, sizeof(double) * 6 * (size_t)K);
+        check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6 * (size_t)K, hipMemcpyHostToDevice,
+                             stream_),
+              "upload params");
+        const kern::Events ev = events();
+        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        kern::launch_excitation_band(ev, d_params_.as<double>(), K, grad, 0, d_sband_batch_.as<double>(), stream_);
+        check(hipGetLastError(), "excitation pass launch");
+        SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+        sym_batch_.enqueue_shared_background(ref, D, d_params_.as<double>(), grad, fp32_, K,
+                                             d_sums_[sums_cur_].as<double>(), d_sband_batch_.as<double>(),
+                                             d_result_.as<double>(), stream_, timing_ ? ev_[1] : nullptr,
+                                             timing_ ? ev_[2] : nullptr);
+        check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * (size_t)K * kern::kResultWidth,
+                             hipMemcpyDeviceToHost, stream_),
+              "download result");
+        check(hipStreamSynchronize(stream_), "stream sync");
+        if (timing_) record_timing();
+        return true;
+    }
+
     // Single-vector evaluation; caches ll (and the gradient). The probabilities it writes are kept
     // only when the pass skipped exact zeros alone: a request for probabilities runs such a pass
     // (excitation_skip_arg bounds the cut absolutely, not relatively, for tiny probabilities).
@@ -1302,6 +1403,8 @@ class HipHawkes : public HawkesEngine {
     // per-event moments of the last single-vector evaluation (two buffers: one read, one written)
     DevBuf d_sums_[2];
     DevBuf d_sband_; // excitation sums of a re-evaluation ([3][npad])
+    DevBuf d_sband_batch_; // excitation sums of a batch sharing tauX and tauT ([K][3 or 1][npad])
+    SymRunner sym_batch_;  // finalize plans of such batches (kept apart from the single-vector plan)
     int sums_cur_ = 0;
     bool cache_valid_ = false, cache_grad_ = false;
     double cache_params_[6] = {0, 0, 0, 0, 0, 0};
@@ -1428,12 +1531,69 @@ class HipHawkesBatch {
         graph_mode_ = enabled;
     }
 
+    // A fixed work decomposition (rows per lane, target number of work items) instead of the tuned
+    // or built-in one.
     void setConfig(int rows_per_lane, int target_items) {
         if (rows_per_lane != 1 && rows_per_lane != 2) throw std::invalid_argument("rows per lane must be 1 or 2");
         if (target_items < 1 || target_items > (1 << 24)) throw std::invalid_argument("target items out of range");
-        A_ = rows_per_lane;
-        target_ = target_items;
-        drop_graph();
+        forced_ = true;
+        forced_A_ = rows_per_lane;
+        forced_target_ = target_items;
+    }
+
+    // Times each work decomposition (1 or 2 rows per lane; six target item counts) on the current
+    // datasets and vectors (eager launches, device time, the best of `repeats`), keeps the fastest
+    // for this table's shape in the process-wide tuning cache (and the HPH_TUNE_CACHE file), and
+    // uses it from then on unless a configuration is set explicitly.
+    void autotune(int repeats, double *best_ms) {
+        if (!have_params_) throw NotInitialized("parameters are not set");
+        if (repeats < 1) repeats = 1;
+        std::vector<double> ll((size_t)total_), g((size_t)total_ * 6);
+        const bool was_graph = graph_mode_, was_forced = forced_;
+        graph_mode_ = false;
+        double best_t = 1e300;
+        TuneConfig best;
+        best.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+        try {
+            for (int a : {1, 2}) {
+                for (int target : {2048, 4096, 8192, 16384, 32768, 65536}) {
+                    forced_ = true;
+                    forced_A_ = a;
+                    forced_target_ = target;
+                    evaluate(ll.data(), g.data(), ll.size()); // warm: plans, buffers
+                    double t = 1e300;
+                    for (int r = 0; r < repeats; ++r) {
+                        evaluate(ll.data(), g.data(), ll.size());
+                        t = std::min(t, last_ms_);
+                    }
+                    if (t < best_t) {
+                        best_t = t;
+                        best.rows_per_lane = a;
+                        best.target_blocks = target;
+                    }
+                }
+            }
+        } catch (...) {
+            graph_mode_ = was_graph;
+            forced_ = was_forced;
+            throw;
+        }
+        graph_mode_ = was_graph;
+        forced_ = false;
+        tune_cache().store(tune_key(true), best);
+        tune_cache().store(tune_key(false), best);
+        if (best_ms) *best_ms = best_t;
+    }
+
+    // The work decomposition the next evaluation uses: out[0] rows per lane, out[1] target items,
+    // out[2] = 0 built-in, 1 tuned, 2 set explicitly.
+    void getConfig(int *out, size_t length) {
+        if (out == nullptr || length < 3) throw std::invalid_argument("config output needs 3 values");
+        int source = 0;
+        const TuneConfig c = effective_config(grad_, &source);
+        out[0] = c.rows_per_lane;
+        out[1] = c.target_blocks;
+        out[2] = source;
     }
 
     void evaluate(double *loglik, double *grad, size_t total) {
@@ -1446,6 +1606,14 @@ class HipHawkesBatch {
         const bool want_grad = grad != nullptr;
         if (want_grad != grad_) drop_graph();
         grad_ = want_grad;
+        // the work decomposition: set explicitly, tuned for this shape, or built in; the graph
+        // holds one
+        const TuneConfig cfg = effective_config(grad_, nullptr);
+        if (cfg.rows_per_lane != A_ || cfg.target_blocks != target_) {
+            A_ = cfg.rows_per_lane;
+            target_ = cfg.target_blocks;
+            drop_graph();
+        }
         d_params_.ensure(sizeof(double) * 6 * (size_t)total_);
         d_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
         h_result_.ensure(sizeof(double) * kern::kResultWidth * (size_t)total_);
@@ -1516,6 +1684,44 @@ class HipHawkesBatch {
     double lastMs() const { return last_ms_; }
 
   private:
+    // The tuning-cache key of this table: log2 of the pair-evaluations of a launch, dimension,
+    // total vectors, gradient, precision, dataset count.
+    TuneKey tune_key(bool grad) const {
+        double pairs = 0.0;
+        for (size_t d = 0; d < counts_.size(); ++d)
+            pairs += (double)(d < K_.size() ? K_[d] : 1) * (double)counts_[d] * (double)counts_[d];
+        TuneKey key;
+        key.log2n = (int)std::floor(std::log2(std::max(pairs, 1.0)));
+        key.dim = D_;
+        key.K = total_;
+        key.grad = grad ? 1 : 0;
+        key.fp32 = fp32_ ? 1 : 0;
+        key.datasets = (int)counts_.size();
+        return key;
+    }
+
+    // source (may be null): 0 built-in (2 rows per lane, 8192 items), 1 tuned, 2 set explicitly.
+    TuneConfig effective_config(bool grad, int *source) const {
+        TuneConfig c;
+        c.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+        c.rows_per_lane = 2;
+        c.target_blocks = 8192;
+        int src = 0;
+        if (forced_) {
+            c.rows_per_lane = forced_A_;
+            c.target_blocks = forced_target_;
+            src = 2;
+        } else if (have_params_) {
+            TuneConfig t;
+            if (tune_cache().lookup(tune_key(grad), &t)) {
+                c = t;
+                src = 1;
+            }
+        }
+        if (source) *source = src;
+        return c;
+    }
+
     // Per dataset: whether all its vectors of the current parameters allow the base-2 form (in the
     // split passes where split[d] is set).
     std::vector<char> fast_flags(const std::vector<char> &split) const {
@@ -1640,7 +1846,9 @@ class HipHawkesBatch {
         SymRunner runner;
     };
     std::vector<std::unique_ptr<SplitData>> split_;
-    int A_ = 2, target_ = 8192;
+    int A_ = 2, target_ = 8192; // work decomposition of the captured graph / last evaluation
+    bool forced_ = false;
+    int forced_A_ = 2, forced_target_ = 8192;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
@@ -1844,6 +2052,11 @@ int hph_get_last_pass(hph_engine *e, int *out, size_t length) {
     return guarded([&] { e->impl->getLastPass(out, length); });
 }
 
+int hph_get_event_sums(hph_engine *e, double *out, size_t length) {
+    HPH_REQUIRE_ENGINE(e);
+    return guarded([&] { e->impl->getEventSums(out, length); });
+}
+
 int hph_get_last_evaluation(hph_engine *e, int *kind) {
     HPH_REQUIRE_ENGINE(e);
     if (kind == nullptr) {
@@ -1913,6 +2126,16 @@ int hph_batch_set_graph(hph_batch *b, int enabled) {
 int hph_batch_set_config(hph_batch *b, int rows_per_lane, int target_items) {
     HPH_REQUIRE_BATCH(b);
     return guarded([&] { b->impl->setConfig(rows_per_lane, target_items); });
+}
+
+int hph_batch_autotune(hph_batch *b, int repeats, double *best_ms) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->autotune(repeats, best_ms); });
+}
+
+int hph_batch_get_config(hph_batch *b, int *out, size_t length) {
+    HPH_REQUIRE_BATCH(b);
+    return guarded([&] { b->impl->getConfig(out, length); });
 }
 
 int hph_batch_get_timing(hph_batch *b, double *out) {

@@ -188,6 +188,8 @@ struct SymLaunch {
                             // readers skip unflagged tiles. After them, the row-slot flags
                             // (SymData::rflag0): 1 when a chunk item wrote its row partials, 0
                             // when the pass skipped every tile of the item (partials unwritten)
+    int bsum_shared;        // finalize from per-event sums: 1 when bsum holds one set of background
+                            // moments for every vector (vectors sharing tauX and tauT), 0 one per vector
 };
 
 // Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted
@@ -226,7 +228,8 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
 // vector 0 (as launch_finalize).
 // bsum, scache (both null, or both set): per-event background and excitation sums in natural units
 // ([k][3 or 1][npad]; launch_sym_bsums / launch_excitation_band, or the sums of an earlier
-// evaluation); when set, the finalize takes the moments from them instead of the pass's partials.
+// evaluation; bsum without the [k] index when L.bsum_shared); when set, the finalize takes the
+// moments from them instead of the pass's partials.
 // sums (may be null): receives the six moments per event of dataset 0, vector 0 in natural units
 // ([6][npad]; without the gradient moments 0 and 3).
 void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
@@ -246,4 +249,5 @@ void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *
                         double *colpart, double *out, hipStream_t s);
 
 } // namespace kern
+
 } // namespace hph

@@ -72,9 +72,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     for (int q = 0; q < QS; ++q) b[q] = a[q] + cs[q];
     if (GRAD && ds.fast) { // the base-2 pass summed e y (y = r'^2 + dt'^2) and e dt'^2, r' = sx r, dt' = st dt
         const double *pk = params + 6 * (size_t)(ds.param0 + k);
-        const double st = time_scale(pk[2]), sx = time_scale(pk[1]);
-        b[1] = (b[1] - b[2]) / (sx * sx);
-        b[2] /= st * st;
+        b[1] = from_base2_sq(b[1] - b[2], pk[1] * pk[1]);
+        b[2] = from_base2_sq(b[2], pk[2] * pk[2]);
     }
 #pragma unroll
     for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = b[q];

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
             // per-event sums, already complete
 #pragma unroll
             for (int q = 0; q < QS; ++q) {
-                a6[q] = bsum[((size_t)k * QS + q) * ds.npad + i];
+                a6[q] = bsum[((L.bsum_shared ? (size_t)0 : (size_t)k * QS) + q) * ds.npad + i];
                 a6[NR - QS + q] = scache[((size_t)k * QS + q) * ds.npad + i];
             }
         }
@@ -82,14 +82,13 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
             s[4] = a6[4];
             s[5] = a6[5];
             if (ds.fast && !SPLIT) {
-                // the base-2 pass summed e y (y = r'^2 + dt'^2), e dt'^2, e r'^2 and e dt' with
-                // r' = sx r, dt' = st dt
+                // the base-2 pass summed e y (y = r'^2 + dt'^2), e dt'^2, e r'^2 and e dt with
+                // r' = sx r, dt' = st dt (the excitation's dt is natural)
                 const double *pk = params + 6 * (size_t)(ds.param0 + k);
-                const double st = time_scale(pk[2]), sx = time_scale(pk[1]);
-                s[1] = (s[1] - s[2]) / (sx * sx);
-                s[2] /= st * st;
-                s[4] /= sx * sx;
-                s[5] /= st;
+                const double x2 = pk[1] * pk[1], t2 = pk[2] * pk[2];
+                s[1] = from_base2_sq(s[1] - s[2], x2);
+                s[2] = from_base2_sq(s[2], t2);
+                s[4] = from_base2_sq(s[4], x2);
             }
         } else {
             s[0] = a6[0] + cs[0];

@@ -37,22 +37,27 @@ namespace kern {
 //      exp_fp64: background cB0 r^2 + cB1 dt^2 (cB0 = -tauX^2/2, cB1 = -tauT^2/2; times log2 e
 //      for the FP32-bulk v_exp_f32), excitation cS0 r^2 + cS1 dt (cS0 = -sigma^2/2, cS1 = -omega).
 //  base 2 (FAST true): coordinates arrive as x' = sx (x - x0) and times as t' = st (t - t0)
-//      (time_scale of tauX and tauT; x0, t0 = the row block's first event), so that the
-//      background exponent in 2-units is the plain sum of squares y = r'^2 + dt'^2 (clamp-free
-//      exp2_neg_fp64); excitation cS0 r'^2 + cS1 dt' with cS0 = -sigma^2 / tauX^2 and
-//      cS1 = -omega log2(e) / st (exp2_fp64). The background moments are summed as e y and
-//      e dt'^2, the excitation ones as e r'^2 and e dt'; the finalize kernels turn them into
-//      e r^2 = (e y - e dt'^2) / sx^2, e dt^2, e r^2 and e dt.
+//      (time_scale_dd of tauX and tauT, applied by one fma each; x0, t0 = the row block's first
+//      event), so that the background exponent in 2-units is the plain sum of squares y = r'^2 + dt'^2
+//      (clamp-free exp2_neg_fp64); excitation cS0 r'^2 + cS1 dt with cS0 = -sigma^2 / tauX^2,
+//      cS1 = -omega log2(e) and dt the difference of the natural times (staged beside the
+//      scaled ones: a scaled difference carries the rounding of st (t - t0), which an
+//      excitation term would feel as a relative error of order eps omega |t - t0|), evaluated
+//      by exp2_fp64. The background moments are summed as e y and e dt'^2, the excitation ones
+//      as e r'^2 and e dt; the finalize kernels turn them into e r^2 = (e y - e dt'^2) / sx^2,
+//      e dt^2, e r^2 and e dt.
 struct PairConst {
     double cB0, cB1, cS0, cS1;
+    double cS0lo, cS1lo; // base 2: the low parts of cS0 and cS1 (double-doubles), else 0
 };
 
 // Pair geometry from the row's coordinates xi, ti and the column's q (D coordinates, then the
 // time): dt, dt^2 and, in the base-2 form, the background exponent y = r'^2 + dt'^2. r^2 is formed
 // where the natural form or the excitation term needs it (R2); else y takes the squares directly.
-template <int D, bool FAST, bool R2>
-__device__ __forceinline__ void pair_geometry(const double (&xi)[D], double ti, const double (&q)[D + 1], double &r2,
+template <int D, bool FAST, bool R2, int NQ>
+__device__ __forceinline__ void pair_geometry(const double (&xi)[D], double ti, const double (&q)[NQ], double &r2,
                                               double &dt, double &dt2, double &y) {
+    static_assert(NQ >= D + 1, "a column holds D coordinates and the time");
     dt = ti - q[D];
     dt2 = dt * dt;
     if (FAST && !R2) {
@@ -89,8 +94,12 @@ __device__ __forceinline__ double background_term(const PairConst &c, double r2,
 
 template <bool FAST>
 __device__ __forceinline__ double excitation_term(const PairConst &c, double r2, double dt) {
-    const double y = __builtin_fma(c.cS0, r2, c.cS1 * dt);
-    return FAST ? exp2_fp64(y) : exp_fp64(y);
+    if (FAST) {
+        // double-double constants: the exponent's scale matches the finalize's sigma and omega
+        const double u = __builtin_fma(c.cS1, dt, c.cS1lo * dt);
+        return exp2_fp64(__builtin_fma(c.cS0, r2, __builtin_fma(c.cS0lo, r2, u)));
+    }
+    return exp_fp64(__builtin_fma(c.cS0, r2, c.cS1 * dt));
 }
 
 // Adds v to an LDS double (ds_add_f64; the result is not read).
@@ -105,19 +114,22 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 // background exponential; FAST: the base-2 form of PairConst.
 template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
-                                            const double (&ti)[A], const double (&vm)[A], const PairConst &c,
-                                            double (&acc)[A][6], double *cs) {
+                                            const double (&ti)[A], const double (&tn)[A], const double (&vm)[A],
+                                            const PairConst &c, double (&acc)[A][6], double *cs) {
     constexpr int NC = GRAD ? 3 : 1;
+    // components read per column: D coordinates and the time, plus the natural time where the
+    // base-2 form forms excitation terms
+    constexpr int NQ = (FAST && SMODE != kSNone) ? D + 2 : D + 1;
     // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]; the next step's
     // entry is read before this step's slot additions are issued (entry lane + 64 of the last
     // step's read is in the stored copy)
-    double qn[D + 1];
+    double qn[NQ];
 #pragma unroll
-    for (int d = 0; d <= D; ++d) qn[d] = sub[d * 128 + lane];
+    for (int d = 0; d < NQ; ++d) qn[d] = sub[d * 128 + lane];
     auto step = [&](int s) {
-        double q[D + 1];
+        double q[NQ];
 #pragma unroll
-        for (int d = 0; d <= D; ++d) {
+        for (int d = 0; d < NQ; ++d) {
             q[d] = qn[d];
             qn[d] = sub[d * 128 + lane + s + 1];
         }
@@ -127,6 +139,9 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             double r2, dt, dt2, y;
             pair_geometry<D, FAST, !FAST || SMODE != kSNone>(xi[a], ti[a], q, r2, dt, dt2, y);
             const double m1 = FAST ? y : r2; // first background moment: e y (base 2) or e r^2
+            // the excitation term's time difference: natural in both forms
+            double dte = dt;
+            if constexpr (FAST && SMODE != kSNone) dte = tn[a] - q[D + 1];
             if (DOB) {
                 const double eb = background_term<F32, FAST>(c, r2, dt2, y);
                 acc[a][0] += eb;
@@ -150,12 +165,12 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                 }
             }
             if (SMODE != kSNone) {
-                double es = excitation_term<FAST>(c, r2, dt);
-                if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
+                double es = excitation_term<FAST>(c, r2, dte);
+                if (SMODE == kSMixed) es = dte > 0.0 ? es : 0.0;
                 acc[a][3] += es;
                 if (GRAD) {
                     acc[a][4] = __builtin_fma(es, r2, acc[a][4]);
-                    acc[a][5] = __builtin_fma(es, dt, acc[a][5]);
+                    acc[a][5] = __builtin_fma(es, dte, acc[a][5]);
                 }
             }
         }
@@ -175,19 +190,29 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
     }
 }
 
+// Stride (doubles) of one event in a diagonal-block tile: D coordinates, the time and, in the
+// base-2 form, the natural time, padded to 16 bytes.
+template <int D, bool FAST> struct DiagStride {
+    static constexpr int value = FAST ? LocStride<D>::value : TileStride<D>::value;
+};
+
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
 // the dt > 0 mask. PM: the terms of the pass mode (kPassBoth, or kPassCulledB: background only).
+// tn: the row's natural time (the base-2 form's excitation terms).
 template <int D, bool GRAD, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
-                                          const PairConst &c, double (&acc)[6]) {
-    constexpr int ST = TileStride<D>::value;
+                                          double tn, const PairConst &c, double (&acc)[6]) {
+    constexpr int ST = DiagStride<D, FAST>::value;
+    constexpr int NQ = (FAST && PM != kPassCulledB) ? D + 2 : D + 1;
 #pragma unroll 2
     for (int j = 0; j < cnt; ++j) {
-        double q[D + 1];
+        double q[NQ];
 #pragma unroll
-        for (int d = 0; d <= D; ++d) q[d] = tile[j * ST + d];
+        for (int d = 0; d < NQ; ++d) q[d] = tile[j * ST + d];
         double r2, dt, dt2, y;
         pair_geometry<D, FAST, !FAST || PM != kPassCulledB>(xi, ti, q, r2, dt, dt2, y);
+        double dte = dt;
+        if constexpr (FAST && PM != kPassCulledB) dte = tn - q[D + 1];
         {
             const double eb = background_term<F32, FAST>(c, r2, dt2, y);
             acc[0] += eb;
@@ -197,12 +222,12 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
             }
         }
         if (PM != kPassCulledB) {
-            double es = excitation_term<FAST>(c, r2, dt);
-            es = dt > 0.0 ? es : 0.0;
+            double es = excitation_term<FAST>(c, r2, dte);
+            es = dte > 0.0 ? es : 0.0;
             acc[3] += es;
             if (GRAD) {
                 acc[4] = __builtin_fma(es, r2, acc[4]);
-                acc[5] = __builtin_fma(es, dt, acc[5]);
+                acc[5] = __builtin_fma(es, dte, acc[5]);
             }
         }
     }
@@ -211,31 +236,35 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
 // mode = 4 * (background formed) + excitation mode; the culled background pass builds only its own.
 template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST, int PM>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
-                                             const double (&xi)[A][D], const double (&ti)[A], const double (&vm)[A],
-                                             const PairConst &c, double (&acc)[A][6], double *cs) {
+                                             const double (&xi)[A][D], const double (&ti)[A], const double (&tn)[A],
+                                             const double (&vm)[A], const PairConst &c, double (&acc)[A][6],
+                                             double *cs) {
     if constexpr (PM == kPassCulledB) {
-        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         return;
     }
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
         break;
     default: break; // every term of the subtile is exactly zero
     }
 }
+
+// Computed column tiles per partial row sum of a chunk item (sym_item).
+constexpr int kFlushTiles = 4;
 
 // Column side of the symmetric pass, read back per event: the later blocks Q whose pass wrote
 // column tile t (flag set) are listed in ascending order, kQChunk at a time, so that the loads
@@ -360,16 +389,21 @@ __device__ __forceinline__ double box_background_exponent(const double (&alo)[D 
 // One work item of the symmetric pass: row block P of dataset it.x, vector it.w, against a chunk
 // of earlier columns (it.z >= 0) or its own diagonal block (it.z < 0). FAST: the base-2 form.
 // PM: pass mode (kPassBoth; kPassCulledB: background terms of the culled order, bounding-box
-// skips, events of a chunk need not be earlier than the block's).
-template <int D, bool GRAD, int A, bool F32, bool FAST, int PM>
+// skips, events of a chunk need not be earlier than the block's). FLUSH: chunks longer than
+// kFlushTiles tiles sum their rows per group of kFlushTiles tiles (below); a separate variant,
+// because the second set of row sums costs registers that short chunks do not need.
+template <int D, bool GRAD, int A, bool F32, bool FAST, int PM, bool FLUSH>
 __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
                                          double *__restrict__ rowpart, double *__restrict__ colpart,
                                          double *__restrict__ tile, double *__restrict__ colbuf, double *cs, int tid,
                                          int lane, int w) {
-    constexpr int ST = TileStride<D>::value;
+    constexpr int ST = DiagStride<D, FAST>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int R = kSymWaves * 64 * A;
+    // components per column of a chunk subtile: D coordinates, the time and, in the base-2 form,
+    // the natural time
+    constexpr int NCOMP = FAST ? D + 2 : D + 1;
     const SymData &ds = L.data[it.x];
     const int P = it.y, chunk = it.z, k = it.w;
     const double *pk = params + 6 * (size_t)(ds.param0 + k);
@@ -380,45 +414,57 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     const int P0 = P * R;
 
     // base-2 form: coordinates and times relative to the block's first event, scaled by sx and st
-    const double st = FAST ? time_scale(tauT) : 1.0;
-    const double sx = FAST ? time_scale(tauX) : 1.0;
+    const double2 st = FAST ? time_scale_dd(tauT) : make_double2(1.0, 0.0);
+    const double2 sx = FAST ? time_scale_dd(tauX) : make_double2(1.0, 0.0);
     const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
     double x0[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) x0[d] = FAST ? ex[(size_t)d * npad + min(P0, n - 1)] : 0.0;
     PairConst c;
+    c.cS0lo = 0.0;
+    c.cS1lo = 0.0;
     if (FAST) {
         c.cB0 = 1.0;
         c.cB1 = 0.0;
-        c.cS0 = -(sigma * sigma) / (tauX * tauX);
-        c.cS1 = -omega * kLog2e / st;
+        // -sigma^2 / tauX^2 with sigma^2 and tauX^2 rounded as the finalize rounds them, the
+        // quotient carried as a double-double; -omega log2(e) per natural time difference, the same
+        const double s2 = sigma * sigma, x2 = tauX * tauX;
+        const double q = s2 / x2;
+        const double qlo = __builtin_fma(-q, x2, s2) / x2;
+        c.cS0 = -q;
+        c.cS0lo = -qlo;
+        const double w = -omega * kLog2eHi;
+        c.cS1 = w;
+        c.cS1lo = __builtin_fma(-omega, kLog2eHi, -w) + (-omega * kLog2eLo);
     } else {
         c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
         c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);
         c.cS0 = -0.5 * sigma * sigma;
         c.cS1 = -omega;
     }
-    // skip tests: a tile is skipped when the smallest time gap alone drives every background
-    // exponent past the negligible point or every excitation exponent past the underflow
-    // point (in the units of the form)
-    const double zb = FAST ? ds.zero_arg_b * kLog2e : ds.zero_arg_b;
-    const double xs = L.cut_s ? excitation_skip_arg(pk, D, n) : L.zero_arg_s;
-    const double zs = FAST ? xs * kLog2e : xs;
-    const double gb = FAST ? 1.0 : 0.5 * tauT * tauT; // background exponent per gap^2
-    const double gs = FAST ? -c.cS1 : omega;          // excitation exponent per gap
+    // skip tests, on the natural times in both forms: a tile is skipped when the smallest time
+    // gap alone drives every background exponent tauT^2 gap^2 / 2 past the negligible point or
+    // every excitation exponent omega gap past the cut
+    const double zb = ds.zero_arg_b;
+    const double zs = L.cut_s ? excitation_skip_arg(pk, D, n) : L.zero_arg_s;
+    const double gb = 0.5 * tauT * tauT; // background exponent per gap^2
+    const double gs = omega;             // excitation exponent per gap
     const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
-    double xi[A][D], ti[A], vm[A];
+    // tn: the rows' natural times (skip decisions, excitation terms); ti: their times in the
+    // form's units (base 2: relative to the block's first event)
+    double xi[A][D], ti[A], tn[A], vm[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
         const int r = wrow0 + a * 64 + lane;
         const int rc = min(r, n - 1);
 #pragma unroll
-        for (int d = 0; d < D; ++d) xi[a][d] = FAST ? (ex[(size_t)d * npad + rc] - x0[d]) * sx : ex[(size_t)d * npad + rc];
-        ti[a] = FAST ? (et[rc] - t0) * st : et[rc];
+        for (int d = 0; d < D; ++d) xi[a][d] = FAST ? scale_by(ex[(size_t)d * npad + rc] - x0[d], sx) : ex[(size_t)d * npad + rc];
+        tn[a] = et[rc];
+        ti[a] = FAST ? scale_by(tn[a] - t0, st) : tn[a];
         vm[a] = r < n ? 1.0 : 0.0;
     }
     const bool wave_live = wrow0 < n;
-    const double t_lo = FAST ? (et[min(wrow0, n - 1)] - t0) * st : et[min(wrow0, n - 1)];
+    const double t_lo = et[min(wrow0, n - 1)]; // the wave's earliest row (natural)
 
     double acc[A][6];
 #pragma unroll
@@ -437,15 +483,16 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double v = ex[(size_t)d * npad + j0 + tid];
-                    tile[tid * ST + d] = FAST ? (v - x0[d]) * sx : v;
+                    tile[tid * ST + d] = FAST ? scale_by(v - x0[d], sx) : v;
                 }
-                tile[tid * ST + D] = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
+                tile[tid * ST + D] = FAST ? scale_by(et[j0 + tid] - t0, st) : et[j0 + tid];
+                if (FAST) tile[tid * ST + D + 1] = et[j0 + tid];
             }
             __syncthreads();
             if (wave_live) {
 #pragma unroll
                 for (int a = 0; a < A; ++a)
-                    diag_pass<D, GRAD, F32, FAST, PM>(tile, cnt, xi[a], ti[a], c, acc[a]);
+                    diag_pass<D, GRAD, F32, FAST, PM>(tile, cnt, xi[a], ti[a], tn[a], c, acc[a]);
             }
         }
     } else {
@@ -457,7 +504,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         // flag of column tile t for this block: cf[t * nb]
         unsigned char *cf = L.colflag + ds.flag0 + (size_t)k * ds.flag_kstride + P;
         // the block's first row: the smallest time gap to any earlier tile is its gap
-        const double t_blk = FAST ? (et[min(P0, n - 1)] - t0) * st : et[min(P0, n - 1)];
+        const double t_blk = et[min(P0, n - 1)];
         // culled order: bounding boxes of the block's rows and of this wave's rows
         double blo[D + 1], bhi[D + 1], wlo[D + 1], whi[D + 1];
         const double hx = 0.5 * tauX * tauX, ht = 0.5 * tauT * tauT;
@@ -467,6 +514,16 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         }
         bool wrote = false; // a tile of the item passed the workgroup-level skip tests (uniform)
         const int box_tile0 = culled_groups(n), box_super0 = box_tile0 + culled_groups(n) * kGroup / kTile;
+        // the rows' sums over kFlushTiles computed tiles are formed apart (tacc) and then added to
+        // the item's (acc): a row's sum over a chunk of L columns grows its rounding as
+        // sqrt(kFlushTiles kTile) + sqrt(L / (kFlushTiles kTile)) rather than sqrt(L) (gradient
+        // components that cancel between events amplify it)
+        double tacc[A][6]; // FLUSH only
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) tacc[a][q] = 0.0;
+        int since = 0;
         for (int j0 = c0; j0 < c1; j0 += kTile) {
             // c1 - c0 and P0 are multiples of kTile: every tile here is full.
             if (PM == kPassCulledB && L.skip_zero) {
@@ -493,7 +550,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 // first wave (whose gap is the smallest): when neither term survives there, it
                 // survives for no row of the block; the tile's column partials are then zero
                 // (flagged, not written).
-                const double tl = FAST ? (et[j0 + kTile - 1] - t0) * st : et[j0 + kTile - 1];
+                const double tl = et[j0 + kTile - 1];
                 const double gap = t_blk - tl;
                 if (tl < t_blk && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs) {
                     if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 0;
@@ -502,25 +559,31 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             }
             __syncthreads();
             {
-                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as D + 1 arrays of 128
+                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as NCOMP arrays of 128
                 // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
-                const int e0 = (tid >> 6) * (D + 1) * 128 + (tid & 63);
+                const int e0 = (tid >> 6) * NCOMP * 128 + (tid & 63);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double u = ex[(size_t)d * npad + j0 + tid];
-                    const double v = FAST ? (u - x0[d]) * sx : u;
+                    const double v = FAST ? scale_by(u - x0[d], sx) : u;
                     tile[e0 + d * 128] = v;
                     tile[e0 + d * 128 + 64] = v;
                 }
-                const double tv = FAST ? (et[j0 + tid] - t0) * st : et[j0 + tid];
+                const double tnat = et[j0 + tid];
+                const double tv = FAST ? scale_by(tnat - t0, st) : tnat;
                 tile[e0 + D * 128] = tv;
                 tile[e0 + D * 128 + 64] = tv;
+                if (FAST) {
+                    tile[e0 + (D + 1) * 128] = tnat;
+                    tile[e0 + (D + 1) * 128 + 64] = tnat;
+                }
             }
             __syncthreads();
             if (wave_live) {
                 // time-sorted passes: columns are earlier than the rows, tl <= t_lo; equal times
                 // need the mask
-                const double tl = tile[(3 * (D + 1) + D) * 128 + 63];
+                // the tile's last column, natural time (staged beside the scaled one in base 2)
+                const double tl = tile[(3 * NCOMP + (FAST ? D + 1 : D)) * 128 + 63];
                 int smode = PM == kPassCulledB ? kSNone : (tl < t_lo ? kSFull : kSMixed);
                 int dob = 1;
                 if (PM != kPassCulledB && L.skip_zero) {
@@ -537,11 +600,18 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                         group_box<D>(ds.gstat, (j0 + sb * 64) / kGroup, 1, slo, shi);
                         if (box_background_exponent<D>(wlo, whi, slo, shi, hx, ht) >= ds.zero_arg_b) mode = 0;
                     }
-                    const double *sub = tile + sb * (D + 1) * 128;
-                    if (tail)
-                        sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, cs);
-                    else
-                        sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, vm, c, acc, cs);
+                    const double *sub = tile + sb * NCOMP * 128;
+                    if constexpr (FLUSH) {
+                        if (tail)
+                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, tacc, cs);
+                        else
+                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, tacc, cs);
+                    } else {
+                        if (tail)
+                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, acc, cs);
+                        else
+                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, acc, cs);
+                    }
                     // column lane of the subtile: its slots before and after the wrap; the slots
                     // are zeroed again for the next subtile (the wave's LDS operations take effect
                     // in program order)
@@ -568,6 +638,22 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             }
             if (tid == 0) cf[(size_t)(j0 / kTile) * ds.nb] = 1;
             wrote = true;
+            if (FLUSH && ++since == kFlushTiles) {
+                since = 0;
+#pragma unroll
+                for (int a = 0; a < A; ++a)
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        acc[a][q] += tacc[a][q];
+                        tacc[a][q] = 0.0;
+                    }
+            }
+        }
+        if (FLUSH) {
+#pragma unroll
+            for (int a = 0; a < A; ++a)
+#pragma unroll
+                for (int q = 0; q < 6; ++q) acc[a][q] += tacc[a][q];
         }
         // the row-slot flag: an item whose every tile was skipped leaves its (zero) row partials
         // unwritten
@@ -599,10 +685,12 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 template <int D, bool GRAD, int A, bool F32, int FM, int PM>
 __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(FM == 2 ? 2 : 3, 3))) void sym_kernel(
     SymLaunch L, const double *__restrict__ params, double *__restrict__ rowpart, double *__restrict__ colpart) {
-    constexpr int ST = TileStride<D>::value;
+    // the base-2 form (FM 1, or 2 for some datasets) stages the natural time beside the scaled one
+    constexpr int ST = DiagStride<D, FM != 0>::value;
+    constexpr int NCOMP = FM != 0 ? D + 2 : D + 1;
     constexpr int NC = GRAD ? 3 : 1;
-    // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x (D + 1) x 128 doubles
-    __shared__ double tile[(2 * kTile * (D + 1) > kTile * ST) ? 2 * kTile * (D + 1) : kTile * ST];
+    // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x NCOMP x 128 doubles
+    __shared__ double tile[(2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST];
     __shared__ double colbuf[kSymWaves * NC * kTile];
     __shared__ double colslot[kSymWaves * NC * 128];
     __shared__ int s_item;
@@ -622,10 +710,19 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         if (idx >= L.items) break;
         const int4 it = L.item[idx];
-        if (FM == 1 || (FM == 2 && L.data[it.x].fast))
-            sym_item<D, GRAD, A, F32, true, PM>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
-        else
-            sym_item<D, GRAD, A, F32, false, PM>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+        const bool fast = FM == 1 || (FM == 2 && L.data[it.x].fast);
+        const bool flush = it.z >= 0 && L.data[it.x].L > kFlushTiles * kTile;
+        if (fast) {
+            if (flush)
+                sym_item<D, GRAD, A, F32, true, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+            else
+                sym_item<D, GRAD, A, F32, true, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+        } else {
+            if (flush)
+                sym_item<D, GRAD, A, F32, false, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+            else
+                sym_item<D, GRAD, A, F32, false, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+        }
     }
 }
 

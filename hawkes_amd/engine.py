@@ -139,6 +139,13 @@ class Engine:
         check(_lib.lib().hph_get_last_evaluation(self._h, ctypes.byref(out)))
         return out.value
 
+    def event_sums(self):
+        """N x 6 per-event sums (sum B, B r^2, B dt^2, S, S r^2, S dt; natural units, caller's
+        order) of the last log-likelihood-and-gradient evaluation at the current parameters."""
+        out = np.zeros((self.locationCount, 6))
+        check(_lib.lib().hph_get_event_sums(self._h, dptr(out), out.size))
+        return out
+
     def last_pass(self):
         """(pair pass, base-2 form) of the last full evaluation: pass = _lib.PAIR_KERNEL_ROWWISE,
         PAIR_KERNEL_SYMMETRIC or PAIR_KERNEL_SPLIT; base-2 form True when the pass ran on scaled

@@ -110,7 +110,12 @@ int hph_get_internal_dimension(hph_engine *engine, int *out);
  * row-major; count must be K*6. The single-vector parameters set by hph_set_parameters are
  * not changed. */
 int hph_set_parameters_batch(hph_engine *engine, const double *params, size_t count);
-/* loglik[K]; grad (K x 6, may be NULL for log-likelihood only). */
+/* loglik[K]; grad (K x 6, may be NULL for log-likelihood only). When every vector has the same
+ * tauXprec and tauTprec (the scan set of the reference's sampler, R/Untitled.R:254-422: sigmaXprec,
+ * omega, theta, mu0), the background sums -- which depend on the data, tauXprec and tauTprec
+ * alone -- are formed once (or taken from the single-vector cache below) and each vector runs an
+ * excitation pass only; hph_get_last_evaluation then reports HPH_EVAL_EXCITATION, else
+ * HPH_EVAL_FULL. Dimensions 1..3, K >= 2, not with HPH_FLAG_ALL_PAIRS. */
 int hph_get_log_likelihood_batch(hph_engine *engine, double *loglik, double *grad, size_t K);
 
 /* ---- Kernel timing (extension; the reference's MICRO_BENCHMARK timers,
@@ -128,9 +133,11 @@ int hph_get_timing(hph_engine *engine, double *out, size_t length);
  * omega, theta and mu0 (the parameters the reference's random-scan sampler proposes,
  * R/Untitled.R:254-422) reuses the background sums and runs an excitation-only pass, or, when
  * only theta and mu0 changed, reuses both (O(N)). Data changes, tauXprec or tauTprec changes,
- * probability requests, batches and HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3.
+ * probability requests and HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3. A batch whose
+ * vectors share tauXprec and tauTprec reuses (or refills) the cached background sums too.
  * hph_get_last_evaluation: *kind = HPH_EVAL_FULL, HPH_EVAL_EXCITATION (cached background sums +
- * excitation pass) or HPH_EVAL_CACHED (cached sums only) for the last single-vector call. */
+ * excitation pass) or HPH_EVAL_CACHED (cached sums only) for the last single-vector call or
+ * batch. */
 #define HPH_EVAL_FULL 0
 #define HPH_EVAL_EXCITATION 1
 #define HPH_EVAL_CACHED 2
@@ -142,6 +149,15 @@ int hph_get_last_evaluation(hph_engine *engine, int *kind);
  * relative on the terms), 0 for the natural form. length must be >= 2; HPH_ERR_NOT_INITIALIZED
  * before the first evaluation. */
 int hph_get_last_pass(hph_engine *engine, int *out, size_t length);
+/* The per-event sums behind the last log-likelihood-and-gradient evaluation at the current
+ * parameters (dimensions 1..3): out is N x 6 row-major in the caller's order, length = 6 N, with
+ * sum_j B_ij, sum_j B_ij r_ij^2, sum_j B_ij dt_ij^2, sum_j S_ij, sum_j S_ij r_ij^2, sum_j S_ij dt_ij,
+ * B_ij = exp(-(tauX^2 r_ij^2 + tauT^2 dt_ij^2) / 2) over all j (j = i included) and
+ * S_ij = exp(-omega dt_ij - sigmaX^2 r_ij^2 / 2) over t_j < t_i (the rate is
+ * lambda_i = mu0 tauX^D tauT sum B / 2 pi + theta omega sigmaX^D sum S / sqrt(2 pi)), within the
+ * likelihood passes' negligible-term cuts. HPH_ERR_NOT_INITIALIZED when no such evaluation is
+ * at hand. */
+int hph_get_event_sums(hph_engine *engine, double *out, size_t length);
 
 /* ---- Pair-pass selection (extension) ----
  * AUTO: the tuned or built-in choice among the others (row-wise below ~12k events or above
@@ -191,8 +207,18 @@ int hph_batch_set_parameters(hph_batch *batch, const int *vectors_per_dataset, c
 int hph_batch_get_log_likelihood(hph_batch *batch, double *loglik, double *grad, size_t total_vectors);
 /* enabled != 0: capture the launch sequence in a HIP graph once per shape and replay it. */
 int hph_batch_set_graph(hph_batch *batch, int enabled);
-/* Work decomposition: rows per lane (1 or 2) and the target number of work items. */
+/* Work decomposition: rows per lane (1 or 2) and the target number of work items, fixed (overrides
+ * the tuned and the built-in choice: 2 rows per lane, 8192 items). */
 int hph_batch_set_config(hph_batch *batch, int rows_per_lane, int target_items);
+/* Times the work decompositions (1 or 2 rows per lane x six target item counts) on the current
+ * datasets and vectors, keeps the fastest for this table's shape (log2 of sum_d K_d N_d^2,
+ * dimension, total vectors, precision, dataset count) in the process-wide tuning cache shared with
+ * hph_autotune (and the HPH_TUNE_CACHE file), and clears any fixed configuration: later
+ * evaluations of a table of this shape use it. best_ms (may be NULL): the winner's device time. */
+int hph_batch_autotune(hph_batch *batch, int repeats, double *best_ms);
+/* The decomposition the next evaluation uses: out[0] rows per lane, out[1] target items,
+ * out[2] = 0 built-in, 1 tuned, 2 set by hph_batch_set_config; length >= 3. */
+int hph_batch_get_config(hph_batch *batch, int *out, size_t length);
 /* Device time (ms) of the last evaluation's launch sequence. */
 int hph_batch_get_timing(hph_batch *batch, double *out);
 /* *out = 1 when dataset d took the split passes in the last evaluation (data spread over several

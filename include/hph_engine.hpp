@@ -83,6 +83,9 @@ class HawkesEngine : public AbstractHawkes {
     // Pair pass of the last full evaluation: out[0] = HPH_PAIR_KERNEL_ROWWISE, _SYMMETRIC or
     // _SPLIT, out[1] = 1 when it ran the base-2 form (scaled coordinates and times), else 0.
     virtual void getLastPass(int *out, size_t length) = 0;
+    // Per-event sums (sum B, B r^2, B dt^2, S, S r^2, S dt; natural units, N x 6 row-major in the
+    // caller's order) of the last evaluation with the gradient at the current parameters.
+    virtual void getEventSums(double *out, size_t length) = 0;
     // d ll / d locations, N x D row-major in the caller's order; the reference declares a location
     // gradient for Java callers (src/jni/dr_inference_hawkes_NativeHPHSingleton.h:92-96) without an
     // implementation.

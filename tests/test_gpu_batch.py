@@ -199,3 +199,31 @@ def test_batch_errors():
         b.last_split(2)
     assert ex.value.code == _lib.HPH_ERR_INVALID_ARGUMENT
     assert not b.last_split(0)  # nothing evaluated yet
+
+
+def test_batch_autotune_picks_a_decomposition_for_the_shape():
+    """hph_batch_autotune times the work decompositions on the table and keeps the fastest for its
+    shape: the batch (and a second batch of the same shape) then report the tuned configuration,
+    results agree with the built-in decomposition's to rounding, and hph_batch_set_config
+    overrides the tuned choice."""
+    data = uneven_datasets(4, 2000, 12000, seed=41)
+    params = [synth.jitter_params(2, seed=820 + d) for d in range(len(data))]
+    b = make_batch(data, params)
+    assert b.config() == (2, 8192, 0)
+    ll0, g0 = b.log_likelihood()
+    ms = b.autotune(repeats=1)
+    assert ms > 0
+    A, target, source = b.config()
+    assert source == 1 and A in (1, 2) and target in (2048, 4096, 8192, 16384, 32768, 65536)
+    b.set_graph(True)
+    ll1, g1 = b.log_likelihood()
+    assert np.all(rel_err(ll1, ll0) <= 1e-13)
+    assert np.all(np.abs(g1 - g0) <= 1e-12 * np.maximum(np.abs(g0), 1.0))
+    b2 = make_batch(data, params)
+    assert b2.config() == (A, target, 1)
+    b2.set_config(1, 1024)
+    assert b2.config() == (1, 1024, 2)
+    ll2, _ = b2.log_likelihood()
+    assert np.all(rel_err(ll2, ll0) <= 1e-13)
+    b.close()
+    b2.close()

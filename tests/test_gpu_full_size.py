@@ -9,7 +9,10 @@ checked here), and the engine's own single-vector calls.
 Tolerances: against the oracle, ll 1e-12 relative and gradient component k within
 1e-12 * max(|g_k|, 1e-3 S_k) (S_k = the oracle's sum of |per-event terms|); batched against single
 calls (same kernels, a different work decomposition and so another summation order), ll 1e-14
-and gradient 1e-13 * max(|g_k|, 1e-3 max_k |g_k|).
+and gradient within the FP64 parity bar, 1e-12 * max(|g_k|, 1e-3 S_k) with S_k of P0: the
+gradient components are sums of cancelling per-event terms, and in the base-2 form the tauX
+component is formed as a difference of two moment sums, so their rounding (measured ~3e-15 S_k
+between decompositions) is relative to S_k, not to |g_k|.
 """
 import hashlib
 import json
@@ -46,6 +49,11 @@ def workload(fixture):
     for c in fixture["cluster_b70k"]:
         assert c["checksum"] == checksum(locs, times) and c["N"] == len(times)
     return locs, times, p0
+
+
+def single_err(gb, g, scale):
+    """Gradient difference of a batched and a single call, relative to max(|g_k|, 1e-3 S_k)."""
+    return float(np.max(np.abs(np.asarray(gb) - g) / np.maximum(np.abs(g), 1e-3 * np.asarray(scale))))
 
 
 def oracle_err(ll, g, ref):
@@ -94,10 +102,46 @@ def test_k64_batch_at_full_size(fixture, workload):
         ll, g = e.log_likelihood_and_gradient()
         assert e.last_evaluation() == _lib.EVAL_FULL
         worst_ll = max(worst_ll, rel_err(ll_b[k], ll))
-        worst_g = max(worst_g, float(np.max(np.abs(g_b[k] - g) / np.maximum(np.abs(g), 1e-3 * np.abs(g).max()))))
+        worst_g = max(worst_g, single_err(g_b[k], g, fixture["cluster_b70k"][0]["scale"][1:]))
     e.close()
     print("batch vs single calls: ll %.2e grad %.2e" % (worst_ll, worst_g), flush=True)
-    assert worst_ll <= 1e-14 and worst_g <= 1e-13
+    assert worst_ll <= 1e-14 and worst_g <= 1e-12
+
+
+def test_shared_tau_batch_at_full_size(fixture, workload):
+    """64 vectors sharing tauX and tauT (sigma, omega, theta, mu0 jittered: the benchmark's
+    shared-tau batch): one background pass and 64 excitation passes. Vectors 0 and 63 against the
+    oracle, every vector against a full single-vector evaluation (ll 1e-14, gradient 1e-12 as
+    above)."""
+    locs, times, _ = workload
+    S = synth.jitter_params(64, seed=671)
+    S[:, 1:3] = synth.P0[1:3]
+    e = Engine(2, len(times))
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters_batch(S)
+    ll_b, g_b = e.log_likelihood_batch()
+    assert e.last_evaluation() == _lib.EVAL_EXCITATION
+    e.close()
+    for c in fixture["cluster_b70k_shared_tau"]:
+        k = c["vector"]
+        assert np.array_equal(c["params"], S[k])
+        el, eg = oracle_err(ll_b[k], g_b[k], c)
+        print("shared-tau k=%d vs oracle: ll %.2e grad %.2e" % (k, el, eg.max()), flush=True)
+        assert el <= LL_TOL and eg.max() <= GRAD_TOL, (k, el, eg)
+    f = Engine(2, len(times))
+    f.set_times(times)
+    worst_ll, worst_g = 0.0, 0.0
+    for k in range(64):
+        f.update_locations(locs)  # a data change: the next call is a full pass
+        f.set_parameters(S[k])
+        ll, g = f.log_likelihood_and_gradient()
+        assert f.last_evaluation() == _lib.EVAL_FULL
+        worst_ll = max(worst_ll, rel_err(ll_b[k], ll))
+        worst_g = max(worst_g, single_err(g_b[k], g, fixture["cluster_b70k"][0]["scale"][1:]))
+    f.close()
+    print("shared-tau batch vs full single calls: ll %.2e grad %.2e" % (worst_ll, worst_g), flush=True)
+    assert worst_ll <= 1e-14 and worst_g <= 1e-12
 
 
 @pytest.fixture(scope="module")
@@ -109,14 +153,15 @@ def uneven():
 
 @pytest.mark.parametrize("flags", [_lib.OPENCL, _lib.OPENCL | _lib.FLOAT], ids=["fp64", "fp32_bulk"])
 def test_uneven_dataset_batch_at_full_size(fixture, uneven, flags):
-    """The 32-dataset batch (one graph-replayed launch sequence) against single engines for every
-    vector (FP64: ll 1e-13, gradient 1e-12 of max(|g_k|, 1e-3 max_k |g_k|): the single engines of
-    the smaller datasets take the row-wise pass in the natural form, the batch the symmetric pass
-    in the base-2 form), and vector 0 of every dataset with
-    N <= 50,000 against the oracle: 1e-12 in FP64, 1e-6 (ll and gradient) in FP32-bulk mode. The
-    FP32-bulk batch is compared with the FP64 single engines at 1e-6 as well."""
+    """The 32-dataset batch (one graph-replayed launch sequence): vector 0 of every dataset against
+    the oracle (1e-12 in FP64, 1e-6 for ll and gradient in FP32-bulk mode), and every vector
+    against a single engine's FP64 call (FP64: ll 1e-13, gradient 1e-12 of max(|g_k|, 1e-3 S_k)
+    with S_k of the dataset's vector 0 -- the single engines of the smaller datasets take the
+    row-wise pass in the natural form, the batch the symmetric pass in the base-2 form; FP32-bulk:
+    1e-6)."""
     data, params = uneven
     refs = {c["dataset"]: c for c in fixture["uneven_datasets"]}
+    assert sorted(refs) == list(range(32))
     b = Batch(2, [len(t) for _, t in data], flags=flags)
     for d, (l, t) in enumerate(data):
         b.set_data(d, l, t)
@@ -133,13 +178,12 @@ def test_uneven_dataset_batch_at_full_size(fixture, uneven, flags):
     worst = {"oracle_ll": 0.0, "oracle_g": 0.0, "single_ll": 0.0, "single_g": 0.0}
     row = 0
     for d, (l, t) in enumerate(data):
-        if d in refs:
-            c = refs[d]
-            assert c["checksum"] == checksum(l, t) and np.array_equal(c["params"], params[d][0])
-            el, eg = oracle_err(ll_b[row], g_b[row], c)
-            worst["oracle_ll"] = max(worst["oracle_ll"], el)
-            worst["oracle_g"] = max(worst["oracle_g"], float(eg.max()))
-            assert el <= tol_o and eg.max() <= tol_o, (d, el, eg)
+        c = refs[d]
+        assert c["checksum"] == checksum(l, t) and np.array_equal(c["params"], params[d][0])
+        el, eg = oracle_err(ll_b[row], g_b[row], c)
+        worst["oracle_ll"] = max(worst["oracle_ll"], el)
+        worst["oracle_g"] = max(worst["oracle_g"], float(eg.max()))
+        assert el <= tol_o and eg.max() <= tol_o, (d, el, eg)
         e = Engine(2, len(t))
         e.update_locations(l)
         e.set_times(t)
@@ -147,7 +191,7 @@ def test_uneven_dataset_batch_at_full_size(fixture, uneven, flags):
             e.set_parameters(params[d][k])
             ll, g = e.log_likelihood_and_gradient()
             sl = rel_err(ll_b[row + k], ll)
-            sg = float(np.max(np.abs(g_b[row + k] - g) / np.maximum(np.abs(g), 1e-3 * np.abs(g).max())))
+            sg = single_err(g_b[row + k], g, c["scale"][1:])
             worst["single_ll"] = max(worst["single_ll"], sl)
             worst["single_g"] = max(worst["single_g"], sg)
             assert sl <= tol_ll and sg <= tol_g, (d, k, sl, sg)

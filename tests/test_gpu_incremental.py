@@ -150,3 +150,41 @@ def test_large_chain_and_location_gradient():
     ref.close()
     e.close()
     assert np.all(np.abs(g - g_ref) <= 1e-13 * np.maximum(np.abs(g_ref), 1e-3 * np.abs(g_ref).max()))
+
+
+@pytest.mark.parametrize("config", ["auto", "symmetric", "rowwise_narrow", "split"])
+@pytest.mark.parametrize("flags", [_lib.OPENCL, _lib.OPENCL | _lib.FLOAT], ids=["fp64", "fp32_bulk"])
+def test_shared_tau_batch_matches_full_evaluations(config, flags):
+    """A batch whose vectors share tauX and tauT forms the background sums once (a full pass for
+    vector 0, or the single-vector cache) and one excitation pass per vector: each result equals a
+    fresh engine's full evaluation (ll 1e-14, gradient 1e-13 as above), with and without the
+    gradient, and the single-vector cache it refilled serves a later single call."""
+    import oracle
+    locs, times = synth.cluster_process(3000, side=15.0, seed=15)
+    P = synth.jitter_params(6, seed=16)
+    P[:, 1:3] = synth.P0[1:3]
+    e = engine(locs, times, synth.P0, CONFIGS[config], flags)
+    e.set_parameters_batch(P)
+    for grad in (True, False, True):
+        r = e.log_likelihood_batch(gradient=grad)
+        assert e.last_evaluation() == _lib.EVAL_EXCITATION
+        ll, g = (r if grad else (r, None))
+        for k in range(len(P)):
+            close((ll[k], g[k] if grad else None), fresh(locs, times, P[k], CONFIGS[config], flags, grad), grad)
+    if not flags & _lib.FLOAT:
+        ref = oracle.total(locs, times, P[2])
+        assert rel_err(ll[2], ref["ll"]) <= 1e-12
+        err = np.abs(g[2] - ref["grad"]) / np.maximum(np.abs(ref["grad"]), 1e-3 * ref["scale"][1:])
+        assert err.max() <= 1e-12, err
+    # the cache now holds vector 0's sums: a single call at P[0] reuses all of them
+    e.set_parameters(P[0])
+    close(e.log_likelihood_and_gradient(), fresh(locs, times, P[0], CONFIGS[config], flags, True), True)
+    assert e.last_evaluation() == _lib.EVAL_CACHED
+    # a batch with one tauX apart takes the K-vector pass
+    Q = P.copy()
+    Q[3, 1] *= 1.01
+    e.set_parameters_batch(Q)
+    ll, g = e.log_likelihood_batch()
+    assert e.last_evaluation() == _lib.EVAL_FULL
+    close((ll[3], g[3]), fresh(locs, times, Q[3], CONFIGS[config], flags, True), True)
+    e.close()

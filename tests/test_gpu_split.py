@@ -73,6 +73,32 @@ def test_skipped_groups_match_every_pair(A):
     np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-15, atol=1e-300)
 
 
+@pytest.mark.parametrize("A", [1, 2])
+def test_skipped_supertiles_against_every_pair_and_oracle(A):
+    """Target grid 64: chunks of at least 4,096 events, so the culled pass tests whole supertiles
+    (4,096 events) against a block's box and skips the far ones with one test. N ~ 20k over 250 km:
+    most supertiles are far. Every-pair evaluation (1e-15: ll, gradient of its scale,
+    probabilities) and the double oracle (1e-12)."""
+    locs, times = synth.cluster_process(14000, side=250.0, horizon=20_000.0, seed=83)
+    p = synth.P0
+    ref = oracle.total(locs, times, p, precision="double", probs=True)
+    scale = ref["scale"][1:]
+    out = []
+    for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
+        e = engine(locs, times, p, config=(_lib.PAIR_KERNEL_SPLIT, A, 64), flags=flags)
+        ll, g = e.log_likelihood_and_gradient()
+        assert e.last_pass()[0] == _lib.PAIR_KERNEL_SPLIT
+        out.append((ll, g, e.probs_self_excite()))
+        e.close()
+    print("N=%d ll %r" % (len(times), out[0][0]), flush=True)
+    assert rel_err(out[0][0], out[1][0]) <= 1e-15
+    assert np.all(np.abs(out[0][1] - out[1][1]) <= 1e-15 * scale)
+    np.testing.assert_allclose(out[0][2], out[1][2], rtol=1e-15, atol=1e-300)
+    assert rel_err(out[0][0], ref["ll"]) <= 1e-12
+    assert np.all(np.abs(out[0][1] - ref["grad"]) <= 1e-12 * np.maximum(np.abs(ref["grad"]), 1e-3 * scale))
+    np.testing.assert_allclose(out[0][2], ref["probs"], rtol=1e-12, atol=1e-300)
+
+
 @pytest.fixture(scope="module")
 def wide():
     locs, times = synth.cluster_process(105_000, side=100.0, horizon=17_520.0, seed=91)

@@ -3,34 +3,8 @@ import numpy as np
 import pytest
 from scipy.stats import halfnorm, truncnorm
 
-import oracle
+from conftest import OracleEngine
 from hawkes_amd import sampler, synth
-
-
-class OracleEngine:
-    """Engine-shaped adapter over the CPU oracle (double precision), for sampler tests."""
-
-    def __init__(self, locs, times):
-        self.locs, self.times = locs, times
-
-    def set_parameters(self, p):
-        self.p = np.asarray(p, dtype=np.float64)
-
-    def log_likelihood(self):
-        return oracle.total(self.locs, self.times, self.p, precision="double")["ll"]
-
-    def log_likelihood_and_gradient(self):
-        r = oracle.total(self.locs, self.times, self.p, precision="double")
-        return r["ll"], r["grad"]
-
-    def set_parameters_batch(self, P):
-        self.P = np.asarray(P, dtype=np.float64)
-
-    def log_likelihood_batch(self, gradient=True):
-        rs = [oracle.total(self.locs, self.times, q, precision="double") for q in self.P]
-        ll = np.array([r["ll"] for r in rs])
-        g = np.array([r["grad"] for r in rs])
-        return (ll, g) if gradient else ll
 
 
 @pytest.fixture(scope="module")
