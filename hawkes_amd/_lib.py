@@ -71,6 +71,8 @@ SIGNATURES = {
     "hph_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "hph_batch_get_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), _sz]),
     "hph_batch_last_split": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "hph_log_likelihood_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_long,
+                                                  _vp, ctypes.c_int, _vp]),
 }
 
 _lib = None

@@ -2066,6 +2066,15 @@ int hph_get_last_evaluation(hph_engine *e, int *kind) {
     return guarded([&] { *kind = e->impl->getLastEvaluation(); });
 }
 
+int hph_log_likelihood_noalloc(int embedding_dimension, int location_count, const double *locations,
+                               const double *times, const double *params, int K, long flags, double *result,
+                               int gradient, void *stream) {
+    return guarded([&] {
+        hph::noalloc::evaluate(embedding_dimension, location_count, locations, times, params, K, flags, result,
+                               gradient != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
 struct hph_batch {
     hph::HipHawkesBatch *impl;
 };

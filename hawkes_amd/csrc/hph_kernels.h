@@ -24,6 +24,7 @@ struct Events {
     int npad;          // n rounded up to kRowsPerBlock
     int dim;           // D
     double t_last;     // the caller's last time (T of the model)
+    const double *t_last_at = nullptr; // device copy of T, read instead of t_last when set
 };
 
 // Work decomposition of the pair pass.
@@ -250,4 +251,11 @@ void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *
 
 } // namespace kern
 
+namespace noalloc {
+// hph_log_likelihood_noalloc (hph_noalloc.hip): K vectors of one dataset from device-resident
+// inputs, every intermediate in the library's static scratch array; throws std::invalid_argument
+// for arguments and sizes the array cannot hold, std::runtime_error for device failures.
+void evaluate(int D, int N, const double *locations, const double *times, const double *params, int K, long flags,
+              double *result, bool grad, hipStream_t stream);
+} // namespace noalloc
 } // namespace hph

@@ -265,7 +265,8 @@ __global__ __launch_bounds__(kRowsPerBlock) void finalize_kernel(Events ev, cons
                 if (GRAD || q == 0 || q == 3) sums[(size_t)q * npad + i] = s[q];
         }
         double lam;
-        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], ev.t_last, v, lam);
+        const double T = ev.t_last_at != nullptr ? *ev.t_last_at : ev.t_last;
+        const double p = row_terms<GRAD>(s, params + 6 * k, ev.dim, ev.t[i], T, v, lam);
         if (probs != nullptr && k == 0) probs[ev.perm[i]] = p;
         if (inv_lam != nullptr && k == 0) inv_lam[i] = 1.0 / lam;
     }

@@ -226,6 +226,31 @@ int hph_batch_get_timing(hph_batch *batch, double *out);
  * engine's built-in choice; the other datasets share one time-sorted pass), else 0. */
 int hph_batch_last_split(hph_batch *batch, int dataset, int *out);
 
+/* ---- Allocation-free evaluation (extension) ----
+ * No counterpart in the reference: for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The log-likelihood
+ * (and gradient) of K parameter vectors of one dataset, as hph_get_log_likelihood_batch returns
+ * them, from inputs and an output that are already in device memory (pointers valid on the
+ * current HIP device):
+ *   locations: N x D row-major (the reference's layout, src/NewHawkes.hpp:244-276);
+ *   times: N values in ascending order (T = the last); params: K x 6, in the order of
+ *   hph_set_parameters; result: K x 7 (ll, then d ll / d params when gradient != 0; without the
+ *   gradient only the first value of each 7 is written).
+ * flags: HPH_FLAG_FLOAT and HPH_FLAG_ALL_PAIRS as in hph_create. The work is enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) and the call returns without waiting. Every intermediate
+ * lives in one static device array of HPH_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; the call allocates nothing (the first call on a device looks up that array's address;
+ * make it outside any graph capture). Limit: the events' copy plus the row-wise pass's partial
+ * sums, about 8 (D + 1 + 6 K c) N bytes with c >= 1 column chunks, must fit in the array (with the
+ * gradient, K x N up to about 1.3e6); larger requests return HPH_ERR_INVALID_ARGUMENT.
+ * Non-finite inputs or times out of order give NaN in every result (checked on the device).
+ * All calls on one GPU share the array, which keeps the last call's contents: calls that overlap
+ * in time on one GPU race -- serialise them (one stream, or events between streams). */
+#define HPH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hph_log_likelihood_noalloc(int embedding_dimension, int location_count, const double *locations,
+                               const double *times, const double *params, int K, long flags, double *result,
+                               int gradient, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
