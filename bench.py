@@ -14,6 +14,8 @@ Prints one JSON line. Options: --steps K --warmup W --gpus N (only 1 is supporte
 --large 0|1 (the N ~ 1e6 leg) --legs (extra legs: all_pairs, fp32, small, locgrad, mh = the
 adaptive MH sampler on re-evaluated proposals) --config (a fixed pair-pass configuration instead
 of the autotuned one) --only (run the listed legs alone, e.g. for a profile of one of them).
+Each leg's timed region is a roctx range "bench:<leg>"; --trace-summary DIR [--bench-json FILE]
+reads a `rocprofv3 --kernel-trace --marker-trace` of a run and prints its kernels by leg.
 
 value counts the N^2 ordered pairs the reference forms per call; the engine forms each
 background pair once for both events and skips tiles whose background terms are all below
@@ -161,6 +163,92 @@ def load_traffic(kind):
     return rec.get("bytes_per_launch") if rec.get("kernel", "").startswith(kind + "_kernel") else None
 
 
+class _Ranges:
+    """roctx ranges named "bench:<leg>" around each leg's timed region, so that a
+    `rocprofv3 --kernel-trace --marker-trace` of a whole run assigns every kernel dispatch to the
+    leg that timed it (see trace_summary). A no-op where the roctx library cannot be loaded."""
+
+    def __init__(self):
+        import ctypes
+        self._lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1", "libroctx64.so"):
+            try:
+                lib = ctypes.CDLL(name)
+                lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+                lib.roctxRangePushA.restype = ctypes.c_int
+                lib.roctxRangePop.restype = ctypes.c_int
+                self._lib = lib
+                break
+            except (OSError, AttributeError):
+                continue
+
+    def __call__(self, leg):
+        ranges = self
+
+        class _Range:
+            def __enter__(self):
+                if ranges._lib is not None:
+                    ranges._lib.roctxRangePushA(("bench:" + leg).encode())
+
+            def __exit__(self, *exc):
+                if ranges._lib is not None:
+                    ranges._lib.roctxRangePop()
+        return _Range()
+
+
+RANGE = _Ranges()
+
+
+def trace_summary(trace_dir, bench_json):
+    """Per-leg kernel statistics from a rocprofv3 kernel + marker trace of a bench.py run (the
+    dispatches that started inside each "bench:<leg>" range), and the headline roofline fraction
+    recomputed from the traced average duration of the headline's pair-pass kernel with the FLOP per
+    launch that the same run printed (bench_json: that run's JSON line)."""
+    import csv
+    import glob
+    ktrace = sorted(glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True))
+    mtrace = sorted(glob.glob(os.path.join(trace_dir, "**", "*marker_api_trace.csv"), recursive=True))
+    if not ktrace or not mtrace:
+        sys.exit("trace_summary: need *kernel_trace.csv and *marker_api_trace.csv under " + trace_dir)
+    ranges = []
+    for path in mtrace:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                label = next((v for v in r.values() if isinstance(v, str) and v.startswith("bench:")), None)
+                if label is not None:
+                    ranges.append((label[6:], int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    disp = []
+    for path in ktrace:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                disp.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    disp.sort()
+    out = {"trace": trace_dir, "legs": {}}
+    for leg, s, e in ranges:
+        per = {}
+        for a, b, name in disp:
+            if s <= a <= e:
+                per.setdefault(name, []).append((b - a) / 1e3)
+        kernels = sorted(per.items(), key=lambda kv: -sum(kv[1]))
+        out["legs"][leg] = [{"kernel": name.split("(")[0], "calls": len(us), "avg_us": float(np.mean(us)),
+                             "min_us": float(np.min(us)), "max_us": float(np.max(us)),
+                             "total_ms": float(np.sum(us)) / 1e3} for name, us in kernels]
+    if bench_json:
+        with open(bench_json) as f:
+            rec = json.loads(f.read().strip().splitlines()[-1])
+        roof = rec.get("roofline") or {}
+        head = out["legs"].get("headline")
+        if head and roof.get("flop_per_launch"):
+            k = head[0]
+            traced = roof["flop_per_launch"] / (k["avg_us"] * 1e-6) / 1e12
+            out["headline_roofline"] = {
+                "kernel": k["kernel"], "calls": k["calls"], "traced_avg_us": k["avg_us"],
+                "in_run_hip_event_avg_us": 1e3 * roof["kernel_ms"], "flop_per_launch": roof["flop_per_launch"],
+                "traced_tflops": traced, "traced_frac": traced / FP64_VECTOR_PEAK_TFLOPS,
+                "bench_frac": roof["frac"], "ms_per_step": rec.get("ms_per_step")}
+    print(json.dumps(out, indent=1))
+
+
 def host_cpu():
     """Model name and logical CPU count of this host (recorded beside every CPU number)."""
     model = "unknown"
@@ -265,11 +353,12 @@ def dataset_batch(count):
         tuned_ms = b.autotune(repeats=1)
         b.set_graph(True)
         b.log_likelihood()  # captures the graph
-        t0 = time.perf_counter()
         reps = 2
-        for _ in range(reps):
-            ll, g = b.log_likelihood()
-        wall = (time.perf_counter() - t0) / reps
+        with RANGE("datasets_" + name):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ll, g = b.log_likelihood()
+            wall = (time.perf_counter() - t0) / reps
         res[name] = (ll, g)
         A, target, source = b.config()
         rec = {"value": pairs / wall, "ms_per_launch": 1e3 * wall, "device_ms": b.timing(),
@@ -321,16 +410,18 @@ def large_dataset(n_leapfrog=20):
         e.set_parameters(pw)
         e.set_timing(True)
         e.log_likelihood_and_gradient()
-        t0 = time.perf_counter()
-        e.set_parameters(p)
-        ll, g = e.log_likelihood_and_gradient()
-        t_llg = time.perf_counter() - t0
+        with RANGE("large_" + name):
+            t0 = time.perf_counter()
+            e.set_parameters(p)
+            ll, g = e.log_likelihood_and_gradient()
+            t_llg = time.perf_counter() - t0
         assert e.last_evaluation() == _lib.EVAL_FULL
         # probabilities: a log-likelihood pass that skips exact zeros only (a likelihood pass's own
         # excitation cut is bounded relative to the rates, not to tiny probabilities)
-        t0 = time.perf_counter()
-        probs = e.probs_self_excite()
-        t_probs = time.perf_counter() - t0
+        with RANGE("large_probs_" + name):
+            t0 = time.perf_counter()
+            probs = e.probs_self_excite()
+            t_probs = time.perf_counter() - t0
         res[name] = (ll, g, probs)
         out[name] = {"ms_loglik_grad": 1e3 * t_llg, "ms_probs_pass": 1e3 * t_probs, "value": float(n) * n / t_llg,
                      "pair_pass": kernel_name(e.tuned_config())}
@@ -450,10 +541,11 @@ def batched_leg(e, K, reps, n, plain_value):
     P = synth.jitter_params(K, seed=668)
     e.set_parameters_batch(P)
     e.log_likelihood_batch()
-    tb = time.perf_counter()
-    for _ in range(reps):
-        e.log_likelihood_batch()
-    bt = (time.perf_counter() - tb) / reps
+    with RANGE("batched"):
+        tb = time.perf_counter()
+        for _ in range(reps):
+            e.log_likelihood_batch()
+        bt = (time.perf_counter() - tb) / reps
     bvalue = K * n * n / bt
     out = {"K": K, "value": bvalue, "ms_per_batch": 1e3 * bt, "evaluation": e.last_evaluation()}
     if plain_value:
@@ -474,17 +566,19 @@ def shared_tau_leg(e, K, reps, n, plain_value):
     e.set_parameters_batch(P)
     e.log_likelihood_batch()
     assert e.last_evaluation() == _lib.EVAL_EXCITATION
-    tb = time.perf_counter()
-    for _ in range(reps):
-        e.log_likelihood_batch()
-    bt = (time.perf_counter() - tb) / reps
+    with RANGE("shared_tau"):
+        tb = time.perf_counter()
+        for _ in range(reps):
+            e.log_likelihood_batch()
+        bt = (time.perf_counter() - tb) / reps
     e.set_parameters(P[-1])
     e.log_likelihood_and_gradient()
-    ts = time.perf_counter()
-    for k in range(K):
-        e.set_parameters(P[k])
-        e.log_likelihood_and_gradient()
-    st = time.perf_counter() - ts
+    with RANGE("shared_tau_single_calls"):
+        ts = time.perf_counter()
+        for k in range(K):
+            e.set_parameters(P[k])
+            e.log_likelihood_and_gradient()
+        st = time.perf_counter() - ts
     bvalue = K * n * n / bt
     out = {"K": K, "value": bvalue, "ms_per_batch": 1e3 * bt, "ms_sequential_single_calls": 1e3 * st,
            "speedup_vs_sequential_single_calls": st / bt}
@@ -524,10 +618,17 @@ def main():
     ap.add_argument("--only", default="all",
                     help="comma list of legs to run, for profiling one of them alone: " + ", ".join(LEGS) +
                          " (default all: the headline and every leg the other options enable)")
+    ap.add_argument("--trace-summary", default=None, metavar="DIR",
+                    help="no run: summarise the rocprofv3 kernel + marker trace under DIR of a bench.py run by "
+                         "leg (with --bench-json: that run's JSON line, for the headline roofline)")
+    ap.add_argument("--bench-json", default=None)
     ap.add_argument("--config", default="auto",
                     help="pair-pass configuration: 'auto' (autotuned) or kind,rows_per_lane,target_grid with "
                          "kind sym or rowwise, e.g. sym,2,8192 (no autotuning; for profiling one kernel)")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.bench_json)
+        return
     legs = set() if args.legs == "none" else set(args.legs.split(","))
     only = None if args.only == "all" else set(args.only.split(","))
     if only is not None and not only <= set(LEGS):
@@ -562,22 +663,23 @@ def main():
             e.autotune(repeats=2)        # picks the pair-pass configuration for this shape
         return e
 
-    def run(flags, steps):
+    def run(flags, steps, leg):
         e = engine(flags)
         for s in range(args.warmup):
             e.set_parameters(params[s])
             e.log_likelihood_and_gradient()
         pair_ms, fin_ms, results, kinds, passes = [], [], [], [], []
-        t0 = time.perf_counter()
-        for s in range(steps):
-            e.set_parameters(params[args.warmup + s])
-            results.append(e.log_likelihood_and_gradient())
-            a, b = e.timing()
-            pair_ms.append(a)
-            fin_ms.append(b)
-            kinds.append(e.last_evaluation())
-            passes.append(e.last_pass())
-        wall = time.perf_counter() - t0
+        with RANGE(leg):
+            t0 = time.perf_counter()
+            for s in range(steps):
+                e.set_parameters(params[args.warmup + s])
+                results.append(e.log_likelihood_and_gradient())
+                a, b = e.timing()
+                pair_ms.append(a)
+                fin_ms.append(b)
+                kinds.append(e.last_evaluation())
+                passes.append(e.last_pass())
+            wall = time.perf_counter() - t0
         # every step jitters all six parameters: each one is a full pass, none a re-evaluation
         assert all(k == _lib.EVAL_FULL for k in kinds), kinds
         assert len(set(passes)) == 1, passes
@@ -601,7 +703,7 @@ def main():
     value = None
     fp64_results = None
     if want("headline"):
-        e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps)
+        e, wall, pair_ms, fin_ms, formed = run(_lib.OPENCL, args.steps, "headline")
         fp64_results = run.results
         out["ms_per_step"] = 1e3 * wall / args.steps
         value = n * n / (wall / args.steps)
@@ -619,6 +721,7 @@ def main():
             "pass": {"kind": kind, "base2_form": base2},
             "kernel_ms": pair_avg_ms,
             "kernel_ms_min_max": [float(np.min(pair_ms)), float(np.max(pair_ms))],
+            "kernel_ms_per_step": [round(float(m), 4) for m in pair_ms],
             "finalize_reduce_ms": float(np.mean(fin_ms)),
             "timing": "HIP events around the pair pass on the engine's stream, averaged over the timed steps",
             "every_pair_flop": every_pair_flop,
@@ -626,7 +729,9 @@ def main():
         }
         if all(f is not None for f in formed):
             flops = [flop_model(b, x) for b, x in formed]
-            achieved = float(np.mean([f / (ms * 1e-3) for f, ms in zip(flops, pair_ms)])) / 1e12
+            # mean FLOP per launch over the mean launch duration: what a profile's average
+            # duration of the same dispatches reproduces
+            achieved = float(np.mean(flops)) / (pair_avg_ms * 1e-3) / 1e12
             nb = float(np.mean([b for b, _ in formed]))
             nx = float(np.mean([x for _, x in formed]))
             roofline.update({
@@ -643,7 +748,7 @@ def main():
 
     # Every pair formed (no tile skipping): the same results to rounding, more work.
     if want("all_pairs", "all_pairs" in legs):
-        ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4))
+        ea, wall_a, pair_a, _, formed_a = run(_lib.OPENCL | _lib.ALL_PAIRS, max(3, args.steps // 4), "all_pairs")
         steps_a = len(pair_a)
         rec = {"value": n * n / (wall_a / steps_a), "ms_per_step": 1e3 * wall_a / steps_a,
                "kernel_ms": float(np.mean(pair_a)), "kernel": kernel_name(ea.tuned_config())}
@@ -656,7 +761,7 @@ def main():
     # FP32-bulk mode: the background exponential in FP32, everything else FP64; error against the
     # FP64 results above (when they ran).
     if want("fp32", "fp32" in legs):
-        ef, wall_f, pair_f, _, formed_f = run(_lib.OPENCL | _lib.FLOAT, args.steps)
+        ef, wall_f, pair_f, _, formed_f = run(_lib.OPENCL | _lib.FLOAT, args.steps, "fp32")
         rec = {"value": n * n / (wall_f / args.steps), "ms_per_step": 1e3 * wall_f / args.steps,
                "kernel_ms": float(np.mean(pair_f)), "kernel": kernel_name(ef.tuned_config())}
         if formed_f[0] is not None:
@@ -685,16 +790,18 @@ def main():
         e.location_gradient()
         reps = 3
         cold, cold_ms = [], []
-        for r in range(reps):
-            e.set_parameters(params[r])
+        with RANGE("locgrad"):
+            for r in range(reps):
+                e.set_parameters(params[r])
+                t0 = time.perf_counter()
+                e.location_gradient()
+                cold.append(time.perf_counter() - t0)
+                cold_ms.append(e.timing())
+        with RANGE("locgrad_rates_reused"):
             t0 = time.perf_counter()
-            e.location_gradient()
-            cold.append(time.perf_counter() - t0)
-            cold_ms.append(e.timing())
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            e.location_gradient()
-        warm = (time.perf_counter() - t0) / reps
+            for _ in range(reps):
+                e.location_gradient()
+            warm = (time.perf_counter() - t0) / reps
         lt = float(np.mean(cold))
         out["location_gradient"] = {
             "ms_per_call": 1e3 * lt, "rate_pass_ms": float(np.mean([a for a, _ in cold_ms])),
