@@ -297,6 +297,7 @@ class SymRunner {
         for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_,
                           &d_flag_})
             b->release();
+        h_items_.release();
     }
 
     // Plans (if the shape changed) and enqueues the pass, finalize and reduction on stream s.
@@ -304,14 +305,26 @@ class SymRunner {
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     // sums (may be null): the per-event moments of dataset 0, vector 0 (natural units, [6][npad]).
+    // host_t, host_params (both or neither): each dataset's time-sorted times and the parameter
+    // vectors on the host; with them the work queue is ordered by each item's expected cost for
+    // these parameters (order_queue) and copied to the device queue before the pass.
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
                  bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, int cut_s,
                  const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
                  hipStream_t s, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr) {
+                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr,
+                 const std::vector<const double *> *host_t = nullptr, const double *host_params = nullptr) {
         if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
         kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
         L.cut_s = skip_zero && cut_s ? 1 : 0;
+        if (host_t != nullptr && host_params != nullptr) {
+            // (a call without host data runs whichever permutation of the plan's items the device
+            // queue holds: the order is a matter of time only)
+            order_queue(*host_t, host_params, dim, skip_zero, L.cut_s);
+            check(hipMemcpyAsync(d_items_.ptr, h_items_.ptr, sizeof(int4) * plan_.items.size(), hipMemcpyHostToDevice,
+                                 s),
+                  "upload work queue order");
+        }
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
         check(hipGetLastError(), "symmetric pair kernel launch");
@@ -412,6 +425,105 @@ class SymRunner {
     }
 
     const kern::SymPlan &plan() const { return plan_; }
+
+    // Orders the current plan's work queue (prepare has run for this shape) by each item's
+    // expected cost under the given parameters, heaviest first, into the pinned staging buffer
+    // that enqueue copies to the device. Items are independent (each writes its own partial-sum
+    // slots, which the finalize adds in a fixed order), so the order changes the pass's time and
+    // not its results. Plan order alone (by nominal size: chunk rows x columns) puts blocks whose
+    // earlier columns are all within the background's reach -- every tile computed -- among
+    // blocks whose chunks are mostly skipped; the queue's tail then waits on the former.
+    // Cost of a chunk item, in (block rows x kTile)-pair tiles: the tiles that pass the
+    // workgroup-wide skip test of sym_item (the same test, on the block's first time), weighted
+    // 21 where the block's first row also forms excitation terms and 10 where it forms the
+    // background only; skipped tiles count 0.1 (a flag write). Diagonal items: rows^2 pairs of both
+    // terms. Both tests are monotone in the tile index (times are sorted), so each (dataset,
+    // vector, block) takes two binary searches; the order is a counting sort of the quantised
+    // costs (stable: equal costs keep the plan order).
+    void order_queue(const std::vector<const double *> &ht, const double *hparams, int dim, int skip_zero, int cut_s) {
+        const size_t m = plan_.items.size();
+        if (ht.size() != plan_.data.size()) throw std::invalid_argument("order_queue: one time array per dataset");
+        h_items_.ensure(sizeof(int4) * m);
+        const int R = plan_.R;
+        const int T = kern::kTile;
+        // first tile of each (dataset, vector, block) that is not skipped workgroup-wide, and the
+        // first with excitation terms for the block's first row
+        std::vector<size_t> base(plan_.data.size());
+        size_t nthr = 0;
+        for (size_t d = 0; d < plan_.data.size(); ++d) {
+            base[d] = nthr;
+            nthr += (size_t)plan_.data[d].K * plan_.data[d].nb;
+        }
+        thr_.resize(2 * nthr);
+        for (size_t d = 0; d < plan_.data.size(); ++d) {
+            const kern::SymData &sd = plan_.data[d];
+            const double *t = ht[d];
+            for (int k = 0; k < sd.K; ++k) {
+                const double *pk = hparams + 6 * (size_t)(sd.param0 + k);
+                const double gb = 0.5 * pk[2] * pk[2], gs = pk[3];
+                const double zb = sd.zero_arg_b;
+                const double zs = cut_s ? kern::excitation_skip_arg(pk, dim, sd.n) : kern::kZeroExpArg;
+                for (int P = 0; P < sd.nb; ++P) {
+                    const double tb = t[std::min(P * R, sd.n - 1)];
+                    const int J = P * R / T; // earlier tiles of block P
+                    auto gap_of = [&](int j) { return tb - t[j * T + T - 1]; };
+                    auto skipped = [&](int j) {
+                        const double tl = t[j * T + T - 1], gap = tb - tl;
+                        return skip_zero && tl < tb && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs;
+                    };
+                    auto no_exc = [&](int j) {
+                        const double gap = gap_of(j);
+                        return skip_zero && gap > 0.0 && gs >= 0.0 && gs * gap >= zs;
+                    };
+                    int lo = 0, hi = J; // first j in [0, J] with !skipped(j)
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (skipped(mid)) lo = mid + 1; else hi = mid;
+                    }
+                    const size_t q = base[d] + (size_t)k * sd.nb + P;
+                    thr_[2 * q] = lo;
+                    lo = 0;
+                    hi = J;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (no_exc(mid)) lo = mid + 1; else hi = mid;
+                    }
+                    thr_[2 * q + 1] = lo;
+                }
+            }
+        }
+        cost_.resize(m);
+        double cmax = 0.0;
+        for (size_t i = 0; i < m; ++i) {
+            const int4 it = plan_.items[i];
+            const kern::SymData &sd = plan_.data[it.x];
+            double c;
+            if (it.z < 0) {
+                const double rows = (double)std::min(R, sd.n - it.y * R);
+                c = 21.0 * rows * rows / ((double)R * T);
+            } else {
+                const size_t q = base[it.x] + (size_t)it.w * sd.nb + it.y;
+                const int a = it.z * sd.L / T, b = std::min((it.z + 1) * sd.L, it.y * R) / T;
+                const int j1 = std::min(std::max(thr_[2 * q], a), b);      // first computed tile
+                const int j2 = std::min(std::max(thr_[2 * q + 1], j1), b); // first with excitation
+                c = 0.1 * (j1 - a) + 10.0 * (j2 - j1) + 21.0 * (b - j2);
+            }
+            cost_[i] = c;
+            cmax = std::max(cmax, c);
+        }
+        int nbuck = 1024;
+        while (nbuck < 65536 && (size_t)nbuck < m) nbuck *= 2;
+        count_.assign(nbuck + 1, 0);
+        key_.resize(m);
+        for (size_t i = 0; i < m; ++i) {
+            const int key = cmax > 0.0 ? (int)(cost_[i] / cmax * (nbuck - 1) + 0.5) : 0;
+            key_[i] = nbuck - 1 - key; // ascending key = descending cost
+            ++count_[key_[i] + 1];
+        }
+        for (int b = 0; b < nbuck; ++b) count_[b + 1] += count_[b];
+        int4 *out = h_items_.as<int4>();
+        for (size_t i = 0; i < m; ++i) out[count_[key_[i]]++] = plan_.items[i];
+    }
 
   private:
     // Plans (if the shape changed), uploads the tables (if they changed) and fills the launch.
@@ -515,6 +627,10 @@ class SymRunner {
     kern::SymData b_data_{};
     bool b_valid_ = false;
     DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_, d_flag_;
+    // work-queue order (order_queue): pinned staging buffer and scratch
+    HostBuf h_items_;
+    std::vector<int> thr_, count_, key_;
+    std::vector<double> cost_;
 };
 
 // Pair-pass configuration chosen by the autotuner.
@@ -990,6 +1106,7 @@ class HipHawkes : public HawkesEngine {
         check(hipMemcpyAsync(d_perm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_), "upload perm");
         check(hipStreamSynchronize(stream_), "upload sync");
         extents_ = extents_of(locs_, ht, (size_t)N, D);
+        tsorted_.assign(ht, ht + N);
         data_dirty_ = false;
         culled_valid_ = false;
         cache_valid_ = false;
@@ -1144,10 +1261,12 @@ class HipHawkes : public HawkesEngine {
                                    timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
                                    timing_ ? ev_[2] : nullptr, sums);
             } else {
+                const std::vector<const double *> host_t{tsorted_.data()};
                 sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
                              cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
                              probs_dev, w_dev, d_result_.as<double>(), stream_, timing_ ? ev_[0] : nullptr,
-                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr, sums);
+                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr, sums, &host_t,
+                             h_params_.as<double>());
             }
             last_split_ = split;
             last_kind_ = split ? HPH_PAIR_KERNEL_SPLIT : HPH_PAIR_KERNEL_SYMMETRIC;
@@ -1366,6 +1485,7 @@ class HipHawkes : public HawkesEngine {
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
     Extents extents_;
+    std::vector<double> tsorted_; // the times in ascending order (host), for the work-queue order
 
     int device_;
     bool fp32_;
@@ -1442,6 +1562,7 @@ class HipHawkesBatch {
         locs_.resize(nd);
         times_.resize(nd);
         sorted_.resize(nd);
+        tsorted_.resize(nd);
         split_.resize(nd);
         ext_.resize(nd);
         have_.assign(nd, false);
@@ -1494,6 +1615,7 @@ class HipHawkesBatch {
         }
         ext_[d] = extents_of(locs_[d], ht.data(), (size_t)n, D_);
         sorted_[d] = perm;
+        tsorted_[d].assign(ht.begin(), ht.begin() + n);
         if (split_[d]) split_[d]->valid = false;
         // x planes of dataset d live at d_x_ + D * off_[d]: plane q at + q * npad_[d]
         check(hipMemcpy(d_x_.as<double>() + (size_t)D_ * off_[d], hx.data(), sizeof(double) * hx.size(),
@@ -1653,6 +1775,12 @@ class HipHawkesBatch {
                 graph_ = g;
                 check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
             }
+            // the work queue's order for these parameters, into the staging buffer the graph copies
+            std::vector<const double *> host_t;
+            for (int d = 0; d < nd; ++d)
+                if (!split[d]) host_t.push_back(tsorted_[d].data());
+            if (!host_t.empty())
+                runner_.order_queue(host_t, h_params_.as<double>(), D_, (flags_ & ALL_PAIRS) ? 0 : 1, 1);
             check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
             check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
@@ -1789,9 +1917,15 @@ class HipHawkesBatch {
                 K.push_back(K_[d]);
                 f.push_back(fast[d]);
             }
-        if (!refs.empty())
+        if (!refs.empty()) {
+            // the queue's order for this call's parameters (a captured graph copies the staging
+            // buffer that evaluate refills before each replay)
+            std::vector<const double *> host_t;
+            for (size_t d = 0; d < counts_.size(); ++d)
+                if (!split[d]) host_t.push_back(tsorted_[d].data());
             runner_.enqueue(refs, K, D_, params, grad_, fp32_, A_, target_, skip_zero, 1, f, nullptr, nullptr, result,
-                            stream_);
+                            stream_, nullptr, nullptr, nullptr, nullptr, &host_t, h_params_.as<double>());
+        }
         for (size_t d = 0; d < counts_.size(); ++d)
             if (split[d]) {
                 SplitData &sd = *split_[d];
@@ -1831,6 +1965,7 @@ class HipHawkesBatch {
     std::vector<int> npad_;
     std::vector<std::vector<double>> locs_, times_;
     std::vector<std::vector<int>> sorted_; // per dataset: time-sorted position -> caller's index
+    std::vector<std::vector<double>> tsorted_; // per dataset: its times in ascending order
     std::vector<Extents> ext_;
     std::vector<bool> have_;
     std::vector<int> K_;

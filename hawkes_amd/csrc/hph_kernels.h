@@ -135,7 +135,9 @@ void launch_locgrad(const Events &ev, const double *params, const double *w, con
 // The pass runs over a table of datasets (one for an engine, many for a batch); every dataset
 // has its own events, parameter vectors and slices of the partial buffers. Work items
 // (dataset, row block, chunk of earlier columns | -1 for the diagonal block, parameter vector)
-// are sorted heaviest first and taken from a queue by a persistent grid.
+// are taken from a queue by a persistent grid, heaviest first: the plan orders them by nominal
+// size, and a caller with the times and parameters on the host reorders them by expected cost
+// (the engine's SymRunner::order_queue). Any order gives the same results.
 
 // One dataset of a symmetric pass (device table entry).
 struct SymData {
@@ -167,7 +169,7 @@ struct SymData {
 
 struct SymLaunch {
     const SymData *data;    // device table
-    const int4 *item;       // (dataset, block, chunk | -1, k), heaviest first
+    const int4 *item;       // (dataset, block, chunk | -1, k), heaviest first (any order is correct)
     int items;
     int *counter;           // work-queue counter, zeroed by launch_sym before the pass
     int grid;               // persistent workgroups launched
