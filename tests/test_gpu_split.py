@@ -8,7 +8,8 @@ terms) on data spread over a domain much wider than the background kernel's reac
     holds only background terms below e^-(50 + ln N);
   * at N ~ 1.5e5 on 100 km: the built-in choice picks the split passes, which agree with the
     symmetric pass (ll 1e-13, gradient 1e-12), FP32-bulk within 1e-6 of FP64, K-batches equal
-    to single calls to rounding.
+    to single calls to rounding;
+  * at N ~ 1e6, a K = 8 batch stays on the split passes and equals its single calls.
 """
 import numpy as np
 import pytest
@@ -153,6 +154,27 @@ def test_fp32_bulk_and_batches(wide):
     assert rel_err(ll32, ll64) <= 1e-6
     assert np.all(np.abs(g32 - g64) <= 1e-6 * np.maximum(np.abs(g64), 1e-3 * np.abs(g64).max()))
     assert np.max(np.abs(p32 - p64)) <= 1e-6
+
+
+def test_k8_batch_at_n1e6_stays_on_the_split_passes():
+    """K = 8 vectors of the N ~ 1e6 workload (cluster_b700k, the bench's large leg) in one batch:
+    the column partials of the background pass (~23 GB per vector, allocated whole) fit the
+    device, so the batch runs the split passes, and each vector equals its single call to
+    rounding (measured bitwise)."""
+    locs, times, p = synth.workload("cluster_b700k")
+    P = synth.jitter_params(8, seed=668)
+    print("N=%d K=8 batch on the split passes" % len(times), flush=True)
+    e = engine(locs, times, p, config=None)
+    e.set_parameters_batch(P)
+    ll_b, g_b = e.log_likelihood_batch()
+    assert e.last_pass()[0] == _lib.PAIR_KERNEL_SPLIT
+    for k in range(8):
+        e.set_parameters(P[k])
+        ll, g = e.log_likelihood_and_gradient()
+        assert e.last_pass()[0] == _lib.PAIR_KERNEL_SPLIT
+        assert rel_err(ll, ll_b[k]) <= 1e-14
+        assert np.all(np.abs(g - g_b[k]) <= 1e-13 * np.maximum(np.abs(g), 1e-3 * np.abs(g).max()))
+    e.close()
 
 
 @pytest.mark.parametrize("case", ["one_location", "one_time", "two_clusters"])
