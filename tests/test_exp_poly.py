@@ -104,3 +104,69 @@ def test_exp2_neg_fp64():
     assert exp2_neg_fp64(c, 0.0) == 1.0  # the self pair's term is exactly 1
     assert exp2_neg_fp64(c, 3.0) == 0.125
     assert exp2_neg_fp64(c, 1100.0) == 0.0
+
+
+# ---- double-double constants of the base-2 form (hph_device.h) ----
+
+DEVICE_H = os.path.join(HERE, "..", "hawkes_amd", "csrc", "hph_device.h")
+
+
+def device_constants():
+    """The `constexpr double kName = value` constants of hph_device.h."""
+    c = {}
+    with open(DEVICE_H) as f:
+        text = f.read()
+    for m in re.finditer(r"\b(k\w+)\s*=\s*([-+0-9.eE]+)\s*[,;]", text):
+        c[m.group(1)] = float(m.group(2))
+    return c
+
+
+DD_VALUES = {
+    "kSqrtHalfLog2e": lambda: mp.sqrt(mp.log(mp.e, 2) / 2),
+    "kTwoLn2": lambda: 2 * mp.log(2),
+    "kLog2e": lambda: 1 / mp.log(2),
+}
+
+
+def test_double_double_constants():
+    """Each (Hi, Lo) pair is the value rounded to a double plus the rounded remainder, so that
+    Hi + Lo carries it to ~2^-106 relative; and 2 ln 2 is 1 / (sqrt(log2(e) / 2))^2 (what the
+    finalize uses to undo the scaling of squared coordinates and times)."""
+    c = device_constants()
+    for name, value in DD_VALUES.items():
+        v = value()
+        hi, lo = c[name + "Hi"], c[name + "Lo"]
+        assert hi == float(v), name
+        assert lo == float(v - mp.mpf(hi)), name
+        assert abs(mp.mpf(hi) + mp.mpf(lo) - v) <= mp.mpf(2) ** -104 * abs(v), name
+    assert abs(DD_VALUES["kTwoLn2"]() * DD_VALUES["kSqrtHalfLog2e"]() ** 2 - 1) < mp.mpf(10) ** -50
+
+
+def time_scale_dd(c, tau):
+    """hph_device.h time_scale_dd: |tau| sqrt(log2(e)/2) as a double-double (two-product)."""
+    a = abs(tau)
+    hi = float(mp.mpf(a) * mp.mpf(c["kSqrtHalfLog2eHi"]))
+    err = fma(a, c["kSqrtHalfLog2eHi"], -hi)
+    return hi, fma(a, c["kSqrtHalfLog2eLo"], err)
+
+
+def test_scaling_round_trip():
+    """A squared difference scaled by the double-double st and brought back by from_base2_sq
+    (m 2 ln 2 / tau^2) returns the natural square to a few ulp: no systematic bias from a rounded
+    scale (which is what a single rounded st would leave on every term of a dataset)."""
+    c = device_constants()
+    rng = np.random.default_rng(11)
+    worst_scale = worst_back = 0.0
+    for tau in np.concatenate([rng.uniform(1e-4, 10.0, 200), [1 / 336.0, 1 / 1.6, 1.0]]):
+        hi, lo = time_scale_dd(c, float(tau))
+        exact = abs(mp.mpf(float(tau))) * DD_VALUES["kSqrtHalfLog2e"]()
+        worst_scale = max(worst_scale, float(abs(mp.mpf(hi) + mp.mpf(lo) - exact) / exact))
+        d = float(rng.uniform(0.1, 100.0))
+        # scale_by: fma(v, s.x, v * s.y)
+        vs = fma(d, hi, float(mp.mpf(d) * mp.mpf(lo)))
+        m = float(mp.mpf(vs) * mp.mpf(vs))
+        tau2 = float(mp.mpf(float(tau)) * mp.mpf(float(tau)))
+        back = float(mp.mpf(fma(m, c["kTwoLn2Hi"], float(mp.mpf(m) * mp.mpf(c["kTwoLn2Lo"])))) / mp.mpf(tau2))
+        worst_back = max(worst_back, float(abs(mp.mpf(back) - mp.mpf(d) ** 2) / mp.mpf(d) ** 2))
+    assert worst_scale < 1e-30, worst_scale
+    assert worst_back < 8 * 2.0 ** -53, worst_back
