@@ -342,12 +342,21 @@ class SymRunner {
     // its per-event sums into bsum (device, K x (grad ? 3 : 1) x npad), the excitation sums per
     // event of the time-sorted order into sband (same shape), then the finalize and the
     // reduction. Arguments otherwise as enqueue; ev1 is recorded after both passes.
+    // host_gstat, host_params (both or neither): the culled order's box table and the parameter
+    // vectors on the host, for the background pass's queue order (order_queue_culled).
     void enqueue_split(const DataRef &sorted, const DataRef &culled, const double *gstat, int K, int dim,
                        const double *params, bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero,
                        int cut_s, bool fast, double *bsum, double *sband, double *probs_dev, double *inv_lam_dev,
-                       double *result, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, double *sums) {
+                       double *result, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, double *sums,
+                       const double *host_gstat = nullptr, const double *host_params = nullptr) {
         kern::SymLaunch L =
             prepare({sorted}, {K}, dim, grad, fp32, rows_per_lane, target_items, skip_zero, {fast ? (char)1 : (char)0});
+        if (host_gstat != nullptr && host_params != nullptr && skip_zero) {
+            order_queue_culled(host_gstat, host_params, dim);
+            check(hipMemcpyAsync(d_items_.ptr, h_items_.ptr, sizeof(int4) * plan_.items.size(), hipMemcpyHostToDevice,
+                                 s),
+                  "upload work queue order");
+        }
         // the background pass's table: the sorted table with the culled order's arrays
         kern::SymData b = plan_.data[0];
         b.x = culled.x;
@@ -525,6 +534,112 @@ class SymRunner {
         for (size_t i = 0; i < m; ++i) out[count_[key_[i]]++] = plan_.items[i];
     }
 
+    // Forgets the culled pass's order (its box table changed).
+    void reset_culled_order() { culled_order_.clear(); }
+
+    // The same for the split passes' background pass over the culled order of dataset 0 (one
+    // dataset, K vectors): there the kernel skips a (block, supertile of kSuperTile columns)
+    // pair, else a (block, tile) pair, when the two bounding boxes put every background exponent
+    // tauX^2 r^2 / 2 + tauT^2 dt^2 / 2 at or past the skip point (box_background_exponent). The
+    // cost of a chunk item counts its tiles that pass these tests (10 each; skipped tiles 0.1,
+    // skipped supertiles 0.2), with the same box tests on the host copy of the box table. The tests
+    // take about 1e6 box pairs per vector at N = 1e6, so the order is kept while every vector's
+    // tauX and tauT stay within 2% of the ones it was computed for (sampler steps change them
+    // by far less), and recomputed otherwise.
+    void order_queue_culled(const double *gstat, const double *hparams, int dim) {
+        const size_t m = plan_.items.size();
+        const kern::SymData &sd = plan_.data[0];
+        // the cached order is a permutation of the items it was computed for: reusable while the
+        // plan lists the same items (a gradient and a probability request plan the same items)
+        bool reuse = culled_order_.size() == m && culled_tau_.size() == 2 * (size_t)sd.K &&
+                     culled_items_.size() == m &&
+                     std::memcmp(culled_items_.data(), plan_.items.data(), sizeof(int4) * m) == 0;
+        for (int k = 0; reuse && k < sd.K; ++k) {
+            const double *pk = hparams + 6 * (size_t)(sd.param0 + k);
+            reuse = std::fabs(pk[1] / culled_tau_[2 * k] - 1.0) <= 0.02 &&
+                    std::fabs(pk[2] / culled_tau_[2 * k + 1] - 1.0) <= 0.02;
+        }
+        if (!reuse) {
+            const int R = plan_.R, T = kern::kTile, S = kern::kSuperTile, G = kern::kGroup;
+            const int GS = 2 * dim + 2;
+            const int box_tile0 = kern::culled_groups(sd.n);
+            const int box_super0 = box_tile0 + kern::culled_groups(sd.n) * G / T;
+            // merged box of `count` consecutive table entries from g
+            auto box = [&](int g, int count, double *lo, double *hi) {
+                for (int d = 0; d <= dim; ++d) {
+                    lo[d] = gstat[(size_t)g * GS + d];
+                    hi[d] = gstat[(size_t)g * GS + dim + 1 + d];
+                }
+                for (int q = 1; q < count; ++q)
+                    for (int d = 0; d <= dim; ++d) {
+                        lo[d] = std::fmin(lo[d], gstat[(size_t)(g + q) * GS + d]);
+                        hi[d] = std::fmax(hi[d], gstat[(size_t)(g + q) * GS + dim + 1 + d]);
+                    }
+            };
+            auto far = [&](const double *alo, const double *ahi, const double *blo, const double *bhi, double hx,
+                           double ht) {
+                double r2 = 0.0;
+                for (int d = 0; d < dim; ++d) {
+                    const double g = std::fmax(0.0, std::fmax(blo[d] - ahi[d], alo[d] - bhi[d]));
+                    r2 = std::fma(g, g, r2);
+                }
+                const double gt = std::fmax(0.0, std::fmax(blo[dim] - ahi[dim], alo[dim] - bhi[dim]));
+                return hx * r2 + ht * gt * gt >= sd.zero_arg_b;
+            };
+            cost_.assign(m, 0.0);
+            double blo[kern::kMaxDim + 1], bhi[kern::kMaxDim + 1], tlo[kern::kMaxDim + 1], thi[kern::kMaxDim + 1];
+            for (size_t i = 0; i < m; ++i) {
+                const int4 it = plan_.items[i];
+                const int P = it.y, P0 = P * R;
+                if (it.z < 0) {
+                    const double rows = (double)std::min(R, sd.n - P0);
+                    cost_[i] = 10.0 * rows * rows / ((double)R * T);
+                    continue;
+                }
+                const double *pk = hparams + 6 * (size_t)(sd.param0 + it.w);
+                const double hx = 0.5 * pk[1] * pk[1], ht = 0.5 * pk[2] * pk[2];
+                box(P0 / G, R / G, blo, bhi);
+                const int c0 = it.z * sd.L, c1 = std::min(c0 + sd.L, P0);
+                double c = 0.0;
+                for (int j0 = c0; j0 < c1; j0 += T) {
+                    if (j0 % S == 0 && j0 + S <= c1) {
+                        box(box_super0 + j0 / S, 1, tlo, thi);
+                        if (far(blo, bhi, tlo, thi, hx, ht)) {
+                            c += 0.2;
+                            j0 += S - T;
+                            continue;
+                        }
+                    }
+                    box(box_tile0 + j0 / T, 1, tlo, thi);
+                    c += far(blo, bhi, tlo, thi, hx, ht) ? 0.1 : 10.0;
+                }
+                cost_[i] = c;
+            }
+            double cmax = 0.0;
+            for (double c : cost_) cmax = std::max(cmax, c);
+            int nbuck = 1024;
+            while (nbuck < 65536 && (size_t)nbuck < m) nbuck *= 2;
+            count_.assign(nbuck + 1, 0);
+            key_.resize(m);
+            for (size_t i = 0; i < m; ++i) {
+                const int key = cmax > 0.0 ? (int)(cost_[i] / cmax * (nbuck - 1) + 0.5) : 0;
+                key_[i] = nbuck - 1 - key;
+                ++count_[key_[i] + 1];
+            }
+            for (int b = 0; b < nbuck; ++b) count_[b + 1] += count_[b];
+            culled_order_.resize(m);
+            for (size_t i = 0; i < m; ++i) culled_order_[count_[key_[i]]++] = plan_.items[i];
+            culled_items_ = plan_.items;
+            culled_tau_.resize(2 * (size_t)sd.K);
+            for (int k = 0; k < sd.K; ++k) {
+                culled_tau_[2 * k] = hparams[6 * (size_t)(sd.param0 + k) + 1];
+                culled_tau_[2 * k + 1] = hparams[6 * (size_t)(sd.param0 + k) + 2];
+            }
+        }
+        h_items_.ensure(sizeof(int4) * m);
+        std::memcpy(h_items_.ptr, culled_order_.data(), sizeof(int4) * m);
+    }
+
   private:
     // Plans (if the shape changed), uploads the tables (if they changed) and fills the launch.
     // fast[d]: dataset d runs the base-2 form.
@@ -631,6 +746,10 @@ class SymRunner {
     HostBuf h_items_;
     std::vector<int> thr_, count_, key_;
     std::vector<double> cost_;
+    // order_queue_culled: the last order, the plan items and the (tauX, tauT) per vector it was
+    // computed for
+    std::vector<int4> culled_order_, culled_items_;
+    std::vector<double> culled_tau_;
 };
 
 // Pair-pass configuration chosen by the autotuner.
@@ -1126,6 +1245,8 @@ class HipHawkes : public HawkesEngine {
               "upload culled order");
         check(hipMemcpy(d_gstat_.ptr, c.gstat.data(), sizeof(double) * c.gstat.size(), hipMemcpyHostToDevice),
               "upload group boxes");
+        gstat_host_ = c.gstat;
+        sym_.reset_culled_order();
         culled_valid_ = true;
     }
 
@@ -1259,7 +1380,7 @@ class HipHawkes : public HawkesEngine {
                                    last_fast_, d_bsum_.as<double>(), d_sband_.as<double>(), probs_dev, w_dev,
                                    d_result_.as<double>(), stream_,
                                    timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
-                                   timing_ ? ev_[2] : nullptr, sums);
+                                   timing_ ? ev_[2] : nullptr, sums, gstat_host_.data(), h_params_.as<double>());
             } else {
                 const std::vector<const double *> host_t{tsorted_.data()};
                 sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
@@ -1486,6 +1607,7 @@ class HipHawkes : public HawkesEngine {
     SymRunner sym_;
     Extents extents_;
     std::vector<double> tsorted_; // the times in ascending order (host), for the work-queue order
+    std::vector<double> gstat_host_; // the culled order's box table (host), for its queue order
 
     int device_;
     bool fp32_;
