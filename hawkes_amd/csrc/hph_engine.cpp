@@ -1903,6 +1903,11 @@ class HipHawkesBatch {
                 if (!split[d]) host_t.push_back(tsorted_[d].data());
             if (!host_t.empty())
                 runner_.order_queue(host_t, h_params_.as<double>(), D_, (flags_ & ALL_PAIRS) ? 0 : 1, 1);
+            if (!(flags_ & ALL_PAIRS))
+                for (int d = 0; d < nd; ++d)
+                    if (split[d])
+                        split_[d]->runner.order_queue_culled(split_[d]->gstat_host.data(),
+                                                             h_params_.as<double>() + 6 * (size_t)slot_[d], D_);
             check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
             check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
             check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
@@ -2012,6 +2017,8 @@ class HipHawkesBatch {
                   "upload culled order");
             check(hipMemcpy(sd.gstat.ptr, c.gstat.data(), sizeof(double) * c.gstat.size(), hipMemcpyHostToDevice),
                   "upload group boxes");
+            sd.gstat_host = c.gstat;
+            sd.runner.reset_culled_order();
             sd.valid = true;
             drop_graph();
         }
@@ -2057,7 +2064,8 @@ class HipHawkesBatch {
                                         grad_, fp32_, A_, target_, skip_zero, 1, fast[d] != 0, sd.bsum.as<double>(),
                                         sd.sband.as<double>(), nullptr, nullptr,
                                         result + kern::kResultWidth * (size_t)slot_[d], stream_, nullptr, nullptr,
-                                        nullptr, nullptr);
+                                        nullptr, nullptr, sd.gstat_host.data(),
+                                        h_params_.as<double>() + 6 * (size_t)slot_[d]);
             }
         if (events) check(hipEventRecord(ev_[2], stream_), "hipEventRecord");
         check(hipMemcpyAsync(h_result_.ptr, d_result_.ptr, sizeof(double) * kern::kResultWidth * (size_t)total_,
@@ -2099,6 +2107,7 @@ class HipHawkesBatch {
     // a dataset of the split passes: its culled order, runner and per-event sums
     struct SplitData {
         DevBuf x, t, perm, gstat, bsum, sband;
+        std::vector<double> gstat_host; // the box table on the host (its queue order)
         bool valid = false;
         SymRunner runner;
     };
