@@ -394,10 +394,14 @@ class SymRunner {
     // background moments from bsum ([3 or 1][npad]), excitation moments from scache ([3 or
     // 1][npad]); the moments used are written to sums_out ([6][npad]). Arguments otherwise as
     // enqueue; ev2 is recorded after the reduction.
+    // plan_grad (-1: grad): the gradient flag of the plan to use (a finalize from sums reads no
+    // partial sums, so it runs on a plan made for either; reusing the current one saves re-planning).
     void enqueue_from_sums(const DataRef &ref, int dim, const double *params, bool grad, bool fp32, int rows_per_lane,
                            int target_items, const double *bsum, const double *scache, double *sums_out,
-                           double *probs_dev, double *inv_lam_dev, double *result, hipStream_t s, hipEvent_t ev2) {
-        const kern::SymLaunch L = prepare({ref}, {1}, dim, grad, fp32, rows_per_lane, target_items, 1, {0});
+                           double *probs_dev, double *inv_lam_dev, double *result, hipStream_t s, hipEvent_t ev2,
+                           int plan_grad = -1) {
+        const kern::SymLaunch L =
+            prepare({ref}, {1}, dim, plan_grad < 0 ? grad : plan_grad != 0, fp32, rows_per_lane, target_items, 1, {0});
         kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, scache, sums_out,
                                   d_block_.as<double>(), probs_dev, inv_lam_dev, s);
         check(hipGetLastError(), "finalize kernel launch");
@@ -939,6 +943,7 @@ class HipHawkes : public HawkesEngine {
                   "download probs");
             return;
         }
+        if (probs_from_cache(result)) return;
         evaluate_single(false, result);
     }
 
@@ -1494,6 +1499,61 @@ class HipHawkes : public HawkesEngine {
         if (!grad) cache_grad_ = false; // only moments 0 and 3 were written
         std::memcpy(cache_params_, params_, sizeof(params_));
         if (!reuse_s) cache_xs_ = xs;
+        return true;
+    }
+
+    // A probability request at the tauX and tauT of the cached per-event sums (kept by the last
+    // full or re-evaluated log-likelihood call): their background moments are the numbers a
+    // probability pass would form (both passes cut the background at the same point), so only the
+    // excitation moments are formed anew, by the band kernel skipping exact zeros alone (the
+    // probability passes' cut), then the finalize from the sums writes the probabilities. The
+    // cached sums, the cached log-likelihood and 1/lambda are left as they were (a later
+    // log-likelihood call re-evaluates from the same sums as before). Dimensions 1..3, not with
+    // HPH_FLAG_ALL_PAIRS; returns false when no such sums are at hand (the caller then runs the
+    // full probability pass).
+    bool probs_from_cache(double *probs_host) {
+        if (!have_params_ || !cache_valid_ || data_dirty_ || embeddingDimension > 3 || (flags & ALL_PAIRS) ||
+            force_natural_)
+            return false;
+        if (params_[1] != cache_params_[1] || params_[2] != cache_params_[2]) return false;
+        const bool plan_grad = cache_grad_; // the plan of the cached evaluation: no re-planning
+        TuneConfig cfg = config_for(1, plan_grad);
+        if (cfg.kind != HPH_PAIR_KERNEL_SYMMETRIC && cfg.kind != HPH_PAIR_KERNEL_SPLIT) {
+            cfg.kind = HPH_PAIR_KERNEL_SYMMETRIC;
+            cfg.rows_per_lane = locationCount < kSymA1MaxEvents ? 1 : 2;
+            cfg.target_blocks = 4096;
+        }
+        if (!sym_fits(1, cfg, plan_grad)) return false;
+        check(hipSetDevice(device_), "hipSetDevice");
+        d_params_.ensure(sizeof(double) * 6);
+        const int fin_blocks = (locationCount + kern::kRowsPerBlock - 1) / kern::kRowsPerBlock;
+        d_block_.ensure(sizeof(double) * (size_t)fin_blocks * kern::kResultWidth * 2);
+        d_result_.ensure(sizeof(double) * kern::kResultWidth);
+        h_params_.ensure(sizeof(double) * 6);
+        d_probs_.ensure(sizeof(double) * (size_t)locationCount);
+        d_sband_.ensure(sizeof(double) * 3 * (size_t)npad_);
+        d_sums_[1 - sums_cur_].ensure(sizeof(double) * 6 * (size_t)npad_);
+        std::memcpy(h_params_.ptr, params_, sizeof(params_));
+        check(hipMemcpyAsync(d_params_.ptr, h_params_.ptr, sizeof(double) * 6, hipMemcpyHostToDevice, stream_),
+              "upload params");
+        const kern::Events ev = events();
+        SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
+        if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+        // probabilities need the sums S and B only: the band kernel and the finalize without moments
+        kern::launch_excitation_band(ev, d_params_.as<double>(), 1, false, 1, d_sband_.as<double>(), stream_);
+        check(hipGetLastError(), "excitation pass launch");
+        if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+        // the moments written go to the spare sums buffer, which the cache does not point at
+        sym_.enqueue_from_sums(ref, embeddingDimension, d_params_.as<double>(), false, fp32_, cfg.rows_per_lane,
+                               cfg.target_blocks, d_sums_[sums_cur_].as<double>(), d_sband_.as<double>(),
+                               d_sums_[1 - sums_cur_].as<double>(), d_probs_.as<double>(), nullptr,
+                               d_result_.as<double>(), stream_, timing_ ? ev_[2] : nullptr, plan_grad ? 1 : 0);
+        check(hipMemcpyAsync(probs_host, d_probs_.ptr, sizeof(double) * locationCount, hipMemcpyDeviceToHost, stream_),
+              "download probs");
+        check(hipStreamSynchronize(stream_), "stream sync");
+        if (timing_) record_timing();
+        probs_valid_ = true;
+        last_eval_ = HPH_EVAL_EXCITATION;
         return true;
     }
 

@@ -6,12 +6,15 @@ full evaluation at the same parameters: ll to 1e-14 relative, gradient component
 max(|g_k|, 1e-3 max|g|). Background sums reused across the chain are the ones a full pass forms
 for the unchanged tauX and tauT; excitation sums come from an excitation-only pass or, for theta
 and mu0 moves, are reused within their skip bound (every rate moves by under 4e-22 relative).
-A change of tauX, tauT or of the data, a probability request and HPH_FLAG_ALL_PAIRS take the full
-pass.
+A change of tauX, tauT or of the data and HPH_FLAG_ALL_PAIRS take the full pass. A probability
+request at the cached tauX, tauT reuses the background sums and forms the excitation sums anew
+(exact zeros skipped only); compared with a fresh engine's full probability pass to 1e-13 relative
+(the excitation sums are added in another order) and with the oracle to 1e-12.
 """
 import numpy as np
 import pytest
 
+import oracle
 from conftest import rel_err
 from hawkes_amd import Engine, _lib, sampler, synth
 
@@ -116,9 +119,50 @@ def test_other_changes_take_the_full_pass():
     e.set_parameters(p4)
     close(e.log_likelihood_and_gradient(), fresh(l2, times, p4, None, _lib.OPENCL, True), True)
     assert e.last_evaluation() == _lib.EVAL_FULL
-    # probabilities after a chain step equal a fresh engine's
+    # probabilities after a chain step equal a fresh engine's (from the cached background sums
+    # here, a full probability pass there)
     ref = engine(l2, times, p4)
-    np.testing.assert_allclose(e.probs_self_excite(), ref.probs_self_excite(), rtol=1e-15, atol=1e-300)
+    np.testing.assert_allclose(e.probs_self_excite(), ref.probs_self_excite(), rtol=1e-13, atol=1e-300)
+    assert e.last_evaluation() == _lib.EVAL_EXCITATION
+    ref.close()
+    e.close()
+
+
+@pytest.mark.parametrize("config", ["auto", "symmetric", "split"])
+@pytest.mark.parametrize("flags", [_lib.OPENCL, _lib.OPENCL | _lib.FLOAT])
+def test_probabilities_from_cached_sums(config, flags):
+    """A probability request after a likelihood call at the same tauX, tauT: background sums from
+    the cache, excitation sums from the band kernel (exact zeros skipped only); the cache, the
+    log-likelihood and 1/lambda are left as they were."""
+    locs, times = synth.cluster_process(3000, side=np.sqrt(3000 / 100) * (3.0 if config == "split" else 1.0),
+                                        seed=31)
+    p = np.array([1.05, 0.6, 1 / 320.0, 0.95, 0.9, 1.1])
+    e = engine(locs, times, p, CONFIGS[config], flags)
+    ll1 = e.log_likelihood_and_gradient()
+    probs = e.probs_self_excite()
+    assert e.last_evaluation() == _lib.EVAL_EXCITATION
+    ref = engine(locs, times, p, CONFIGS[config], flags)
+    full = ref.probs_self_excite()  # no cache: the full probability pass
+    assert ref.last_evaluation() == _lib.EVAL_FULL
+    ref.close()
+    np.testing.assert_allclose(probs, full, rtol=1e-13, atol=1e-300)
+    if flags == _lib.OPENCL:
+        o = oracle.total(locs, times, p, probs=True)
+        np.testing.assert_allclose(probs, o["probs"], rtol=1e-12, atol=1e-300)
+    # the likelihood cache is untouched: the same call re-evaluates to the same bits
+    e.set_parameters(p)
+    ll2 = e.log_likelihood_and_gradient()
+    assert e.last_evaluation() == _lib.EVAL_CACHED
+    assert ll1[0] == ll2[0] and np.array_equal(ll1[1], ll2[1])
+    # after a sampler move the request uses the moved sigma, omega
+    q = p.copy()
+    q[0] *= 1.1
+    q[3] *= 0.9
+    e.set_parameters(q)
+    e.log_likelihood()
+    pq = e.probs_self_excite()
+    ref = engine(locs, times, q, CONFIGS[config], flags)
+    np.testing.assert_allclose(pq, ref.probs_self_excite(), rtol=1e-13, atol=1e-300)
     ref.close()
     e.close()
 

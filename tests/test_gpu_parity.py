@@ -157,8 +157,16 @@ def test_tile_skip_matches_all_pairs(name, params, kernel):
     out = []
     for flags in (_lib.OPENCL, _lib.OPENCL | _lib.ALL_PAIRS):
         e = make_engine(locs, times, params, kernel=kernel, flags=flags)
+        probs = e.probs_self_excite()  # first: a full probability pass on both engines
         ll, g = e.log_likelihood_and_gradient()
-        out.append((ll, g, e.probs_self_excite()))
+        if flags == _lib.OPENCL:
+            # a later request at the cached sums: background sums of the likelihood pass, excitation
+            # sums of the band kernel (another summation order)
+            e.set_parameters(params)
+            e.log_likelihood_and_gradient()
+            np.testing.assert_allclose(e.probs_self_excite(), probs, rtol=1e-13, atol=0, err_msg=name)
+            assert e.last_evaluation() == _lib.EVAL_EXCITATION
+        out.append((ll, g, probs))
         e.close()
     assert rel_err(out[0][0], out[1][0]) <= 1e-15, name
     assert np.all(np.abs(out[0][1] - out[1][1]) <= 1e-15 * scale), (name, out[0][1] - out[1][1])
@@ -171,7 +179,7 @@ def test_probs_from_likelihood_pass_equal_fresh_pass(kernel):
     e = make_engine(locs, times, synth.P0, kernel=kernel)
     e.log_likelihood_and_gradient()
     cached = e.probs_self_excite()
-    e.set_parameters(synth.P0)  # invalidates: the next call runs a likelihood-only pass
+    e.set_parameters(synth.P0)  # invalidates: the next request forms the probabilities again
     fresh = e.probs_self_excite()
     assert np.array_equal(cached, fresh)
     P = synth.jitter_params(3)
