@@ -132,9 +132,11 @@ int hph_get_timing(hph_engine *engine, double *out, size_t length);
  * excitation sums of its parameters. A later call whose parameters differ only in sigmaXprec,
  * omega, theta and mu0 (the parameters the reference's random-scan sampler proposes,
  * R/Untitled.R:254-422) reuses the background sums and runs an excitation-only pass, or, when
- * only theta and mu0 changed, reuses both (O(N)). Data changes, tauXprec or tauTprec changes,
- * probability requests and HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3. A batch whose
- * vectors share tauXprec and tauTprec reuses (or refills) the cached background sums too.
+ * only theta and mu0 changed, reuses both (O(N)). Data changes, tauXprec or tauTprec changes and
+ * HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3. A probability request at the cached
+ * tauXprec and tauTprec reuses the background sums and forms the excitation sums anew, skipping
+ * exact zeros only (it leaves the cache as it was). A batch whose vectors share tauXprec and
+ * tauTprec reuses (or refills) the cached background sums too.
  * hph_get_last_evaluation: *kind = HPH_EVAL_FULL, HPH_EVAL_EXCITATION (cached background sums +
  * excitation pass) or HPH_EVAL_CACHED (cached sums only) for the last single-vector call or
  * batch. */
