@@ -379,7 +379,9 @@ class SymRunner {
         check(hipGetLastError(), "background sums launch");
         // the excitation terms of the time-sorted events, per event (FP64 in both precision modes)
         kern::Events ev{sorted.x, sorted.t, sorted.perm, sorted.n, sorted.npad, dim, sorted.t_last};
-        kern::launch_excitation_band(ev, params, K, grad, !skip_zero ? 2 : (cut_s ? 0 : 1), sband, s);
+        // cut_s 0 (a probability request, or the rates of the location gradient): exact zeros only,
+        // with the per-event relative stop where no moments are formed
+        kern::launch_excitation_band(ev, params, K, grad, !skip_zero ? 2 : (cut_s ? 0 : (grad ? 1 : 3)), sband, s);
         check(hipGetLastError(), "excitation pass launch");
         if (ev1) check(hipEventRecord(ev1, s), "hipEventRecord");
         kern::launch_sym_finalize(L, params, grad, d_row_.as<double>(), d_col_.as<double>(), bsum, sband, sums,
@@ -1540,7 +1542,7 @@ class HipHawkes : public HawkesEngine {
         SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
         if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
         // probabilities need the sums S and B only: the band kernel and the finalize without moments
-        kern::launch_excitation_band(ev, d_params_.as<double>(), 1, false, 1, d_sband_.as<double>(), stream_);
+        kern::launch_excitation_band(ev, d_params_.as<double>(), 1, false, 3, d_sband_.as<double>(), stream_);
         check(hipGetLastError(), "excitation pass launch");
         if (timing_) check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
         // the moments written go to the spare sums buffer, which the cache does not point at

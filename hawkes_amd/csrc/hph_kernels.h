@@ -118,7 +118,10 @@ void launch_reduce(const Events &ev, int K, int nblocks, bool grad, const double
 // Excitation sums per event of K vectors (dimensions 1..3), natural units, out [K][3 or 1][npad]
 // (S, S r^2, S dt; without the gradient S only) from the earlier events of each 64-event group,
 // per-pair dt > 0 mask. cut 0: within the likelihood passes' excitation cut; 1: within the
-// exact-zero point (probability requests); 2: every earlier event.
+// exact-zero point; 2: every earlier event; 3 (probability requests, sums without moments): within
+// the exact-zero point, and each group of 64 events stops as soon as, for every one of its events,
+// the earlier events not yet added could change its sum by at most 2^-60 of it (a relative bound
+// per event; used where theta omega sigma^D > 0 and mu0 tauX^D tauT >= 0, else as cut 1).
 void launch_excitation_band(const Events &ev, const double *params, int K, bool grad, int cut, double *out,
                             hipStream_t s);
 

@@ -323,7 +323,20 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
     const double *pk = params + 6 * (size_t)k;
     const double sigma = pk[0], omega = pk[3];
     const double cSx = -0.5 * sigma * sigma;
-    const double xs = cut == 0 ? excitation_skip_arg(pk, D, n) : (cut == 1 ? kZeroExpArg : HUGE_VAL);
+    const double xs = cut == 0 ? excitation_skip_arg(pk, D, n) : (cut == 1 || cut == 3 ? kZeroExpArg : HUGE_VAL);
+    // cut 3 (sums without moments): each row also stops, relative to its own sum, once every
+    // earlier event left could add no more than 2^-60 of it (below)
+    double sD = 1.0, xD = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        sD *= sigma;
+        xD *= pk[1];
+    }
+    // only where the rate is a sum of non-negative parts, so that a bound relative to the
+    // excitation sum is one relative to the rate and to the probability
+    const bool rel = !GRAD && cut == 3 && pk[4] * omega * sD > 0.0 && pk[5] * xD * pk[2] >= 0.0;
+    __shared__ double part[4][kNarrow];
+    bool done = false;
     out += (size_t)k * NQ * npad;
     double xi[D];
 #pragma unroll
@@ -335,6 +348,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
     for (int tj = t_own; tj >= 0; --tj) {
         const int j0 = tj * kTile;
         if (tj < t_own && omega >= 0.0 && omega * (t_first - ev.t[j0 + kTile - 1]) >= xs) break; // uniform
+        if (done) break; // uniform: every wave took the same vote
         // columns up to the group's last row: later events excite none of its rows
         const int cnt_all = min(min(kTile, n - j0), (rg + 1) * kNarrow - j0);
         __syncthreads();
@@ -367,6 +381,20 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
                 acc[1] = __builtin_fma(es, r2, acc[1]);
                 acc[2] = __builtin_fma(es, dt, acc[2]);
             }
+        }
+        if (rel && j0 > 0) {
+            // The j0 events still ahead are no later than t[j0 - 1]: each of their terms is at most
+            // exp(-omega (t_i - t[j0 - 1])) (the spatial factor is at most 1), so together they
+            // change row i's sum by at most j0 exp(-omega gap) <= 2^-60 of the sum formed so far
+            // once the test below holds: a relative bound for every row, tiny sums included. The
+            // rows' sums so far are the four waves' partials, added in a fixed order; every wave
+            // evaluates the same test for the same 64 rows, so the vote is uniform.
+            part[w][lane] = acc[0];
+            __syncthreads();
+            const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+            const double gap = ti - ev.t[j0 - 1];
+            const bool ok = i >= n || (omega > 0.0 && (double)j0 * exp_fp64(-omega * gap) <= 0x1p-60 * sum);
+            done = __all(ok) != 0;
         }
     }
     if (w > 0) {
@@ -462,7 +490,7 @@ void launch_pair(const Events &ev, const double *params, int K, const PairGrid &
 void launch_excitation_band(const Events &ev, const double *params, int K, bool grad, int cut, double *out,
                             hipStream_t s) {
     if (K < 1 || K > 65535) throw std::invalid_argument("excitation pass: 1..65535 vectors");
-    if (cut < 0 || cut > 2) throw std::invalid_argument("excitation pass: cut mode 0, 1 or 2");
+    if (cut < 0 || cut > 3) throw std::invalid_argument("excitation pass: cut mode 0, 1, 2 or 3");
     const dim3 grid((unsigned)((ev.n + kNarrow - 1) / kNarrow), (unsigned)K);
     const int ex = cut;
     switch (ev.dim * 2 + (grad ? 1 : 0)) {
