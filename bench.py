@@ -19,7 +19,7 @@ reads a `rocprofv3 --kernel-trace --marker-trace` of a run and prints its kernel
 
 value counts the N^2 ordered pairs the reference forms per call; the engine forms each
 background pair once for both events and skips tiles whose background terms are all below
-e^-(50 + ln N), or whose excitation terms are all below e^-X_S (excitation_skip_arg; rates change
+e^-(45 + ln N) (each event's sums change by < 4e-18 relative), or whose excitation terms are all below e^-X_S (excitation_skip_arg; rates change
 by < 2e-22 relative) (roofline.pairs_formed gives the pairs actually formed, all_pairs the rate
 with no skipping).
 """
@@ -50,9 +50,9 @@ ZERO_ARG = 746.0  # exp(-746) is exactly +0.0 in FP64
 
 
 def background_skip_arg(n):
-    """FP64 background skip point of the kernels: min(746, 50 + ln N) (terms below e^-(50 + ln N)
-    change every event's sums by < 5e-20 relative)."""
-    return min(ZERO_ARG, 50.0 + np.log(max(n, 1)))
+    """FP64 background skip point of the kernels: min(746, 45 + ln N) (terms below e^-(45 + ln N)
+    change every event's sums by < 4e-18 relative)."""
+    return min(ZERO_ARG, 45.0 + np.log(max(n, 1)))
 
 
 def excitation_skip_arg(params, n, D=2):

@@ -46,15 +46,16 @@ constexpr double kZeroExpArg = 746.0;
 constexpr double kZeroExpArg32 = 105.0;
 
 // Background tiles are skipped where every term is below e^-X, X = min(exact-zero point,
-// 50 + ln n). Bound: a skipped pair has b = e^-(a + c) with a = tauX^2 r^2/2 >= 0 and
+// 45 + ln n). Bound: a skipped pair has b = e^-(a + c) with a = tauX^2 r^2/2 >= 0 and
 // c = tauT^2 dt^2/2 >= X, so b, b 2a and b 2c (the pair's terms in sum b, tauX^2 sum b r^2 and
 // tauT^2 sum b dt^2) are each at most 2X e^-X. Every event's own pair is in its sums (b = 1), so
-// the n pairs of one event change those sums by at most n 2X e^-X = 2X e^-50 < 5e-20 relative
-// to sum b (X <= 110): far below one FP64 rounding. The excitation term has no such floor and
+// the n pairs of one event change those sums by at most n 2X e^-X = 2X e^-45 < 4e-18 relative
+// to sum b (X <= 67 for n <= 2^31): far below one FP64 rounding (1.1e-16), and still ~3e-15 where
+// a gradient component cancels to 1e-3 of its summation scale. The excitation term has no such floor and
 // is skipped only where it is exactly zero.
 inline double background_skip_arg(long long n, bool fp32) {
     const double exact = fp32 ? kZeroExpArg32 : kZeroExpArg;
-    const double negl = 50.0 + std::log((double)(n > 1 ? n : 1));
+    const double negl = 45.0 + std::log((double)(n > 1 ? n : 1));
     return negl < exact ? negl : exact;
 }
 

@@ -28,10 +28,11 @@ extern "C" {
 #define HPH_FLAG_AVX (1L << 6)    /* accepted; CPU-only, no effect here */
 #define HPH_FLAG_AVX512 (1L << 7) /* accepted; CPU-only, no effect here */
 /* Extension: evaluate every pair. By default a likelihood or gradient pass skips tiles of
- * time-sorted events whose background terms are all below e^-(50 + ln N) and whose excitation
+ * time-sorted events whose background terms are all below e^-(45 + ln N) and whose excitation
  * terms are all below e^-X_S, X_S = 50 + ln N + max(0, ln(|cS| / |cB|)) (cB = mu0 tauX^D tauT /
- * 2 pi, cS = theta omega sigmaX^D / sqrt(2 pi)); the skipped terms move every event's rate and
- * sums by under 5e-20 relative, so results agree with this flag's to FP64 rounding, not bitwise.
+ * 2 pi, cS = theta omega sigmaX^D / sqrt(2 pi)); the skipped terms move every event's background
+ * sums by under 4e-18 and its rate by under 2e-22 relative, so results agree with this flag's to
+ * FP64 rounding, not bitwise.
  * The excitation cut is made only where cB and cS have the same sign. A probability request
  * skips exactly zero excitation terms only (their cut is bounded relative to the rate, not to a
  * tiny probability). */

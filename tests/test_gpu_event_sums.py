@@ -4,7 +4,7 @@ against the oracle's per-row sums (oracle.rows, long double).
 The engine's sums carry no normalising constants (B_ij = exp(-(tauX^2 r^2 + tauT^2 dt^2) / 2),
 S_ij = exp(-omega dt - sigma^2 r^2 / 2)); the oracle's are pdf-form (B / 2 pi, S / sqrt(2 pi)).
 Tolerances: the background sums contain each event's own pair (B_ii = 1) and their cut terms are
-below e^-(50 + ln N) of the sum, so they are compared at 1e-12 relative. The excitation sums are
+below e^-(45 + ln N) of the sum, so they are compared at 1e-12 relative. The excitation sums are
 cut relative to the rate (terms below e^-X_S |cB|), so they are compared in the rate's units:
 |cS dS| <= 1e-12 lambda_i, with the moments scaled as the gradient uses them (sigma^2 S r^2,
 omega S dt). Checked after a full pass of each kind and after a sampler move re-evaluated from

@@ -81,7 +81,7 @@ def test_fp32_config2_size():
 @pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_fp32_tile_skip_matches_all_pairs(kernel):
     """FP32-bulk with and without tile skipping (skipped background terms are below
-    e^-(50 + ln N) relative to each event's sums): equal to within rounding."""
+    e^-(45 + ln N) relative to each event's sums): equal to within rounding."""
     locs, times = synth.cluster_process(6000, side=20.0, horizon=43_800.0, seed=41)
     out = []
     for flags in (_lib.OPENCL | _lib.FLOAT, _lib.OPENCL | _lib.FLOAT | _lib.ALL_PAIRS):

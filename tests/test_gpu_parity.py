@@ -146,8 +146,8 @@ def test_batch_equals_single_calls_bitwise(kernel):
 ])
 @pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
 def test_tile_skip_matches_all_pairs(name, params, kernel):
-    """Tiles are skipped where every background term is below e^-(50 + ln N) (each event's sums
-    change by < 5e-20 relative; its own pair contributes 1) or, in likelihood and gradient passes,
+    """Tiles are skipped where every background term is below e^-(45 + ln N) (each event's sums
+    change by < 4e-18 relative; its own pair contributes 1) or, in likelihood and gradient passes,
     every excitation term is below e^-X_S (rates change by < 2e-22 relative; passes that return
     probabilities skip exact zeros only). Results equal the every-pair evaluation
     (HPH_FLAG_ALL_PAIRS) to within rounding: ll and probabilities to 1e-15 relative, gradient

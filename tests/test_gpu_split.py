@@ -5,7 +5,7 @@ terms) on data spread over a domain much wider than the background kernel's reac
     probabilities 1e-12 relative or 1e-300 absolute for subnormal ones) in D = 1, 2, 3, with
     many windows of the culled order;
   * against the every-pair evaluation (HPH_FLAG_ALL_PAIRS) to 1e-15: every skipped group pair
-    holds only background terms below e^-(50 + ln N);
+    holds only background terms below e^-(45 + ln N);
   * at N ~ 1.5e5 on 100 km: the built-in choice picks the split passes, which agree with the
     symmetric pass (ll 1e-13, gradient 1e-12), FP32-bulk within 1e-6 of FP64, K-batches equal
     to single calls to rounding;
