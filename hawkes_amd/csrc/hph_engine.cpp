@@ -1174,6 +1174,7 @@ class HipHawkes : public HawkesEngine {
         if (last_kind_ == 0) throw NotInitialized("no pair pass has run yet");
         out[0] = last_kind_;
         out[1] = last_fast_ ? 1 : 0;
+        if (length >= 3) out[2] = last_kind_ == HPH_PAIR_KERNEL_ROWWISE ? last_k_ : last_sub_batch_;
     }
 
     // The six per-event sums of the current parameters' last single-vector evaluation with the
@@ -1360,11 +1361,18 @@ class HipHawkes : public HawkesEngine {
         double *probs_dev = (probs_host || write_probs) ? d_probs_.as<double>() : nullptr;
         TuneConfig cfg = cfg_in;
         const bool sym_kind = cfg.kind == HPH_PAIR_KERNEL_SYMMETRIC || cfg.kind == HPH_PAIR_KERNEL_SPLIT;
+        // K vectors whose symmetric partial sums do not fit run as consecutive sub-batches that do
+        // (the row-wise pass, the other way out, forms every pair of every vector without culling)
+        int sub = K;
         if (sym_kind && !sym_fits(K, cfg, grad)) {
-            if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC || kernel_ == HPH_PAIR_KERNEL_SPLIT)
-                throw std::runtime_error("the symmetric pass's partial sums do not fit in device memory at this N "
-                                         "and K; choose the row-wise pass");
-            cfg = TuneConfig{}; // row-wise: partials of K x chunks x 6 x N doubles only
+            while (sub > 1 && !sym_fits(sub, cfg, grad)) sub = (sub + 1) / 2;
+            if (!sym_fits(sub, cfg, grad)) {
+                if (kernel_ == HPH_PAIR_KERNEL_SYMMETRIC || kernel_ == HPH_PAIR_KERNEL_SPLIT)
+                    throw std::runtime_error("the symmetric pass's partial sums do not fit in device memory at "
+                                             "this N; choose the row-wise pass");
+                cfg = TuneConfig{}; // row-wise: partials of K x chunks x 6 x N doubles only
+                sub = K;
+            }
         }
         double *w_dev = nullptr; // 1 / lambda of vector 0 (sorted order), kept for the location gradient
         if (write_probs) {
@@ -1376,32 +1384,43 @@ class HipHawkes : public HawkesEngine {
             const bool split = cfg.kind == HPH_PAIR_KERNEL_SPLIT ||
                                (kernel_ == HPH_PAIR_KERNEL_AUTO && skip_zero && split_pays(params, K));
             last_fast_ = !force_natural_ && fast_ok(extents_, params, K, split);
-            if (split) {
-                if (!culled_valid_) build_culled();
-                d_bsum_.ensure(sizeof(double) * (size_t)K * (grad ? 3 : 1) * npad_);
-                d_sband_.ensure(sizeof(double) * (size_t)K * 3 * npad_);
-                SymRunner::DataRef culled{d_xc_.as<double>(), d_tc_.as<double>(), d_cperm_.as<int>(), ev.n, ev.npad,
-                                          ev.t_last};
-                sym_.enqueue_split(ref, culled, d_gstat_.as<double>(), K, embeddingDimension, d_params_.as<double>(),
-                                   grad, fp32_, cfg.rows_per_lane, cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
-                                   last_fast_, d_bsum_.as<double>(), d_sband_.as<double>(), probs_dev, w_dev,
-                                   d_result_.as<double>(), stream_,
-                                   timing_ ? ev_[0] : nullptr, timing_ ? ev_[1] : nullptr,
-                                   timing_ ? ev_[2] : nullptr, sums, gstat_host_.data(), h_params_.as<double>());
-            } else {
-                const std::vector<const double *> host_t{tsorted_.data()};
-                sym_.enqueue({ref}, {K}, embeddingDimension, d_params_.as<double>(), grad, fp32_, cfg.rows_per_lane,
-                             cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
-                             probs_dev, w_dev, d_result_.as<double>(), stream_, timing_ ? ev_[0] : nullptr,
-                             timing_ ? ev_[1] : nullptr, timing_ ? ev_[2] : nullptr, sums, &host_t,
-                             h_params_.as<double>());
+            if (split && !culled_valid_) build_culled();
+            // vectors [k0, k0 + kk) per launch (one launch unless the partial sums of all K do not fit;
+            // probabilities, 1/lambda and the cached sums belong to single-vector calls, never split);
+            // with timing, the first launch's start and the last launch's end are recorded
+            for (int k0 = 0; k0 < K; k0 += sub) {
+                const int kk = std::min(sub, K - k0);
+                const bool first = k0 == 0, last = k0 + kk == K;
+                hipEvent_t e0 = timing_ && first ? ev_[0] : nullptr;
+                hipEvent_t e1 = timing_ && last ? ev_[1] : nullptr, e2 = timing_ && last ? ev_[2] : nullptr;
+                const double *pdev = d_params_.as<double>() + 6 * (size_t)k0;
+                const double *phost = h_params_.as<double>() + 6 * (size_t)k0;
+                double *rdev = d_result_.as<double>() + (size_t)kern::kResultWidth * k0;
+                if (split) {
+                    d_bsum_.ensure(sizeof(double) * (size_t)kk * (grad ? 3 : 1) * npad_);
+                    d_sband_.ensure(sizeof(double) * (size_t)kk * 3 * npad_);
+                    SymRunner::DataRef culled{d_xc_.as<double>(), d_tc_.as<double>(), d_cperm_.as<int>(), ev.n,
+                                              ev.npad, ev.t_last};
+                    sym_.enqueue_split(ref, culled, d_gstat_.as<double>(), kk, embeddingDimension, pdev, grad, fp32_,
+                                       cfg.rows_per_lane, cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, last_fast_,
+                                       d_bsum_.as<double>(), d_sband_.as<double>(), probs_dev, w_dev, rdev, stream_,
+                                       e0, e1, e2, sums, gstat_host_.data(), phost);
+                } else {
+                    const std::vector<const double *> host_t{tsorted_.data()};
+                    sym_.enqueue({ref}, {kk}, embeddingDimension, pdev, grad, fp32_, cfg.rows_per_lane,
+                                 cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
+                                 probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums, &host_t, phost);
+                }
             }
+            last_sub_batch_ = sub;
+            last_k_ = K;
             last_split_ = split;
             last_kind_ = split ? HPH_PAIR_KERNEL_SPLIT : HPH_PAIR_KERNEL_SYMMETRIC;
         } else {
             last_fast_ = false;
             last_split_ = false;
             last_kind_ = HPH_PAIR_KERNEL_ROWWISE;
+            last_k_ = K;
             kern::PairGrid g = kern::plan_pair_grid(locationCount, K, cfg.target_blocks);
             g.skip_zero = skip_zero;
             g.zero_arg_b = kern::background_skip_arg(locationCount, fp32_);
@@ -1670,6 +1689,8 @@ class HipHawkes : public HawkesEngine {
     Extents extents_;
     std::vector<double> tsorted_; // the times in ascending order (host), for the work-queue order
     std::vector<double> gstat_host_; // the culled order's box table (host), for its queue order
+    int last_sub_batch_ = 0;         // vectors per launch of the last symmetric / split evaluation
+    int last_k_ = 0;                 // vectors of the last full evaluation
 
     int device_;
     bool fp32_;

@@ -154,6 +154,13 @@ class Engine:
         check(_lib.lib().hph_get_last_pass(self._h, out, 2))
         return int(out[0]), bool(out[1])
 
+    def vectors_per_launch(self):
+        """Parameter vectors per launch of the last full evaluation (fewer than the batch's when
+        its symmetric or split partial sums did not fit in device memory at once)."""
+        out = (ctypes.c_int * 3)()
+        check(_lib.lib().hph_get_last_pass(self._h, out, 3))
+        return int(out[2])
+
     def timing(self):
         """(pair-pass ms, finalize + reduction ms) of the last evaluation."""
         out = np.zeros(2)

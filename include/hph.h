@@ -149,8 +149,10 @@ int hph_get_last_evaluation(hph_engine *engine, int *kind);
  * out[0] = HPH_PAIR_KERNEL_ROWWISE, HPH_PAIR_KERNEL_SYMMETRIC or HPH_PAIR_KERNEL_SPLIT; out[1] = 1
  * when it ran the base-2 form (coordinates and times scaled so that the background term is 2^-y
  * of a sum of squares; chosen only where the rounding of the scaled values stays below 1e-13
- * relative on the terms), 0 for the natural form. length must be >= 2; HPH_ERR_NOT_INITIALIZED
- * before the first evaluation. */
+ * relative on the terms), 0 for the natural form; with length >= 3, out[2] = parameter vectors
+ * per launch (a batch whose symmetric or split partial sums do not fit in device memory at once
+ * runs as consecutive launches of that many vectors). length must be >= 2;
+ * HPH_ERR_NOT_INITIALIZED before the first evaluation. */
 int hph_get_last_pass(hph_engine *engine, int *out, size_t length);
 /* The per-event sums behind the last log-likelihood-and-gradient evaluation at the current
  * parameters (dimensions 1..3): out is N x 6 row-major in the caller's order, length = 6 N, with

@@ -9,7 +9,8 @@ terms) on data spread over a domain much wider than the background kernel's reac
   * at N ~ 1.5e5 on 100 km: the built-in choice picks the split passes, which agree with the
     symmetric pass (ll 1e-13, gradient 1e-12), FP32-bulk within 1e-6 of FP64, K-batches equal
     to single calls to rounding;
-  * at N ~ 1e6, a K = 8 batch stays on the split passes and equals its single calls.
+  * at N ~ 1e6, a K = 8 batch stays on the split passes and equals its single calls; a K = 16
+    batch, whose partial sums do not fit at once, runs as two launches of 8.
 """
 import numpy as np
 import pytest
@@ -174,6 +175,25 @@ def test_k8_batch_at_n1e6_stays_on_the_split_passes():
         assert e.last_pass()[0] == _lib.PAIR_KERNEL_SPLIT
         assert rel_err(ll, ll_b[k]) <= 1e-14
         assert np.all(np.abs(g - g_b[k]) <= 1e-13 * np.maximum(np.abs(g), 1e-3 * np.abs(g).max()))
+    e.close()
+
+
+def test_k16_batch_at_n1e6_runs_as_launches_that_fit():
+    """K = 16 vectors at N ~ 1e6: the background pass's column partials of all 16 (~375 GB) do
+    not fit the device, so the batch runs as consecutive launches of 8 vectors on the split
+    passes, each bitwise equal to a batch of those 8 vectors."""
+    locs, times, p = synth.workload("cluster_b700k")
+    P = synth.jitter_params(16, seed=670)
+    print("N=%d K=16 batch" % len(times), flush=True)
+    e = engine(locs, times, p, config=None)
+    e.set_parameters_batch(P)
+    ll, g = e.log_likelihood_batch()
+    assert e.last_pass()[0] == _lib.PAIR_KERNEL_SPLIT and e.vectors_per_launch() == 8
+    for half in (slice(0, 8), slice(8, 16)):
+        e.set_parameters_batch(P[half])
+        ll8, g8 = e.log_likelihood_batch()
+        assert e.vectors_per_launch() == 8
+        assert np.array_equal(ll8, ll[half]) and np.array_equal(g8, g[half])
     e.close()
 
 
