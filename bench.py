@@ -416,8 +416,9 @@ def large_dataset(n_leapfrog=20):
             ll, g = e.log_likelihood_and_gradient()
             t_llg = time.perf_counter() - t0
         assert e.last_evaluation() == _lib.EVAL_FULL
-        # probabilities: a log-likelihood pass that skips exact zeros only (a likelihood pass's own
-        # excitation cut is bounded relative to the rates, not to tiny probabilities)
+        # probabilities: the excitation sums formed again with a cut relative to each event's own
+        # sum (a likelihood pass's cut is bounded relative to the rates, not to tiny
+        # probabilities), on the background sums the likelihood call above left in the cache
         with RANGE("large_probs_" + name):
             t0 = time.perf_counter()
             probs = e.probs_self_excite()

@@ -67,7 +67,8 @@ inline double background_skip_arg(long long n, bool fp32) {
 // excitation parts of the gradient terms, by at most |cS| n e^-X_S <= e^-50 |cB| = 2e-22 |cB|,
 // i.e. relatively by under 2e-22. The self-excitation probability S/(B+S) then moves by under
 // 2e-22 in absolute terms, which is not a relative bound for tiny probabilities: passes that
-// return probabilities skip exact zeros only. Without a background (cB = 0) nothing is cut, and
+// return probabilities skip exact zeros only, or stop each event's excitation sum relative to
+// itself (launch_excitation_band, cut 3). Without a background (cB = 0) nothing is cut, and
 // neither when cB and cS differ in sign (theta, omega, mu0 or a precision negative: the rate is
 // then a difference and the floor lambda_i >= |cB| does not hold).
 __host__ __device__ inline double excitation_skip_arg(const double *pk, int D, long long n) {

@@ -34,8 +34,10 @@ extern "C" {
  * sums by under 4e-18 and its rate by under 2e-22 relative, so results agree with this flag's to
  * FP64 rounding, not bitwise.
  * The excitation cut is made only where cB and cS have the same sign. A probability request
- * skips exactly zero excitation terms only (their cut is bounded relative to the rate, not to a
- * tiny probability). */
+ * (their likelihood cut is bounded relative to the rate, not to a tiny probability) skips exactly
+ * zero excitation terms only, or, where the excitation sums come from the band pass (the split
+ * passes, and after a likelihood call at the same tauXprec and tauTprec), stops each event's sum
+ * once the terms left could change it by at most 2^-60 of itself. */
 #define HPH_FLAG_ALL_PAIRS (1L << 16)
 
 /* Status codes. */
@@ -135,8 +137,8 @@ int hph_get_timing(hph_engine *engine, double *out, size_t length);
  * R/Untitled.R:254-422) reuses the background sums and runs an excitation-only pass, or, when
  * only theta and mu0 changed, reuses both (O(N)). Data changes, tauXprec or tauTprec changes and
  * HPH_FLAG_ALL_PAIRS take the full pass; dimensions 1..3. A probability request at the cached
- * tauXprec and tauTprec reuses the background sums and forms the excitation sums anew, skipping
- * exact zeros only (it leaves the cache as it was). A batch whose vectors share tauXprec and
+ * tauXprec and tauTprec reuses the background sums and forms the excitation sums anew, each to
+ * 2^-60 of itself (it leaves the cache as it was). A batch whose vectors share tauXprec and
  * tauTprec reuses (or refills) the cached background sums too.
  * hph_get_last_evaluation: *kind = HPH_EVAL_FULL, HPH_EVAL_EXCITATION (cached background sums +
  * excitation pass) or HPH_EVAL_CACHED (cached sums only) for the last single-vector call or

@@ -149,7 +149,8 @@ def test_tile_skip_matches_all_pairs(name, params, kernel):
     """Tiles are skipped where every background term is below e^-(45 + ln N) (each event's sums
     change by < 4e-18 relative; its own pair contributes 1) or, in likelihood and gradient passes,
     every excitation term is below e^-X_S (rates change by < 2e-22 relative; passes that return
-    probabilities skip exact zeros only). Results equal the every-pair evaluation
+    probabilities skip exact zeros only, or terms below 2^-60 of each event's own excitation sum).
+    Results equal the every-pair evaluation
     (HPH_FLAG_ALL_PAIRS) to within rounding: ll and probabilities to 1e-15 relative, gradient
     component k to 1e-15 of its summation scale S_k (the oracle's sum of |per-event terms|)."""
     locs, times = synth.cluster_process(6000, side=20.0, horizon=43_800.0, seed=41)
