@@ -12,7 +12,12 @@ on [-1/2, 1/2].
 exp2 of a non-negative exponent (background terms of the base-2 pair form, 2^-y with y >= 0):
     y = i + f,   i = trunc(y),   f = fract(y) in [0, 1) exact,   2^-y = 2^-i * s(f),   s ~ 2^-f
 on [0, 1] (the same relative error as q: 2^-f on [0, 1] is 2^-1/2 times 2^g on [-1/2, 1/2]).
-All three polynomials have value 1 at 0 exactly and the other coefficients minimise
+exp2 of a non-negative exponent from a table (the FP64 background terms of the base-2 pair form,
+whose exponent arrives scaled by 32: y32 = 32 y >= 0):
+    -trunc(y32) = 32 a + b (b in [0, 31]),   f = fract(y32) in [0, 1) exact,
+    2^-y = 2^a * M[b] * u(f),   M[b] = 2^(b/32) correctly rounded,   u ~ 2^(-f/32) on [0, 1].
+u has degree 5: the interval is 1/32 of the one above.
+All four polynomials have value 1 at 0 exactly and the other coefficients minimise
 the weighted (relative) least-squares error on 3000 Chebyshev nodes at 40 digits, which lands
 close to the minimax polynomial. The script then measures the max relative error of the
 double-precision Horner evaluation with correctly rounded fma at 20000 points.
@@ -90,11 +95,28 @@ def main():
         i = int(y)
         v = mp.ldexp(mp.mpf(horner(coef3_d, y - i)), -i)
         worst3 = max(worst3, abs(float((v - mp.power(2, -y)) / mp.power(2, -y))))
+    tdeg = 5
+    coef4 = fit(tdeg, mp.mpf(1), -ln2 / 32, mp.mpf(0))
+    coef4_d = [float(c) for c in coef4]
+    mant = [float(mp.power(2, mp.mpf(b) / 32)) for b in range(32)]
+    worst4 = 0.0
+    for _ in range(20000):
+        y = random.choice((random.uniform(0.0, 4.0), random.uniform(0.0, 1020.0)))
+        y32 = float(mp.mpf(y) * 32)
+        ni = -int(y32)
+        f = y32 + ni
+        t = mant[ni & 31]
+        q = coef4_d[-1]
+        for c in reversed(coef4_d[1:-1]):
+            q = fma_d(q, f, c)
+        v = mp.ldexp(mp.mpf(fma_d(t, float(mp.mpf(f) * mp.mpf(q)), t)), ni >> 5)
+        ref = mp.power(2, -mp.mpf(y32) / 32)
+        worst4 = max(worst4, abs(float((v - ref) / ref)))
     path = os.path.join(HERE, "hph_exp_coeffs.h")
     with open(path, "w") as f:
         f.write("// Generated by gen_exp_poly.py (degree %d); do not edit.\n" % degree)
-        f.write("// Measured max relative error of the FP64 evaluation: exp %.3e, exp2 %.3e, exp2 of -y %.3e\n"
-                % (worst, worst2, worst3))
+        f.write("// Measured max relative error of the FP64 evaluation: exp %.3e, exp2 %.3e, exp2 of -y %.3e,\n"
+                "// exp2 of -y from the table %.3e\n" % (worst, worst2, worst3, worst4))
         f.write("#pragma once\n")
         f.write("#define HPH_EXP_DEGREE %d\n" % degree)
         f.write("#define HPH_LOG2E %r\n" % log2e)
@@ -106,7 +128,13 @@ def main():
             f.write("#define HPH_EXP2_C%d %r\n" % (k, c))
         for k, c in enumerate(coef3_d):
             f.write("#define HPH_EXP2N_C%d %r\n" % (k, c))
-    print("degree", degree, "max rel err exp %.3e exp2 %.3e exp2 of -y %.3e" % (worst, worst2, worst3), "->", path)
+        f.write("#define HPH_EXP2T_DEGREE %d\n" % tdeg)
+        for k, c in enumerate(coef4_d):
+            f.write("#define HPH_EXP2T_C%d %r\n" % (k, c))
+        for j, m in enumerate(mant):
+            f.write("#define HPH_EXP2T_M%d %r\n" % (j, m))
+    print("degree", degree, "max rel err exp %.3e exp2 %.3e exp2 of -y %.3e table %.3e" % (worst, worst2, worst3, worst4),
+          "->", path)
 
 
 if __name__ == "__main__":

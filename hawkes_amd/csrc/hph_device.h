@@ -89,6 +89,41 @@ __device__ __forceinline__ double exp2_neg_fp64(double y) {
     return __builtin_amdgcn_ldexp(p, mi);
 }
 
+// 2^-y for y >= 0 from a 32-entry table: the FP64 background terms of the base-2 form, whose
+// exponent arrives scaled by 32 (y32 = 32 y, a sum of squares of coordinates and times scaled by
+// time_scale32_dd; the host keeps y32 below 2^31). -trunc(y32) = 32 a + b with a = its arithmetic
+// shift by 5 and b in [0, 31] its low bits, f = fract(y32) in [0, 1) (exact for y32 >= 0):
+//   2^-y = 2^a tab[b] u(f),   tab[b] = 2^(b/32),   u(f) = 1 + f q(f) ~ 2^(-f/32)
+// with u the degree-5 polynomial of gen_exp_poly.py, applied as tab[b] + tab[b] (f q(f)), and
+// v_ldexp_f64 for 2^a. Eleven VALU operations (convert, fract, two for the address, shift, 4 fma,
+// mul, fma, ldexp) and one LDS read where exp2_neg_fp64 takes 13. The table is 256 bytes, one
+// entry per pair of LDS banks, so the 64 lanes' reads never conflict (lanes reading one entry
+// share it). Max relative error 2.7e-16 (table entries correctly rounded, so their errors do not
+// share a sign as a polynomial's do); exactly 0 below the FP64 range, as IEEE exp.
+constexpr int kExp2TabSize = 32;
+__device__ __forceinline__ double exp2_neg_tab(double y32, const double *tab) {
+    const int ni = (int)(-y32); // -trunc(y32)
+    const double f = __builtin_amdgcn_fract(y32);
+    double q = HPH_EXP2T_C5;
+    q = __builtin_fma(q, f, HPH_EXP2T_C4);
+    q = __builtin_fma(q, f, HPH_EXP2T_C3);
+    q = __builtin_fma(q, f, HPH_EXP2T_C2);
+    q = __builtin_fma(q, f, HPH_EXP2T_C1);
+    const double t = tab[ni & 31];
+    return __builtin_amdgcn_ldexp(__builtin_fma(t, f * q, t), ni >> 5);
+}
+// Fills exp2_neg_tab's table (LDS): entry b = 2^(b/32) correctly rounded (gen_exp_poly.py).
+// Threads tid < 32 of the workgroup write it; the caller barriers before use.
+__device__ __forceinline__ void fill_exp2_table(double *tab, int tid) {
+    constexpr double M[kExp2TabSize] = {
+        HPH_EXP2T_M0,  HPH_EXP2T_M1,  HPH_EXP2T_M2,  HPH_EXP2T_M3,  HPH_EXP2T_M4,  HPH_EXP2T_M5,  HPH_EXP2T_M6,
+        HPH_EXP2T_M7,  HPH_EXP2T_M8,  HPH_EXP2T_M9,  HPH_EXP2T_M10, HPH_EXP2T_M11, HPH_EXP2T_M12, HPH_EXP2T_M13,
+        HPH_EXP2T_M14, HPH_EXP2T_M15, HPH_EXP2T_M16, HPH_EXP2T_M17, HPH_EXP2T_M18, HPH_EXP2T_M19, HPH_EXP2T_M20,
+        HPH_EXP2T_M21, HPH_EXP2T_M22, HPH_EXP2T_M23, HPH_EXP2T_M24, HPH_EXP2T_M25, HPH_EXP2T_M26, HPH_EXP2T_M27,
+        HPH_EXP2T_M28, HPH_EXP2T_M29, HPH_EXP2T_M30, HPH_EXP2T_M31};
+    if (tid < kExp2TabSize) tab[tid] = M[tid];
+}
+
 // Scaled time of the base-2 pair form: t' = (t - t0) * st with st = |tauT| sqrt(log2(e) / 2), so
 // that the background exponent tauT^2 dt^2 / 2 becomes dt'^2 / log2(e) and 2^-(dt'^2) =
 // e^-(tauT^2 dt^2 / 2). Kernels and finalize compute st with this one expression.
@@ -100,6 +135,8 @@ __device__ __forceinline__ double time_scale(double tauT) { return fabs(tauT) * 
 constexpr double kSqrtHalfLog2eHi = 0.8493218002880191, kSqrtHalfLog2eLo = -2.5959464104597403e-17;
 constexpr double kTwoLn2Hi = 1.3862943611198906, kTwoLn2Lo = 4.638093627692599e-17;
 constexpr double kLog2eHi = 1.4426950408889634, kLog2eLo = 2.0355273740931033e-17;
+// 4 sqrt(log2(e)) = sqrt(32) sqrt(log2(e) / 2): the scale of the table form (exponents times 32).
+constexpr double kSqrt16Log2eHi = 4.804489635145799, kSqrt16Log2eLo = -1.1835725830478566e-16;
 
 // The base-2 form's scales as double-doubles: a rounded scale st would make every scaled time
 // st (t - t0) carry the same relative error, i.e. the pass would evaluate the background kernel at
@@ -113,13 +150,21 @@ __device__ __forceinline__ double2 time_scale_dd(double tau) {
     const double err = __builtin_fma(a, kSqrtHalfLog2eHi, -hi); // exact error of the product
     return make_double2(hi, __builtin_fma(a, kSqrtHalfLog2eLo, err));
 }
+// The same with |tau| 4 sqrt(log2(e)) (exponents scaled by 32, for exp2_neg_tab).
+__device__ __forceinline__ double2 time_scale32_dd(double tau) {
+    const double a = fabs(tau);
+    const double hi = a * kSqrt16Log2eHi;
+    const double err = __builtin_fma(a, kSqrt16Log2eHi, -hi);
+    return make_double2(hi, __builtin_fma(a, kSqrt16Log2eLo, err));
+}
 // v (s.hi + s.lo), rounded once.
 __device__ __forceinline__ double scale_by(double v, double2 s) { return __builtin_fma(v, s.x, v * s.y); }
-// A moment summed in the base-2 form's squared units (r'^2 = r^2 tau^2 log2(e) / 2) back to
-// natural units: m 2 ln2 / tau2 with tau2 = tau * tau as the finalize's gradient formulas round it,
-// so that tau2 times the result is m 2 ln 2 to one rounding.
-__device__ __forceinline__ double from_base2_sq(double m, double tau2) {
-    return __builtin_fma(m, kTwoLn2Hi, m * kTwoLn2Lo) / tau2;
+// A moment summed in the base-2 form's squared units (r'^2 = r^2 tau^2 log2(e) / 2, or 32 times
+// that in the table form: unit = 1/32) back to natural units: m unit 2 ln2 / tau2 with
+// tau2 = tau * tau as the finalize's gradient formulas round it, so that tau2 times the result is
+// m unit 2 ln 2 to one rounding. unit is a power of two (the constants' products are exact).
+__device__ __forceinline__ double from_base2_sq(double m, double tau2, double unit = 1.0) {
+    return __builtin_fma(m, kTwoLn2Hi * unit, m * (kTwoLn2Lo * unit)) / tau2;
 }
 
 // Background exponential exp(cBt dt^2 + cBx r^2): FP64 (exp_fp64) or FP32-bulk (v_exp_f32; the

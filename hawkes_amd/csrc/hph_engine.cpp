@@ -132,8 +132,8 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
 }
 
 // True when the pass may use the base-2 form (scaled coordinates and times, clamp-free exp2):
-// every background (tauX^2 r^2/2 + tauT^2 dt^2/2) and excitation (|omega| dt + sigma^2 r^2/2)
-// exponent of the K vectors is below 1e9 in size, the scales st = |tauT| sqrt(log2(e)/2) and
+// every background (tauX^2 r^2/2 + tauT^2 dt^2/2) exponent of the K vectors is below 4e7 and
+// every excitation (|omega| dt + sigma^2 r^2/2) exponent below 1e9 in size, the scales st = |tauT| sqrt(log2(e)/2) and
 // sx = |tauX| sqrt(log2(e)/2) lie in [1e-100, 1e100] with omega log2(e) / st and sigma^2 / tauX^2
 // finite, and distinct times are at least 8 ulp(max |t|) apart: then (t - t0) st, rounded, keeps
 // every strict order and every tie of the times, so the dt > 0 mask of the excitation term is
@@ -156,7 +156,9 @@ static bool fast_ok(const Extents &e, const double *params, int K, bool split = 
         if (!(std::fabs(p[3]) * e.block_span <= 300.0)) return false;
         const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
-        if (!(aB < 1e9 && aS < 1e9)) return false;
+        // background: 32 log2(e) aB < 2^31 (the FP64 table exponential converts 32 x its base-2
+        // exponent to an int)
+        if (!(aB < 4e7 && aS < 1e9)) return false;
         const double st = std::fabs(p[2]) * 0.84932180028801904272;
         const double sx = std::fabs(p[1]) * 0.84932180028801904272;
         if (!(st >= 1e-100 && st <= 1e100 && sx >= 1e-100 && sx <= 1e100)) return false;
@@ -732,6 +734,7 @@ class SymRunner {
         int nfast = 0;
         for (char f : fast) nfast += f ? 1 : 0;
         L.fast = nfast == 0 ? 0 : (nfast == (int)fast.size() ? 1 : 2);
+        L.b2tab = fp32 ? 0 : 1;
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();

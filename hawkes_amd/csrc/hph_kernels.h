@@ -198,6 +198,10 @@ struct SymLaunch {
                             // when the pass skipped every tile of the item (partials unwritten)
     int bsum_shared;        // finalize from per-event sums: 1 when bsum holds one set of background
                             // moments for every vector (vectors sharing tauX and tauT), 0 one per vector
+    int b2tab;              // 1: the base-2 form ran in FP64 with exponents scaled by 32 (the table
+                            // exponential exp2_neg_tab); its squared moments are 32 times the
+                            // unscaled ones. 0: FP32-bulk (unscaled). Set from the precision by the
+                            // host; launch_sym checks that it matches.
 };
 
 // Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted

@@ -11,6 +11,7 @@ namespace kern {
 void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart,
                       double *colpart, hipStream_t s) {
     if (L.fast != 0 && L.fast != 1) throw std::invalid_argument("split passes take one form for the whole pass");
+    if (L.fast != 0 && L.b2tab != (fp32 ? 0 : 1)) throw std::invalid_argument("base-2 units do not match the precision");
     const int v = (fp32 ? 2 : 0) + L.fast;
     if (L.pm == kPassCulledB) {
         switch (v) {
@@ -72,8 +73,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     for (int q = 0; q < QS; ++q) b[q] = a[q] + cs[q];
     if (GRAD && ds.fast) { // the base-2 pass summed e y (y = r'^2 + dt'^2) and e dt'^2, r' = sx r, dt' = st dt
         const double *pk = params + 6 * (size_t)(ds.param0 + k);
-        b[1] = from_base2_sq(b[1] - b[2], pk[1] * pk[1]);
-        b[2] = from_base2_sq(b[2], pk[2] * pk[2]);
+        const double unit = L.b2tab ? 0.03125 : 1.0;
+        b[1] = from_base2_sq(b[1] - b[2], pk[1] * pk[1], unit);
+        b[2] = from_base2_sq(b[2], pk[2] * pk[2], unit);
     }
 #pragma unroll
     for (int q = 0; q < QS; ++q) bsum[((size_t)k * QS + q) * ds.npad + i] = b[q];

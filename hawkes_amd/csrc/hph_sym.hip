@@ -86,9 +86,10 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
                 // r' = sx r, dt' = st dt (the excitation's dt is natural)
                 const double *pk = params + 6 * (size_t)(ds.param0 + k);
                 const double x2 = pk[1] * pk[1], t2 = pk[2] * pk[2];
-                s[1] = from_base2_sq(s[1] - s[2], x2);
-                s[2] = from_base2_sq(s[2], t2);
-                s[4] = from_base2_sq(s[4], x2);
+                const double unit = L.b2tab ? 0.03125 : 1.0;
+                s[1] = from_base2_sq(s[1] - s[2], x2, unit);
+                s[2] = from_base2_sq(s[2], t2, unit);
+                s[4] = from_base2_sq(s[4], x2, unit);
             }
         } else {
             s[0] = a6[0] + cs[0];
@@ -259,6 +260,8 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
     if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
+    if (L.fast != 0 && L.b2tab != (fp32 ? 0 : 1))
+        throw std::invalid_argument("symmetric pass: base-2 units do not match the precision");
     launch_queue_reset(L, s);
     if (L.pm != kPassBoth) {
         launch_sym_split(L, params, grad, fp32, rowpart, colpart, s);

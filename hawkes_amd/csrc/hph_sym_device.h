@@ -46,9 +46,13 @@ namespace kern {
 //      by exp2_fp64. The background moments are summed as e y and e dt'^2, the excitation ones
 //      as e r'^2 and e dt; the finalize kernels turn them into e r^2 = (e y - e dt'^2) / sx^2,
 //      e dt^2, e r^2 and e dt.
+//      In FP64 (F32 false) the scales are sqrt(32) times larger (time_scale32_dd), so that y
+//      arrives as 32 times the exponent for the table exponential exp2_neg_tab; cS0 is divided
+//      by 32 to match, and the finalize divides the squared moments by 32 (SymLaunch::b2tab).
 struct PairConst {
     double cB0, cB1, cS0, cS1;
     double cS0lo, cS1lo; // base 2: the low parts of cS0 and cS1 (double-doubles), else 0
+    const double *tab;   // base 2 in FP64: exp2_neg_tab's table (LDS), else unused
 };
 
 // Pair geometry from the row's coordinates xi, ti and the column's q (D coordinates, then the
@@ -84,10 +88,11 @@ __device__ __forceinline__ void pair_geometry(const double (&xi)[D], double ti, 
 
 // Background term of one pair (geometry from pair_geometry).
 template <bool F32, bool FAST>
-__device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt2, double y) {
+__device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt2, double y,
+                                                  const double *__restrict__ tab) {
     if (FAST) {
         if (F32) return (double)exp2_f32((float)(-y));
-        return exp2_neg_fp64(y); // y >= 0: a sum of squares
+        return exp2_neg_tab(y, tab); // y = 32 x the exponent >= 0: a sum of squares
     }
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
 }
@@ -115,7 +120,10 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&tn)[A], const double (&vm)[A],
-                                            const PairConst &c, double (&acc)[A][6], double *cs) {
+                                            const PairConst &c, double (&acc)[A][6], double *cs,
+                                            const double *__restrict__ tab) {
+    // tab: exp2_neg_tab's table (c.tab), restrict so that its reads may be scheduled ahead of the
+    // column-slot additions to the same LDS
     constexpr int NC = GRAD ? 3 : 1;
     // components read per column: D coordinates and the time, plus the natural time where the
     // base-2 form forms excitation terms
@@ -143,7 +151,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             double dte = dt;
             if constexpr (FAST && SMODE != kSNone) dte = tn[a] - q[D + 1];
             if (DOB) {
-                const double eb = background_term<F32, FAST>(c, r2, dt2, y);
+                const double eb = background_term<F32, FAST>(c, r2, dt2, y, tab);
                 acc[a][0] += eb;
                 if (GRAD) {
                     acc[a][1] = __builtin_fma(eb, m1, acc[a][1]);
@@ -214,7 +222,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
         double dte = dt;
         if constexpr (FAST && PM != kPassCulledB) dte = tn - q[D + 1];
         {
-            const double eb = background_term<F32, FAST>(c, r2, dt2, y);
+            const double eb = background_term<F32, FAST>(c, r2, dt2, y, c.tab);
             acc[0] += eb;
             if (GRAD) {
                 acc[1] = __builtin_fma(eb, FAST ? y : r2, acc[1]);
@@ -240,24 +248,24 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
                                              const double (&vm)[A], const PairConst &c, double (&acc)[A][6],
                                              double *cs) {
     if constexpr (PM == kPassCulledB) {
-        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         return;
     }
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     default: break; // every term of the subtile is exactly zero
     }
@@ -395,8 +403,8 @@ __device__ __forceinline__ double box_background_exponent(const double (&alo)[D 
 template <int D, bool GRAD, int A, bool F32, bool FAST, int PM, bool FLUSH>
 __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
                                          double *__restrict__ rowpart, double *__restrict__ colpart,
-                                         double *__restrict__ tile, double *__restrict__ colbuf, double *cs, int tid,
-                                         int lane, int w) {
+                                         double *__restrict__ tile, double *__restrict__ colbuf, double *cs,
+                                         const double *tab, int tid, int lane, int w) {
     constexpr int ST = DiagStride<D, FAST>::value;
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
@@ -414,8 +422,9 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     const int P0 = P * R;
 
     // base-2 form: coordinates and times relative to the block's first event, scaled by sx and st
-    const double2 st = FAST ? time_scale_dd(tauT) : make_double2(1.0, 0.0);
-    const double2 sx = FAST ? time_scale_dd(tauX) : make_double2(1.0, 0.0);
+    // (in FP64 sqrt(32) times larger: exponents times 32 for the table exponential)
+    const double2 st = FAST ? (F32 ? time_scale_dd(tauT) : time_scale32_dd(tauT)) : make_double2(1.0, 0.0);
+    const double2 sx = FAST ? (F32 ? time_scale_dd(tauX) : time_scale32_dd(tauX)) : make_double2(1.0, 0.0);
     const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
     double x0[D];
 #pragma unroll
@@ -423,16 +432,19 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     PairConst c;
     c.cS0lo = 0.0;
     c.cS1lo = 0.0;
+    c.tab = tab;
     if (FAST) {
         c.cB0 = 1.0;
         c.cB1 = 0.0;
         // -sigma^2 / tauX^2 with sigma^2 and tauX^2 rounded as the finalize rounds them, the
-        // quotient carried as a double-double; -omega log2(e) per natural time difference, the same
+        // quotient carried as a double-double (divided by 32, exactly, against the table form's
+        // r'^2); -omega log2(e) per natural time difference, the same
         const double s2 = sigma * sigma, x2 = tauX * tauX;
         const double q = s2 / x2;
         const double qlo = __builtin_fma(-q, x2, s2) / x2;
-        c.cS0 = -q;
-        c.cS0lo = -qlo;
+        const double unit = F32 ? 1.0 : 0.03125;
+        c.cS0 = -q * unit;
+        c.cS0lo = -qlo * unit;
         const double w = -omega * kLog2eHi;
         c.cS1 = w;
         c.cS1lo = __builtin_fma(-omega, kLog2eHi, -w) + (-omega * kLog2eLo);
@@ -689,13 +701,24 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     constexpr int ST = DiagStride<D, FM != 0>::value;
     constexpr int NCOMP = FM != 0 ? D + 2 : D + 1;
     constexpr int NC = GRAD ? 3 : 1;
+    constexpr bool TAB = FM != 0 && !F32; // the base-2 form in FP64: exp2_neg_tab
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x NCOMP x 128 doubles
-    __shared__ double tile[(2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST];
-    __shared__ double colbuf[kSymWaves * NC * kTile];
-    __shared__ double colslot[kSymWaves * NC * 128];
+    constexpr int kTileD = (2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST;
+    constexpr int kSlotD = kSymWaves * NC * 128;
+    constexpr int kTabD = TAB ? kExp2TabSize : 0;
+    // One LDS array, laid out by hand: the arrays the pair loop reads (subtiles, column slots,
+    // the exponential's table) first, so that their addresses fit the 16-bit offset field of the
+    // LDS instructions (no address additions in the loop); the per-wave column sums last.
+    __shared__ double lds[kTileD + kSlotD + kTabD + kSymWaves * NC * kTile];
+    double *const tile = lds;
+    double *const colslot = lds + kTileD;
+    const double *const etab = lds + kTileD + kSlotD;
+    double *const colbuf = lds + kTileD + kSlotD + kTabD;
     __shared__ int s_item;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // the table is complete before the first item: the queue loop's barrier follows
+    if (TAB) fill_exp2_table(lds + kTileD + kSlotD, tid);
     // this wave's column slots (sym_subtile), zero between subtiles
     double *cs = colslot + w * NC * 128;
 #pragma unroll
@@ -714,14 +737,14 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         const bool flush = it.z >= 0 && L.data[it.x].L > kFlushTiles * kTile;
         if (fast) {
             if (flush)
-                sym_item<D, GRAD, A, F32, true, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+                sym_item<D, GRAD, A, F32, true, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
             else
-                sym_item<D, GRAD, A, F32, true, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+                sym_item<D, GRAD, A, F32, true, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
         } else {
             if (flush)
-                sym_item<D, GRAD, A, F32, false, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+                sym_item<D, GRAD, A, F32, false, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
             else
-                sym_item<D, GRAD, A, F32, false, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, tid, lane, w);
+                sym_item<D, GRAD, A, F32, false, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
         }
     }
 }

@@ -2,7 +2,9 @@
 on the CPU: the constants of hph_exp_coeffs.h evaluated as the kernels evaluate them (the same
 reductions, Horner with correctly rounded fma, emulated at 60 digits) against mpmath at 60
 digits. exp_fp64 (Cody-Waite, natural form), exp2_fp64 (shifter, exponents of either sign) and
-exp2_neg_fp64 (2^-y for y >= 0 from trunc and fract: the background terms of the base-2 form).
+exp2_neg_fp64 (2^-y for y >= 0 from trunc and fract) and exp2_neg_tab (2^-y from 32 y: a 32-entry
+table of 2^(b/32), a degree-5 polynomial and v_ldexp_f64; the FP64 background terms of the base-2
+form).
 A dropped or mistyped coefficient, a wrong sign of the 2^-f polynomial or an off-by-one in the
 reduction moves the error far above the bound.
 """
@@ -72,6 +74,22 @@ def exp2_neg_fp64(c, y):
     return ldexp(horner(poly(c, "HPH_EXP2N_C"), f), -i)
 
 
+def exp2_table(c):
+    """fill_exp2_table: entry b = 2^(b/32)."""
+    return [c["HPH_EXP2T_M%d" % b] for b in range(32)]
+
+
+def exp2_neg_tab(c, tab, y32):
+    """2^(-y32/32): -trunc(y32) = 32 a + b, f = fract(y32), ldexp(tab[b] + tab[b] (f q(f)), a)."""
+    ni = -int(y32)  # v_cvt_i32_f64 of -y32
+    f = y32 - np.floor(y32)
+    q = c["HPH_EXP2T_C5"]
+    for k in (4, 3, 2, 1):
+        q = fma(q, f, c["HPH_EXP2T_C%d" % k])
+    t = tab[ni & 31]  # Python's & and >> on negative ints: two's complement, arithmetic shift
+    return ldexp(fma(t, float(mp.mpf(f) * mp.mpf(q)), t), ni >> 5)
+
+
 def rel(v, ref):
     return abs(float((mp.mpf(v) - ref) / ref))
 
@@ -106,6 +124,30 @@ def test_exp2_neg_fp64():
     assert exp2_neg_fp64(c, 1100.0) == 0.0
 
 
+def test_exp2_table_mantissas():
+    """The 32 table entries are 2^(b/32) correctly rounded."""
+    c = constants()
+    for b in range(32):
+        assert c["HPH_EXP2T_M%d" % b] == float(mp.power(2, mp.mpf(b) / 32)), b
+
+
+def test_exp2_neg_tab():
+    c = constants()
+    tab = exp2_table(c)
+    rng = np.random.default_rng(4)
+    ys = np.concatenate([rng.uniform(0.0, 1020.0, 300), rng.uniform(0.0, 4.0, 300),
+                         [0.0, 1.0 / 32, 1.0, 2.0, 95.0, 1e-300, 1.0 - 2.0 ** -53, 31.0 / 32, 33.0 / 32]])
+    worst = 0.0
+    for y in ys:
+        y32 = float(mp.mpf(float(y)) * 32)
+        worst = max(worst, rel(exp2_neg_tab(c, tab, y32), mp.power(2, -mp.mpf(y32) / 32)))
+    assert worst < 4e-16, worst
+    assert exp2_neg_tab(c, tab, 0.0) == 1.0  # the self pair's term is exactly 1
+    assert exp2_neg_tab(c, tab, 96.0) == 0.125  # 2^-3
+    assert exp2_neg_tab(c, tab, 32.0 * 1100) == 0.0  # below the FP64 range, as IEEE exp
+    assert exp2_neg_tab(c, tab, 2.0 ** 31 - 1) == 0.0
+
+
 # ---- double-double constants of the base-2 form (hph_device.h) ----
 
 DEVICE_H = os.path.join(HERE, "..", "hawkes_amd", "csrc", "hph_device.h")
@@ -125,6 +167,7 @@ DD_VALUES = {
     "kSqrtHalfLog2e": lambda: mp.sqrt(mp.log(mp.e, 2) / 2),
     "kTwoLn2": lambda: 2 * mp.log(2),
     "kLog2e": lambda: 1 / mp.log(2),
+    "kSqrt16Log2e": lambda: 4 * mp.sqrt(mp.log(mp.e, 2)),
 }
 
 
@@ -140,14 +183,17 @@ def test_double_double_constants():
         assert lo == float(v - mp.mpf(hi)), name
         assert abs(mp.mpf(hi) + mp.mpf(lo) - v) <= mp.mpf(2) ** -104 * abs(v), name
     assert abs(DD_VALUES["kTwoLn2"]() * DD_VALUES["kSqrtHalfLog2e"]() ** 2 - 1) < mp.mpf(10) ** -50
+    # the table form's scale: 32 times the squared scale, so 2 ln 2 / 32 undoes it
+    assert abs(DD_VALUES["kTwoLn2"]() / 32 * DD_VALUES["kSqrt16Log2e"]() ** 2 - 1) < mp.mpf(10) ** -50
 
 
-def time_scale_dd(c, tau):
-    """hph_device.h time_scale_dd: |tau| sqrt(log2(e)/2) as a double-double (two-product)."""
+def time_scale_dd(c, tau, name="kSqrtHalfLog2e"):
+    """hph_device.h time_scale_dd (time_scale32_dd with name kSqrt16Log2e): |tau| times the
+    constant as a double-double (two-product)."""
     a = abs(tau)
-    hi = float(mp.mpf(a) * mp.mpf(c["kSqrtHalfLog2eHi"]))
-    err = fma(a, c["kSqrtHalfLog2eHi"], -hi)
-    return hi, fma(a, c["kSqrtHalfLog2eLo"], err)
+    hi = float(mp.mpf(a) * mp.mpf(c[name + "Hi"]))
+    err = fma(a, c[name + "Hi"], -hi)
+    return hi, fma(a, c[name + "Lo"], err)
 
 
 def test_scaling_round_trip():
@@ -156,17 +202,21 @@ def test_scaling_round_trip():
     scale (which is what a single rounded st would leave on every term of a dataset)."""
     c = device_constants()
     rng = np.random.default_rng(11)
-    worst_scale = worst_back = 0.0
-    for tau in np.concatenate([rng.uniform(1e-4, 10.0, 200), [1 / 336.0, 1 / 1.6, 1.0]]):
-        hi, lo = time_scale_dd(c, float(tau))
-        exact = abs(mp.mpf(float(tau))) * DD_VALUES["kSqrtHalfLog2e"]()
-        worst_scale = max(worst_scale, float(abs(mp.mpf(hi) + mp.mpf(lo) - exact) / exact))
-        d = float(rng.uniform(0.1, 100.0))
-        # scale_by: fma(v, s.x, v * s.y)
-        vs = fma(d, hi, float(mp.mpf(d) * mp.mpf(lo)))
-        m = float(mp.mpf(vs) * mp.mpf(vs))
-        tau2 = float(mp.mpf(float(tau)) * mp.mpf(float(tau)))
-        back = float(mp.mpf(fma(m, c["kTwoLn2Hi"], float(mp.mpf(m) * mp.mpf(c["kTwoLn2Lo"])))) / mp.mpf(tau2))
-        worst_back = max(worst_back, float(abs(mp.mpf(back) - mp.mpf(d) ** 2) / mp.mpf(d) ** 2))
-    assert worst_scale < 1e-30, worst_scale
-    assert worst_back < 8 * 2.0 ** -53, worst_back
+    # (scale constant, unit of from_base2_sq): FP32-bulk's base-2 form and the FP64 table form
+    for name, unit in (("kSqrtHalfLog2e", 1.0), ("kSqrt16Log2e", 1.0 / 32)):
+        worst_scale = worst_back = 0.0
+        for tau in np.concatenate([rng.uniform(1e-4, 10.0, 200), [1 / 336.0, 1 / 1.6, 1.0]]):
+            hi, lo = time_scale_dd(c, float(tau), name)
+            exact = abs(mp.mpf(float(tau))) * DD_VALUES[name]()
+            worst_scale = max(worst_scale, float(abs(mp.mpf(hi) + mp.mpf(lo) - exact) / exact))
+            d = float(rng.uniform(0.1, 100.0))
+            # scale_by: fma(v, s.x, v * s.y)
+            vs = fma(d, hi, float(mp.mpf(d) * mp.mpf(lo)))
+            m = float(mp.mpf(vs) * mp.mpf(vs))
+            tau2 = float(mp.mpf(float(tau)) * mp.mpf(float(tau)))
+            # from_base2_sq(m, tau2, unit): fma(m, 2ln2_hi unit, m (2ln2_lo unit)) / tau2
+            chi, clo = c["kTwoLn2Hi"] * unit, c["kTwoLn2Lo"] * unit
+            back = float(mp.mpf(fma(m, chi, float(mp.mpf(m) * mp.mpf(clo)))) / mp.mpf(tau2))
+            worst_back = max(worst_back, float(abs(mp.mpf(back) - mp.mpf(d) ** 2) / mp.mpf(d) ** 2))
+        assert worst_scale < 1e-30, (name, worst_scale)
+        assert worst_back < 8 * 2.0 ** -53, (name, worst_back)
