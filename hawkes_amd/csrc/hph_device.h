@@ -176,6 +176,24 @@ __device__ __forceinline__ double background_exp(double cBx, double cBt, double 
     return exp_fp64(arg);
 }
 
+// Next work item of a persistent pass (the same index in every thread of the workgroup). The
+// queue counter is 0 when a launch starts and the launch returns it to 0: each workgroup fetches
+// until it receives an index >= L.items, so a launch makes exactly L.items + L.grid fetches, and
+// the workgroup that receives the last index, L.items + L.grid - 1 (every other fetch of the launch
+// precedes it on the counter), stores 0. The next launch on the stream then finds it at 0
+// without a reset kernel. L.grid must be the launched grid.
+__device__ __forceinline__ int queue_next(const SymLaunch &L, int *s_item) {
+    if (threadIdx.x == 0) {
+        const int v = atomicAdd(L.counter, 1);
+        if (v == L.items + L.grid - 1) __hip_atomic_store(L.counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_item = v;
+    }
+    __syncthreads();
+    const int idx = __builtin_amdgcn_readfirstlane(*s_item); // uniform: item data in scalar registers
+    __syncthreads();
+    return idx;
+}
+
 // Stride (doubles) of one event in an LDS tile: D coordinates then t, padded to 16 bytes.
 template <int D> struct TileStride {
     static constexpr int value = (D + 2) & ~1;

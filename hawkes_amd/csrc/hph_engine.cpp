@@ -528,8 +528,8 @@ class SymRunner {
             cost_[i] = c;
             cmax = std::max(cmax, c);
         }
-        int nbuck = 1024;
-        while (nbuck < 65536 && (size_t)nbuck < m) nbuck *= 2;
+        // 1024 cost classes (0.1% of the largest cost each; equal classes keep the plan order)
+        const int nbuck = 1024;
         count_.assign(nbuck + 1, 0);
         key_.resize(m);
         for (size_t i = 0; i < m; ++i) {

@@ -253,8 +253,8 @@ void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, co
 void launch_sym_bsums(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                       const double *colpart, double *bsum, hipStream_t s);
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
-// Zeroes the work-queue counter (a one-thread kernel, so that the sequence can be graph-captured).
-void launch_queue_reset(const SymLaunch &L, hipStream_t s);
+// The work-queue counter (SymLaunch::counter) is zero before every persistent pass and is
+// returned to zero by the pass itself (queue_next, hph_sym_device.h): no reset between launches.
 // Symmetric location-gradient pass over dataset 0, vector 0 of a plan made with grad = true
 // (D <= 3): w = 1/lambda in sorted order; out = d ll / d x, N x D in the caller's order.
 void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *w, double *rowpart,

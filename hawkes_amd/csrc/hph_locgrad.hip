@@ -265,10 +265,7 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(L.counter, 1);
-        __syncthreads();
-        const int idx = __builtin_amdgcn_readfirstlane(s_item); // uniform: item data in scalar registers
-        __syncthreads();
+        const int idx = queue_next(L, &s_item);
         if (idx >= L.items) break;
 
         const int4 it = L.item[idx];
@@ -479,7 +476,6 @@ void launch_sym_locgrad(const SymLaunch &L, const double *params, const double *
     if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
     if (L.R != kSymWaves * 64 * L.A) throw std::invalid_argument("symmetric pass: block rows do not match");
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
-    launch_queue_reset(L, s);
     switch (L.dim) {
     case 1: launch_sym_locgrad_d<1>(L, params, w, rowpart, colpart, out, s); break;
     case 2: launch_sym_locgrad_d<2>(L, params, w, rowpart, colpart, out, s); break;

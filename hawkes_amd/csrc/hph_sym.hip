@@ -9,9 +9,6 @@
 namespace hph {
 namespace kern {
 
-// Zeroes the work-queue counter before a pass (a kernel, so the pass can be captured in a graph).
-__global__ void queue_reset_kernel(int *counter) { *counter = 0; }
-
 // Row sums = row-side chunks + diagonal block + column side of every later block. One block of
 // 256 rows of one (dataset, vector) per workgroup.
 // Moment sources: SPLIT false: all six from the pass's row and column partials. SPLIT true: the
@@ -251,10 +248,6 @@ SymPlan plan_sym(const std::vector<int> &n, const std::vector<int> &K, int rows_
     return plan;
 }
 
-void launch_queue_reset(const SymLaunch &L, hipStream_t s) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, L.counter);
-}
-
 void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart, double *colpart,
                 hipStream_t s) {
     if (L.A != 1 && L.A != 2) throw std::invalid_argument("symmetric pass is built for 1 or 2 rows per lane");
@@ -262,7 +255,6 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
     if (L.grid < 1 || L.items < 1) throw std::invalid_argument("symmetric pass: empty launch");
     if (L.fast != 0 && L.b2tab != (fp32 ? 0 : 1))
         throw std::invalid_argument("symmetric pass: base-2 units do not match the precision");
-    launch_queue_reset(L, s);
     if (L.pm != kPassBoth) {
         launch_sym_split(L, params, grad, fp32, rowpart, colpart, s);
         return;

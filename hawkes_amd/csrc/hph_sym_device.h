@@ -727,10 +727,7 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         cs[c * 128 + lane + 64] = 0.0;
     }
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(L.counter, 1);
-        __syncthreads();
-        const int idx = __builtin_amdgcn_readfirstlane(s_item); // uniform: item data in scalar registers
-        __syncthreads();
+        const int idx = queue_next(L, &s_item);
         if (idx >= L.items) break;
         const int4 it = L.item[idx];
         const bool fast = FM == 1 || (FM == 2 && L.data[it.x].fast);
