@@ -450,13 +450,15 @@ class SymRunner {
     // not its results. Plan order alone (by nominal size: chunk rows x columns) puts blocks whose
     // earlier columns are all within the background's reach -- every tile computed -- among
     // blocks whose chunks are mostly skipped; the queue's tail then waits on the former.
-    // Cost of a chunk item, in (block rows x kTile)-pair tiles: the tiles that pass the
+    // Cost of a chunk item, in tenths of a (block rows x kTile)-pair tile: the tiles that pass the
     // workgroup-wide skip test of sym_item (the same test, on the block's first time), weighted
-    // 21 where the block's first row also forms excitation terms and 10 where it forms the
-    // background only; skipped tiles count 0.1 (a flag write). Diagonal items: rows^2 pairs of both
-    // terms. Both tests are monotone in the tile index (times are sorted), so each (dataset,
-    // vector, block) takes two binary searches; the order is a counting sort of the quantised
-    // costs (stable: equal costs keep the plan order).
+    // 210 where the block's first row also forms excitation terms and 100 where it forms the
+    // background only; skipped tiles count 1 (a flag write). Diagonal items: rows^2 pairs of both
+    // terms, 210 rows^2 / (R kTile). Both tests are monotone in the tile index (times are sorted),
+    // so each (dataset, vector, block) takes two binary searches; each item's tile range and block
+    // are computed once per plan (oq_*), so a call is a pass of integer arithmetic over the items
+    // and a counting sort of the integer costs (stable: equal costs keep the plan order; costs
+    // past 65,535 are first shifted down to that range).
     void order_queue(const std::vector<const double *> &ht, const double *hparams, int dim, int skip_zero, int cut_s) {
         const size_t m = plan_.items.size();
         if (ht.size() != plan_.data.size()) throw std::invalid_argument("order_queue: one time array per dataset");
@@ -509,35 +511,53 @@ class SymRunner {
                 }
             }
         }
-        cost_.resize(m);
-        double cmax = 0.0;
+        if (oq_items_ != m) {
+            // per item: its (dataset, vector, block) index into thr_ and its tile range, or for a
+            // diagonal item a = -1 and b = its cost
+            oq_q_.resize(m);
+            oq_a_.resize(m);
+            oq_b_.resize(m);
+            for (size_t i = 0; i < m; ++i) {
+                const int4 it = plan_.items[i];
+                const kern::SymData &sd = plan_.data[it.x];
+                if (it.z < 0) {
+                    const double rows = (double)std::min(R, sd.n - it.y * R);
+                    oq_q_[i] = 0;
+                    oq_a_[i] = -1;
+                    oq_b_[i] = (int)(210.0 * rows * rows / ((double)R * T) + 0.5);
+                } else {
+                    oq_q_[i] = (int)(base[it.x] + (size_t)it.w * sd.nb + it.y);
+                    oq_a_[i] = it.z * sd.L / T;
+                    oq_b_[i] = std::min((it.z + 1) * sd.L, it.y * R) / T;
+                }
+            }
+            oq_items_ = m;
+        }
+        key_.resize(m);
+        int cmax = 0;
         for (size_t i = 0; i < m; ++i) {
-            const int4 it = plan_.items[i];
-            const kern::SymData &sd = plan_.data[it.x];
-            double c;
-            if (it.z < 0) {
-                const double rows = (double)std::min(R, sd.n - it.y * R);
-                c = 21.0 * rows * rows / ((double)R * T);
+            int c;
+            const int a = oq_a_[i], b = oq_b_[i];
+            if (a < 0) {
+                c = b;
             } else {
-                const size_t q = base[it.x] + (size_t)it.w * sd.nb + it.y;
-                const int a = it.z * sd.L / T, b = std::min((it.z + 1) * sd.L, it.y * R) / T;
+                const int q = oq_q_[i];
                 const int j1 = std::min(std::max(thr_[2 * q], a), b);      // first computed tile
                 const int j2 = std::min(std::max(thr_[2 * q + 1], j1), b); // first with excitation
-                c = 0.1 * (j1 - a) + 10.0 * (j2 - j1) + 21.0 * (b - j2);
+                c = (j1 - a) + 100 * (j2 - j1) + 210 * (b - j2);
             }
-            cost_[i] = c;
+            key_[i] = c;
             cmax = std::max(cmax, c);
         }
-        // 1024 cost classes (0.1% of the largest cost each; equal classes keep the plan order)
-        const int nbuck = 1024;
-        count_.assign(nbuck + 1, 0);
-        key_.resize(m);
+        int shift = 0;
+        while ((cmax >> shift) > 65535) ++shift;
+        const int top = cmax >> shift;
+        count_.assign(top + 2, 0);
         for (size_t i = 0; i < m; ++i) {
-            const int key = cmax > 0.0 ? (int)(cost_[i] / cmax * (nbuck - 1) + 0.5) : 0;
-            key_[i] = nbuck - 1 - key; // ascending key = descending cost
+            key_[i] = top - (key_[i] >> shift); // ascending key = descending cost
             ++count_[key_[i] + 1];
         }
-        for (int b = 0; b < nbuck; ++b) count_[b + 1] += count_[b];
+        for (int b = 0; b <= top; ++b) count_[b + 1] += count_[b];
         int4 *out = h_items_.as<int4>();
         for (size_t i = 0; i < m; ++i) out[count_[key_[i]]++] = plan_.items[i];
     }
@@ -659,6 +679,7 @@ class SymRunner {
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
         if (!same) {
             plan_ = kern::plan_sym(n, K, rows_per_lane, target_items, grad);
+            oq_items_ = 0;
             n_ = n;
             K_ = K;
             A_ = rows_per_lane;
@@ -755,6 +776,8 @@ class SymRunner {
     HostBuf h_items_;
     std::vector<int> thr_, count_, key_;
     std::vector<double> cost_;
+    std::vector<int> oq_q_, oq_a_, oq_b_; // per plan item (order_queue); valid for oq_items_ items
+    size_t oq_items_ = 0;                 // of the current plan (0 after a new plan)
     // order_queue_culled: the last order, the plan items and the (tauX, tauT) per vector it was
     // computed for
     std::vector<int4> culled_order_, culled_items_;
