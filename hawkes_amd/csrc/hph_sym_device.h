@@ -122,8 +122,8 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                                             const double (&ti)[A], const double (&tn)[A], const double (&vm)[A],
                                             const PairConst &c, double (&acc)[A][6], double *cs,
                                             const double *__restrict__ tab) {
-    // tab: exp2_neg_tab's table (c.tab), restrict so that its reads may be scheduled ahead of the
-    // column-slot additions to the same LDS
+    // tab: exp2_neg_tab's table (c.tab); restrict: the column-slot additions to the same LDS
+    // array never touch it
     constexpr int NC = GRAD ? 3 : 1;
     // components read per column: D coordinates and the time, plus the natural time where the
     // base-2 form forms excitation terms
