@@ -151,16 +151,22 @@ def kernel_name(cfg):
     return "pair_kernel<2,true> (target grid %d)" % target
 
 
-def load_traffic(kind):
+def load_traffic(kind, cfg):
     """HBM bytes per pair-kernel launch from the committed rocprofv3 --pmc summary, if present and
-    measured on the pass that ran (kind: 'sym' or 'pair')."""
+    measured on the pass that ran (kind: 'sym' or 'pair') at the configuration that ran (cfg:
+    tuned_config(); the record lists one measurement per target grid)."""
     path = os.path.join(ROOT, "profiles", "pair_kernel_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    return rec.get("bytes_per_launch") if rec.get("kernel", "").startswith(kind + "_kernel") else None
+    if not rec.get("kernel", "").startswith(kind + "_kernel"):
+        return None
+    for m in rec.get("measurements", []):
+        if m.get("rows_per_lane") == cfg[1] and m.get("target_grid") == cfg[2]:
+            return m.get("bytes_per_launch")
+    return None
 
 
 class _Ranges:
@@ -715,9 +721,10 @@ def main():
         roofline = {
             "bound": "valu",
             "achieved": None, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": None,
-            "traffic": load_traffic("sym" if kind == _lib.PAIR_KERNEL_SYMMETRIC else "pair"),
+            "traffic": load_traffic("sym" if kind == _lib.PAIR_KERNEL_SYMMETRIC else "pair", e.tuned_config()),
             "traffic_source": "not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
-                              "kernel and workload, calibrated by a plain copy, in profiles/pair_kernel_traffic.json",
+                              "kernel, workload and configuration, calibrated by a plain copy, in "
+                              "profiles/pair_kernel_traffic.json (null when no measurement matches)",
             "kernel": kernel_name(e.tuned_config()),
             "pass": {"kind": kind, "base2_form": base2},
             "kernel_ms": pair_avg_ms,
