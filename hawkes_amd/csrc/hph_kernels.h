@@ -194,8 +194,10 @@ struct SymLaunch {
                             // t nb + P: 1 when the pass wrote the tile's column partials, 0 when it
                             // skipped the tile (its partials are then zero and left unwritten);
                             // readers skip unflagged tiles. After them, the row-slot flags
-                            // (SymData::rflag0): 1 when a chunk item wrote its row partials, 0
-                            // when the pass skipped every tile of the item (partials unwritten)
+                            // (SymData::rflag0): 2 when a chunk item wrote all its row partials,
+                            // 1 when it wrote the background moments only (it formed no
+                            // excitation term: those moments are zero and unwritten), 0 when the
+                            // pass skipped every tile of the item (partials unwritten)
     int bsum_shared;        // finalize from per-event sums: 1 when bsum holds one set of background
                             // moments for every vector (vectors sharing tauX and tauT), 0 one per vector
     int b2tab;              // 1: the base-2 form ran in FP64 with exponents scaled by 32 (the table

@@ -282,7 +282,9 @@ constexpr int kQChunk = 1024;
 // Lists the blocks Q in [qa, qb) (qb - qa <= kQChunk) whose flag is set into list (ascending)
 // and returns their count. ftile: the flags of column tile t, one byte per block Q (the
 // [tile][block] layout of SymLaunch::colflag). Called by every thread of the block; wave 0 builds
-// the list. The caller must barrier before the list is rebuilt.
+// the list. The caller must barrier before the list is rebuilt. An entry is Q, with bit
+// kListFull set where the flag is 2 (row slots whose excitation moments were written, sym_item).
+constexpr int kListFull = 1 << 30;
 __device__ __forceinline__ int flagged_blocks(const unsigned char *__restrict__ ftile, int qa, int qb, int *list,
                                               int *count) {
     if (threadIdx.x < 64) {
@@ -290,9 +292,10 @@ __device__ __forceinline__ int flagged_blocks(const unsigned char *__restrict__ 
         const unsigned long long below = (1ULL << threadIdx.x) - 1ULL;
         for (int q0 = qa; q0 < qb; q0 += 64) {
             const int Q = q0 + (int)threadIdx.x;
-            const bool f = Q < qb && ftile[Q] != 0;
+            const int fl = Q < qb ? ftile[Q] : 0;
+            const bool f = fl != 0;
             const unsigned long long m = __ballot(f);
-            if (f) list[base + __popcll(m & below)] = Q;
+            if (f) list[base + __popcll(m & below)] = fl >= 2 ? (Q | kListFull) : Q;
             base += __popcll(m);
         }
         if (threadIdx.x == 0) *count = base;
@@ -307,7 +310,7 @@ __device__ __forceinline__ void add_listed_columns(const double *__restrict__ cb
                                                    int cnt, double (&cs)[NC]) {
 #pragma unroll 4
     for (int j = 0; j < cnt; ++j) {
-        const int Q = list[j];
+        const int Q = list[j] & (kListFull - 1); // column flags are 0 or 1: no kListFull bit
         const double *cp = cbase + (size_t)NC * R * ((size_t)Q * (Q - 1) / 2);
         const size_t Q0 = (size_t)Q * R;
 #pragma unroll
@@ -331,9 +334,11 @@ __device__ __forceinline__ void add_remaining_columns(const double *__restrict__
 
 // Row side of the symmetric pass, read back for row r of block P: adds the first NQ moments of
 // the row partials of the chunk slots c < nch whose flag rf[c] is set (listed kQChunk at a time,
-// ascending) and of the diagonal slot max_chunks into a. NR moments per slot, slots rpad rows
-// apart. Every thread of the block calls it (rows past n: live false, barriers only); list and
-// count are LDS scratch of the caller's own.
+// ascending) and of the diagonal slot max_chunks into a. NR moments per slot (the first NR / 2
+// of the background, then as many of the excitation), slots rpad rows apart. A chunk slot with
+// flag 1 holds its background moments only: its excitation moments were exactly zero and were
+// not written (adding them would change no bit). Every thread of the block calls it (rows past
+// n: live false, barriers only); list and count are LDS scratch of the caller's own.
 template <int NR, int NQ>
 __device__ __forceinline__ void add_row_slots(const double *__restrict__ rbase, const unsigned char *__restrict__ rf,
                                               int nch, int max_chunks, size_t rpad, int r, bool live,
@@ -345,9 +350,12 @@ __device__ __forceinline__ void add_row_slots(const double *__restrict__ rbase, 
             // unrolled so that several slots' loads are in flight
 #pragma unroll 4
             for (int u = 0; u < cnt; ++u) {
-                const double *rp = rbase + (size_t)list[u] * NR * rpad;
+                const int e = list[u];
+                const double *rp = rbase + (size_t)(e & (kListFull - 1)) * NR * rpad;
+                const bool full = (e & kListFull) != 0;
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) a[q] += rp[(size_t)q * rpad + r];
+                for (int q = 0; q < NQ; ++q)
+                    if (q < NR / 2 || full) a[q] += rp[(size_t)q * rpad + r];
             }
         }
     }
@@ -485,6 +493,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         for (int q = 0; q < 6; ++q) acc[a][q] = 0.0;
 
     int slot;
+    bool full = true; // every moment of the row partials is written (diagonal items always)
     if (chunk < 0) {
         slot = ds.max_chunks;
         const int c1 = min(P0 + R, n);
@@ -525,6 +534,16 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             group_box<D>(ds.gstat, wrow0 / kGroup, A, wlo, whi);
         }
         bool wrote = false; // a tile of the item passed the workgroup-level skip tests (uniform)
+        // Whether any wave can form an excitation term in the item: the per-wave test below skips
+        // them for a tile when omega times the gap between the wave's first row and the tile's
+        // last column passes the cut; the chunk's last tile and the block's first row (wave 0)
+        // give the smallest gap, so when that pair passes the cut every wave skips every tile.
+        // Otherwise the excitation moments are written, zeros included.
+        full = PM != kPassCulledB;
+        if (full && L.skip_zero) {
+            const double tl = et[c1 - 1];
+            if (tl < t_blk && gs >= 0.0 && gs * (t_blk - tl) >= zs) full = false;
+        }
         const int box_tile0 = culled_groups(n), box_super0 = box_tile0 + culled_groups(n) * kGroup / kTile;
         // the rows' sums over kFlushTiles computed tiles are formed apart (tacc) and then added to
         // the item's (acc): a row's sum over a chunk of L columns grows its rounding as
@@ -668,9 +687,10 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 for (int q = 0; q < 6; ++q) acc[a][q] += tacc[a][q];
         }
         // the row-slot flag: an item whose every tile was skipped leaves its (zero) row partials
-        // unwritten
+        // unwritten (0); one that could form no excitation term writes its background moments
+        // only (1: the excitation moments are exactly zero); else all of them (2)
         unsigned char *rf = L.colflag + ds.rflag0 + (size_t)k * ds.rflag_kstride + (size_t)P * ds.max_chunks + chunk;
-        if (tid == 0) *rf = wrote ? 1 : 0;
+        if (tid == 0) *rf = wrote ? (full ? 2 : 1) : 0;
         if (!wrote) return;
     }
     const size_t rpad = (size_t)ds.nb * R;
@@ -680,10 +700,14 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         const int r = wrow0 + a * 64 + lane;
         if (GRAD) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
+            for (int q = 0; q < 3; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
+            if (full) {
+#pragma unroll
+                for (int q = 3; q < 6; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
+            }
         } else {
             rp[r] = acc[a][0];
-            rp[rpad + r] = acc[a][3];
+            if (full) rp[rpad + r] = acc[a][3];
         }
     }
 }
@@ -692,24 +716,39 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 // every item in the natural form; 1: every item in the base-2 form; 2: each item in the base-2
 // form where its dataset allows it (SymData::fast), else natural (both paths in one kernel, at
 // some register cost, so it is used only for mixed dataset tables).
+// LDS of sym_kernel in doubles (below): subtiles or the diagonal tile, column slots, the
+// exponential's table (base-2 form in FP64), per-wave column sums.
+template <int D, bool GRAD, bool F32, int FM> struct SymLds {
+    static constexpr int NCOMP = FM != 0 ? D + 2 : D + 1;
+    static constexpr int ST = DiagStride<D, FM != 0>::value;
+    static constexpr int NC = GRAD ? 3 : 1;
+    static constexpr int tile = (2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST;
+    static constexpr int slot = kSymWaves * NC * 128;
+    static constexpr int tab = (FM != 0 && !F32) ? kExp2TabSize : 0;
+    static constexpr int total = tile + slot + tab + kSymWaves * NC * kTile;
+    // workgroups of kSymWaves waves that fit one CU's 160 KB, at most 3 (waves per SIMD); 1 KB
+    // per workgroup left for the compiler's own (the queue index, __syncthreads_or)
+    static constexpr int waves = (3 * (8 * total + 1024) <= 160 * 1024) ? 3 : 2;
+};
+
 // Three waves per SIMD (at most 168 VGPRs): measured slightly faster than the two that the
-// unconstrained allocation (up to 228 VGPRs for 2 rows per lane) leaves room for.
+// unconstrained allocation (up to 228 VGPRs for 2 rows per lane) leaves room for. Where the LDS
+// allows two workgroups per CU only (D = 3 in the base-2 form), two waves per SIMD (256 VGPRs).
 template <int D, bool GRAD, int A, bool F32, int FM, int PM>
-__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(FM == 2 ? 2 : 3, 3))) void sym_kernel(
+__global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
+    (FM == 2 || SymLds<D, GRAD, F32, FM>::waves == 2) ? 2 : 3, SymLds<D, GRAD, F32, FM>::waves))) void sym_kernel(
     SymLaunch L, const double *__restrict__ params, double *__restrict__ rowpart, double *__restrict__ colpart) {
-    // the base-2 form (FM 1, or 2 for some datasets) stages the natural time beside the scaled one
-    constexpr int ST = DiagStride<D, FM != 0>::value;
-    constexpr int NCOMP = FM != 0 ? D + 2 : D + 1;
+    using Lds = SymLds<D, GRAD, F32, FM>;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr bool TAB = FM != 0 && !F32; // the base-2 form in FP64: exp2_neg_tab
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x NCOMP x 128 doubles
-    constexpr int kTileD = (2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST;
-    constexpr int kSlotD = kSymWaves * NC * 128;
-    constexpr int kTabD = TAB ? kExp2TabSize : 0;
+    constexpr int kTileD = Lds::tile;
+    constexpr int kSlotD = Lds::slot;
+    constexpr int kTabD = Lds::tab;
     // One LDS array, laid out by hand: the arrays the pair loop reads (subtiles, column slots,
     // the exponential's table) first, so that their addresses fit the 16-bit offset field of the
     // LDS instructions (no address additions in the loop); the per-wave column sums last.
-    __shared__ double lds[kTileD + kSlotD + kTabD + kSymWaves * NC * kTile];
+    __shared__ double lds[Lds::total];
     double *const tile = lds;
     double *const colslot = lds + kTileD;
     const double *const etab = lds + kTileD + kSlotD;
