@@ -589,6 +589,17 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 }
             }
             __syncthreads();
+            // culled order: which of the tile's four 64-column groups lie past the negligible
+            // point for this wave's rows (bit sb; wave-uniform). Lane l tests group l mod 4; the
+            // box loads are issued with the tile's and complete under the same barrier.
+            int farmask = 0;
+            static_assert(kGroup == 64, "a box group is one 64-column subtile");
+            if (PM == kPassCulledB && L.skip_zero) {
+                double slo[D + 1], shi[D + 1];
+                group_box<D>(ds.gstat, j0 / kGroup + (lane & 3), 1, slo, shi);
+                const bool far = box_background_exponent<D>(wlo, whi, slo, shi, hx, ht) >= ds.zero_arg_b;
+                farmask = (int)(__ballot(far && lane < 4) & 0xFULL);
+            }
             {
                 // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as NCOMP arrays of 128
                 // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
@@ -625,12 +636,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 const int mode0 = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                 for (int sb = 0; sb < 4; ++sb) {
                     int mode = mode0;
-                    if (PM == kPassCulledB && L.skip_zero) {
-                        // this wave's rows against the subtile's 64 columns
-                        double slo[D + 1], shi[D + 1];
-                        group_box<D>(ds.gstat, (j0 + sb * 64) / kGroup, 1, slo, shi);
-                        if (box_background_exponent<D>(wlo, whi, slo, shi, hx, ht) >= ds.zero_arg_b) mode = 0;
-                    }
+                    // culled order: this wave's rows against the subtile's 64 columns
+                    if (PM == kPassCulledB && L.skip_zero && ((farmask >> sb) & 1)) mode = 0;
                     const double *sub = tile + sb * NCOMP * 128;
                     if constexpr (FLUSH) {
                         if (tail)
