@@ -271,6 +271,34 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
     }
 }
 
+// The culled pass's subtile against one of a lane's two rows (row 1 where s, else row 0; s is
+// wave-uniform): the row's registers are moved into a one-row call and its sums moved back, so the
+// other row's sums are left as they were.
+template <int D, bool GRAD, bool F32, bool FAST, int PM>
+__device__ __forceinline__ void sym_one_row(bool s, bool tail, int mode, const double *__restrict__ sub, int lane,
+                                            const double (&xi)[2][D], const double (&ti)[2], const double (&tn)[2],
+                                            const double (&vm)[2], const PairConst &c, double (&acc)[2][6],
+                                            double *cs) {
+    constexpr int NB = GRAD ? 3 : 1; // the background moments: the only sums this pass forms
+    double x1[1][D], t1[1], n1[1], v1[1], a1[1][6];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x1[0][d] = s ? xi[1][d] : xi[0][d];
+    t1[0] = s ? ti[1] : ti[0];
+    n1[0] = s ? tn[1] : tn[0];
+    v1[0] = s ? vm[1] : vm[0];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a1[0][q] = q < NB ? (s ? acc[1][q] : acc[0][q]) : 0.0;
+    if (tail)
+        sym_dispatch<D, GRAD, 1, true, F32, FAST, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
+    else
+        sym_dispatch<D, GRAD, 1, false, F32, FAST, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        acc[0][q] = s ? acc[0][q] : a1[0][q];
+        acc[1][q] = s ? a1[0][q] : acc[1][q];
+    }
+}
+
 // Computed column tiles per partial row sum of a chunk item (sym_item).
 constexpr int kFlushTiles = 4;
 
@@ -469,13 +497,19 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     const double zs = L.cut_s ? excitation_skip_arg(pk, D, n) : L.zero_arg_s;
     const double gb = 0.5 * tauT * tauT; // background exponent per gap^2
     const double gs = omega;             // excitation exponent per gap
-    const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + w * 64 * A);
+    // the lane's row a lies in 64-row group row_group(a) of the block: the wave's A groups are
+    // consecutive, or in the culled order dealt round the waves (group a kSymWaves + w), since
+    // groups near a column tile cluster along the curve and the tile's work is then spread over
+    // the waves (the workgroup waits for its slowest wave at every tile)
+    constexpr bool DEAL = PM == kPassCulledB;
+    auto row_group = [w](int a) { return DEAL ? a * kSymWaves + w : w * A + a; };
+    const int wrow0 = __builtin_amdgcn_readfirstlane(P0 + 64 * row_group(0)); // the wave's first row
     // tn: the rows' natural times (skip decisions, excitation terms); ti: their times in the
     // form's units (base 2: relative to the block's first event)
     double xi[A][D], ti[A], tn[A], vm[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-        const int r = wrow0 + a * 64 + lane;
+        const int r = P0 + 64 * row_group(a) + lane;
         const int rc = min(r, n - 1);
 #pragma unroll
         for (int d = 0; d < D; ++d) xi[a][d] = FAST ? scale_by(ex[(size_t)d * npad + rc] - x0[d], sx) : ex[(size_t)d * npad + rc];
@@ -526,12 +560,13 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         unsigned char *cf = L.colflag + ds.flag0 + (size_t)k * ds.flag_kstride + P;
         // the block's first row: the smallest time gap to any earlier tile is its gap
         const double t_blk = et[min(P0, n - 1)];
-        // culled order: bounding boxes of the block's rows and of this wave's rows
-        double blo[D + 1], bhi[D + 1], wlo[D + 1], whi[D + 1];
+        // culled order: bounding boxes of the block's rows and of the row group this lane tests
+        // (lane l < 4 A: the wave's row group l / 4; empty groups past n hold empty boxes)
+        double blo[D + 1], bhi[D + 1], glo[D + 1], ghi[D + 1];
         const double hx = 0.5 * tauX * tauX, ht = 0.5 * tauT * tauT;
         if (PM == kPassCulledB) {
             group_box<D>(ds.gstat, P0 / kGroup, R / kGroup, blo, bhi);
-            group_box<D>(ds.gstat, wrow0 / kGroup, A, wlo, whi);
+            group_box<D>(ds.gstat, P0 / kGroup + row_group((lane >> 2) & (A - 1)), 1, glo, ghi);
         }
         bool wrote = false; // a tile of the item passed the workgroup-level skip tests (uniform)
         // Whether any wave can form an excitation term in the item: the per-wave test below skips
@@ -590,15 +625,16 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             }
             __syncthreads();
             // culled order: which of the tile's four 64-column groups lie past the negligible
-            // point for this wave's rows (bit sb; wave-uniform). Lane l tests group l mod 4; the
+            // point for which of the wave's row groups (bit 4 a + sb: row group a, subtile sb;
+            // wave-uniform). Lane l < 4 A tests row group l / 4 against column group l mod 4; the
             // box loads are issued with the tile's and complete under the same barrier.
             int farmask = 0;
             static_assert(kGroup == 64, "a box group is one 64-column subtile");
             if (PM == kPassCulledB && L.skip_zero) {
                 double slo[D + 1], shi[D + 1];
                 group_box<D>(ds.gstat, j0 / kGroup + (lane & 3), 1, slo, shi);
-                const bool far = box_background_exponent<D>(wlo, whi, slo, shi, hx, ht) >= ds.zero_arg_b;
-                farmask = (int)(__ballot(far && lane < 4) & 0xFULL);
+                const bool far = box_background_exponent<D>(glo, ghi, slo, shi, hx, ht) >= ds.zero_arg_b;
+                farmask = (int)(__ballot(far && lane < 4 * A) & ((1ULL << (4 * A)) - 1ULL));
             }
             {
                 // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as NCOMP arrays of 128
@@ -636,19 +672,27 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 const int mode0 = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                 for (int sb = 0; sb < 4; ++sb) {
                     int mode = mode0;
-                    // culled order: this wave's rows against the subtile's 64 columns
-                    if (PM == kPassCulledB && L.skip_zero && ((farmask >> sb) & 1)) mode = 0;
                     const double *sub = tile + sb * NCOMP * 128;
-                    if constexpr (FLUSH) {
+                    auto &ac = FLUSH ? tacc : acc;
+                    if constexpr (PM == kPassCulledB && A == 2) {
+                        // culled order: the subtile against the wave's row groups that are near it
+                        // (both: one two-row call; one: a one-row call on that row's registers)
+                        const bool near0 = !L.skip_zero || !((farmask >> sb) & 1);
+                        const bool near1 = !L.skip_zero || !((farmask >> (4 + sb)) & 1);
+                        if (near0 != near1) {
+                            sym_one_row<D, GRAD, F32, FAST, PM>(near1, tail, mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
+                            mode = 0;
+                        } else if (!near0) {
+                            mode = 0;
+                        }
+                    } else if (PM == kPassCulledB && L.skip_zero && ((farmask >> sb) & 1)) {
+                        mode = 0; // one row group per wave: the subtile is far from it
+                    }
+                    if (mode != 0) {
                         if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, tacc, cs);
+                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
                         else
-                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, tacc, cs);
-                    } else {
-                        if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, acc, cs);
-                        else
-                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, acc, cs);
+                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
                     }
                     // column lane of the subtile: its slots before and after the wrap; the slots
                     // are zeroed again for the next subtile (the wave's LDS operations take effect
@@ -704,7 +748,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     double *rp = rowpart + ds.row0 + (size_t)k * ds.row_kstride + (size_t)slot * NR * rpad;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
-        const int r = wrow0 + a * 64 + lane;
+        const int r = P0 + 64 * row_group(a) + lane;
         if (GRAD) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) rp[(size_t)q * rpad + r] = acc[a][q];
