@@ -112,16 +112,43 @@ __device__ __forceinline__ double exp2_neg_tab(double y32, const double *tab) {
     const double t = tab[ni & 31];
     return __builtin_amdgcn_ldexp(__builtin_fma(t, f * q, t), ni >> 5);
 }
-// Fills exp2_neg_tab's table (LDS): entry b = 2^(b/32) correctly rounded (gen_exp_poly.py).
-// Threads tid < 32 of the workgroup write it; the caller barriers before use.
-__device__ __forceinline__ void fill_exp2_table(double *tab, int tid) {
+// exp2_neg_tab where every argument keeps the result in the normal range with margin
+// (0 <= y32 <= kExp2KTabMaxY32, checked by the caller for all pairs of a launch): the power 2^k,
+// k = ni >> 5, goes into the table entry's exponent field by one integer add instead of a shift
+// and an ldexp after the fma. ni = 32 k + b, so adding ni << 15 to the high word adds k << 20 (the
+// exponent) and b << 15; the table (fill_exp2_table, ktab) holds 2^(b/32) with b << 15 taken off
+// its high word, so the sum is exactly the bit pattern of 2^(b/32) 2^k. With k >= -1011 every
+// t 2^k and the fma's result are normal doubles, where scaling by 2^k commutes with rounding: the
+// result is bitwise exp2_neg_tab's.
+constexpr double kExp2KTabMaxY32 = 32352.0; // 32 x 1011: k = ni >> 5 >= -1011
+__device__ __forceinline__ double exp2_neg_ktab(double y32, const double *tab) {
+    const int ni = (int)(-y32); // -trunc(y32)
+    const double f = __builtin_amdgcn_fract(y32);
+    double q = HPH_EXP2T_C5;
+    q = __builtin_fma(q, f, HPH_EXP2T_C4);
+    q = __builtin_fma(q, f, HPH_EXP2T_C3);
+    q = __builtin_fma(q, f, HPH_EXP2T_C2);
+    q = __builtin_fma(q, f, HPH_EXP2T_C1);
+    uint2 w = __builtin_bit_cast(uint2, tab[ni & 31]);
+    w.y += (unsigned)ni << 15;
+    const double t = __builtin_bit_cast(double, w);
+    return __builtin_fma(t, f * q, t);
+}
+// Fills exp2_neg_tab's table (LDS): entry b = 2^(b/32) correctly rounded (gen_exp_poly.py); with
+// ktab, exp2_neg_ktab's: the same entry with b << 15 taken off its high word (a bit pattern, not
+// used as a number). Threads tid < 32 of the workgroup write it; the caller barriers before use.
+__device__ __forceinline__ void fill_exp2_table(double *tab, int tid, bool ktab = false) {
     constexpr double M[kExp2TabSize] = {
         HPH_EXP2T_M0,  HPH_EXP2T_M1,  HPH_EXP2T_M2,  HPH_EXP2T_M3,  HPH_EXP2T_M4,  HPH_EXP2T_M5,  HPH_EXP2T_M6,
         HPH_EXP2T_M7,  HPH_EXP2T_M8,  HPH_EXP2T_M9,  HPH_EXP2T_M10, HPH_EXP2T_M11, HPH_EXP2T_M12, HPH_EXP2T_M13,
         HPH_EXP2T_M14, HPH_EXP2T_M15, HPH_EXP2T_M16, HPH_EXP2T_M17, HPH_EXP2T_M18, HPH_EXP2T_M19, HPH_EXP2T_M20,
         HPH_EXP2T_M21, HPH_EXP2T_M22, HPH_EXP2T_M23, HPH_EXP2T_M24, HPH_EXP2T_M25, HPH_EXP2T_M26, HPH_EXP2T_M27,
         HPH_EXP2T_M28, HPH_EXP2T_M29, HPH_EXP2T_M30, HPH_EXP2T_M31};
-    if (tid < kExp2TabSize) tab[tid] = M[tid];
+    if (tid < kExp2TabSize) {
+        uint2 w = __builtin_bit_cast(uint2, M[tid]);
+        if (ktab) w.y -= (unsigned)tid << 15;
+        tab[tid] = __builtin_bit_cast(double, w);
+    }
 }
 
 // Scaled time of the base-2 pair form: t' = (t - t0) * st with st = |tauT| sqrt(log2(e) / 2), so

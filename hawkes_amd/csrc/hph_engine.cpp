@@ -170,6 +170,21 @@ static bool fast_ok(const Extents &e, const double *params, int K, bool split = 
     return true;
 }
 
+// True when every background exponent tauX^2 r^2/2 + tauT^2 dt^2/2 that the K vectors can form
+// over the data's extents is at most 700: the FP64 base-2 time-sorted pass may then run
+// exp2_neg_ktab, whose argument (32 log2(e) times the exponent, formed from the scaled values)
+// must stay below kern::kExp2KTabMaxY32 = 32 x 1011; 700 gives 32,316, the rest a margin far
+// beyond the scaled values' rounding. (The split passes' background pass spans every window of
+// the culled order and keeps exp2_neg_tab.)
+static bool ktab_ok(const Extents &e, const double *params, int K) {
+    for (int k = 0; k < K; ++k) {
+        const double *p = params + 6 * (size_t)k;
+        const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
+        if (!(aB <= 700.0)) return false;
+    }
+    return true;
+}
+
 // Split passes (chosen automatically for likelihood passes): from kSplitMinEvents events on, when
 // the locations' diagonal is at least kSplitMinRatio times the negligible-background distance
 // (counted on cluster processes with windows of 32,768: ratio 3.6 forms 0.54x the background
@@ -705,7 +720,8 @@ class SymRunner {
 
   private:
     // Plans (if the shape changed), uploads the tables (if they changed) and fills the launch.
-    // fast[d]: dataset d runs the base-2 form.
+    // fast[d]: dataset d runs the base-2 form (1), or the base-2 form with every background
+    // exponent in exp2_neg_ktab's range (2, ktab_ok; used in FP64 when every dataset has it).
     kern::SymLaunch prepare(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, bool grad,
                             bool fp32, int rows_per_lane, int target_items, int skip_zero,
                             const std::vector<char> &fast, bool partials = true) {
@@ -793,6 +809,9 @@ class SymRunner {
         for (char f : fast) nfast += f ? 1 : 0;
         L.fast = nfast == 0 ? 0 : (nfast == (int)fast.size() ? 1 : 2);
         L.b2tab = fp32 ? 0 : 1;
+        int nktab = 0;
+        for (char f : fast) nktab += f == 2 ? 1 : 0;
+        L.ktab = !fp32 && L.fast == 1 && nktab == (int)fast.size() ? 1 : 0;
         L.fin_item = d_fin_.as<int4>();
         L.fin_items = (int)plan_.fin_items.size();
         L.res_item = d_res_.as<int2>();
@@ -1243,6 +1262,7 @@ class HipHawkes : public HawkesEngine {
         out[0] = last_kind_;
         out[1] = last_fast_ ? 1 : 0;
         if (length >= 3) out[2] = last_kind_ == HPH_PAIR_KERNEL_ROWWISE ? last_k_ : last_sub_batch_;
+        if (length >= 4) out[3] = last_ktab_ ? 1 : 0;
     }
 
     // The six per-event sums of the current parameters' last single-vector evaluation with the
@@ -1453,6 +1473,7 @@ class HipHawkes : public HawkesEngine {
             const bool split = cfg.kind == HPH_PAIR_KERNEL_SPLIT ||
                                (kernel_ == HPH_PAIR_KERNEL_AUTO && skip_zero && split_pays(params, K));
             last_fast_ = !force_natural_ && fast_ok(extents_, params, K, split);
+            last_ktab_ = last_fast_ && !split && !fp32_ && ktab_ok(extents_, params, K);
             if (split && !culled_valid_) build_culled();
             // vectors [k0, k0 + kk) per launch (one launch unless the partial sums of all K do not fit;
             // probabilities, 1/lambda and the cached sums belong to single-vector calls, never split);
@@ -1477,7 +1498,8 @@ class HipHawkes : public HawkesEngine {
                 } else {
                     const std::vector<const double *> host_t{tsorted_.data()};
                     sym_.enqueue({ref}, {kk}, embeddingDimension, pdev, grad, fp32_, cfg.rows_per_lane,
-                                 cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1, {last_fast_ ? (char)1 : (char)0},
+                                 cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
+                                 {last_ktab_ ? (char)2 : last_fast_ ? (char)1 : (char)0},
                                  probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums, &host_t, phost);
                 }
             }
@@ -1487,6 +1509,7 @@ class HipHawkes : public HawkesEngine {
             last_kind_ = split ? HPH_PAIR_KERNEL_SPLIT : HPH_PAIR_KERNEL_SYMMETRIC;
         } else {
             last_fast_ = false;
+            last_ktab_ = false;
             last_split_ = false;
             last_kind_ = HPH_PAIR_KERNEL_ROWWISE;
             last_k_ = K;
@@ -1566,6 +1589,7 @@ class HipHawkes : public HawkesEngine {
         const kern::Events ev = events();
         SymRunner::DataRef ref{ev.x, ev.t, ev.perm, ev.n, ev.npad, ev.t_last};
         last_fast_ = false;
+        last_ktab_ = false;
         const double *cached = d_sums_[sums_cur_].as<double>();
         const double *scache = cached + 3 * (size_t)npad_;
         if (timing_) check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
@@ -1775,7 +1799,7 @@ class HipHawkes : public HawkesEngine {
 
     double ll_ = 0.0, grad_[6] = {0, 0, 0, 0, 0, 0};
     bool ll_valid_ = false, grad_valid_ = false, probs_valid_ = false, w_valid_ = false;
-    bool force_natural_ = false, last_fast_ = false, exact_s_ = false, last_split_ = false;
+    bool force_natural_ = false, last_fast_ = false, last_ktab_ = false, exact_s_ = false, last_split_ = false;
     int last_kind_ = 0; // pass of the last full evaluation (HPH_PAIR_KERNEL_*), 0 before the first
     std::map<long long, bool> sym_fits_;
 
@@ -2130,12 +2154,15 @@ class HipHawkesBatch {
     }
 
     // Per dataset: whether all its vectors of the current parameters allow the base-2 form (in the
-    // split passes where split[d] is set).
+    // split passes where split[d] is set): 1, or 2 where the time-sorted pass may also run
+    // exp2_neg_ktab (FP64, ktab_ok).
     std::vector<char> fast_flags(const std::vector<char> &split) const {
         std::vector<char> fast(counts_.size());
         const double *hp = params_.data();
-        for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d)
-            fast[d] = fast_ok(ext_[d], hp + 6 * k0, K_[d], split[d] != 0) ? 1 : 0;
+        for (size_t d = 0, k0 = 0; d < counts_.size(); k0 += (size_t)K_[d], ++d) {
+            const bool f = fast_ok(ext_[d], hp + 6 * k0, K_[d], split[d] != 0);
+            fast[d] = !f ? 0 : (!split[d] && !fp32_ && ktab_ok(ext_[d], hp + 6 * k0, K_[d])) ? 2 : 1;
+        }
         return fast;
     }
 

@@ -204,6 +204,10 @@ struct SymLaunch {
                             // exponential exp2_neg_tab); its squared moments are 32 times the
                             // unscaled ones. 0: FP32-bulk (unscaled). Set from the precision by the
                             // host; launch_sym checks that it matches.
+    int ktab;               // 1: every background exponent of every pair of the launch lies in
+                            // exp2_neg_ktab's range (host bound from the data's extents and each
+                            // vector's tauX, tauT; FP64 base-2 form, time-sorted pass only): the
+                            // pass runs exp2_neg_ktab (bitwise exp2_neg_tab's results)
 };
 
 // Pass modes of the symmetric pass. kPassBoth: background and excitation terms of the time-sorted

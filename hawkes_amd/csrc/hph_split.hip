@@ -11,6 +11,7 @@ namespace kern {
 void launch_sym_split(const SymLaunch &L, const double *params, bool grad, bool fp32, double *rowpart,
                       double *colpart, hipStream_t s) {
     if (L.fast != 0 && L.fast != 1) throw std::invalid_argument("split passes take one form for the whole pass");
+    if (L.ktab != 0) throw std::invalid_argument("split passes run the table exponential with its ldexp");
     if (L.fast != 0 && L.b2tab != (fp32 ? 0 : 1)) throw std::invalid_argument("base-2 units do not match the precision");
     const int v = (fp32 ? 2 : 0) + L.fast;
     if (L.pm == kPassCulledB) {

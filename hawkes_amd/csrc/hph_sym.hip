@@ -259,6 +259,11 @@ void launch_sym(const SymLaunch &L, const double *params, bool grad, bool fp32, 
         launch_sym_split(L, params, grad, fp32, rowpart, colpart, s);
         return;
     }
+    if (L.ktab != 0) {
+        if (fp32 || L.fast != 1) throw std::invalid_argument("symmetric pass: the exponent-in-table form is FP64 base-2");
+        launch_sym_p<false, 3, kPassBoth>(L, params, grad, rowpart, colpart, s);
+        return;
+    }
     switch ((fp32 ? 3 : 0) + (L.fast >= 0 && L.fast <= 2 ? L.fast : 0)) {
     case 0: launch_sym_p<false, 0, kPassBoth>(L, params, grad, rowpart, colpart, s); break;
     case 1: launch_sym_p<false, 1, kPassBoth>(L, params, grad, rowpart, colpart, s); break;

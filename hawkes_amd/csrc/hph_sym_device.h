@@ -86,13 +86,15 @@ __device__ __forceinline__ void pair_geometry(const double (&xi)[D], double ti, 
     y = FAST ? r2 + dt2 : 0.0;
 }
 
-// Background term of one pair (geometry from pair_geometry).
-template <bool F32, bool FAST>
+// Background term of one pair (geometry from pair_geometry). KT: the launch's exponents all lie
+// in exp2_neg_ktab's range (SymLaunch::ktab; the table is exp2_neg_ktab's).
+template <bool F32, bool FAST, bool KT = false>
 __device__ __forceinline__ double background_term(const PairConst &c, double r2, double dt2, double y,
                                                   const double *__restrict__ tab) {
     if (FAST) {
         if (F32) return (double)exp2_f32((float)(-y));
-        return exp2_neg_tab(y, tab); // y = 32 x the exponent >= 0: a sum of squares
+        // y = 32 x the exponent >= 0: a sum of squares
+        return KT ? exp2_neg_ktab(y, tab) : exp2_neg_tab(y, tab);
     }
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
 }
@@ -117,7 +119,7 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
 // has rows past the last event, whose column contributions are masked by vm; F32: FP32-bulk
 // background exponential; FAST: the base-2 form of PairConst.
-template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST>
+template <int D, bool GRAD, int A, bool DOB, int SMODE, bool TAIL, bool F32, bool FAST, bool KT>
 __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&tn)[A], const double (&vm)[A],
                                             const PairConst &c, double (&acc)[A][6], double *cs,
@@ -151,7 +153,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
             double dte = dt;
             if constexpr (FAST && SMODE != kSNone) dte = tn[a] - q[D + 1];
             if (DOB) {
-                const double eb = background_term<F32, FAST>(c, r2, dt2, y, tab);
+                const double eb = background_term<F32, FAST, KT>(c, r2, dt2, y, tab);
                 acc[a][0] += eb;
                 if (GRAD) {
                     acc[a][1] = __builtin_fma(eb, m1, acc[a][1]);
@@ -207,7 +209,7 @@ template <int D, bool FAST> struct DiagStride {
 // One diagonal-block tile (plain layout, broadcast reads) against one row: ordered pairs with
 // the dt > 0 mask. PM: the terms of the pass mode (kPassBoth, or kPassCulledB: background only).
 // tn: the row's natural time (the base-2 form's excitation terms).
-template <int D, bool GRAD, bool F32, bool FAST, int PM>
+template <int D, bool GRAD, bool F32, bool FAST, bool KT, int PM>
 __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int cnt, const double (&xi)[D], double ti,
                                           double tn, const PairConst &c, double (&acc)[6]) {
     constexpr int ST = DiagStride<D, FAST>::value;
@@ -222,7 +224,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
         double dte = dt;
         if constexpr (FAST && PM != kPassCulledB) dte = tn - q[D + 1];
         {
-            const double eb = background_term<F32, FAST>(c, r2, dt2, y, c.tab);
+            const double eb = background_term<F32, FAST, KT>(c, r2, dt2, y, c.tab);
             acc[0] += eb;
             if (GRAD) {
                 acc[1] = __builtin_fma(eb, FAST ? y : r2, acc[1]);
@@ -242,30 +244,30 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
 }
 
 // mode = 4 * (background formed) + excitation mode; the culled background pass builds only its own.
-template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST, int PM>
+template <int D, bool GRAD, int A, bool TAIL, bool F32, bool FAST, bool KT, int PM>
 __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&tn)[A],
                                              const double (&vm)[A], const PairConst &c, double (&acc)[A][6],
                                              double *cs) {
     if constexpr (PM == kPassCulledB) {
-        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        if (mode == 4 + kSNone) sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         return;
     }
     switch (mode) {
     case 4 + kSNone:
-        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        sym_subtile<D, GRAD, A, true, kSNone, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case 4 + kSFull:
-        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        sym_subtile<D, GRAD, A, true, kSFull, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case 4 + kSMixed:
-        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        sym_subtile<D, GRAD, A, true, kSMixed, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case kSFull:
-        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        sym_subtile<D, GRAD, A, false, kSFull, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     case kSMixed:
-        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
+        sym_subtile<D, GRAD, A, false, kSMixed, TAIL, F32, FAST, KT>(sub, lane, xi, ti, tn, vm, c, acc, cs, c.tab);
         break;
     default: break; // every term of the subtile is exactly zero
     }
@@ -274,7 +276,7 @@ __device__ __forceinline__ void sym_dispatch(int mode, const double *__restrict_
 // The culled pass's subtile against one of a lane's two rows (row 1 where s, else row 0; s is
 // wave-uniform): the row's registers are moved into a one-row call and its sums moved back, so the
 // other row's sums are left as they were.
-template <int D, bool GRAD, bool F32, bool FAST, int PM>
+template <int D, bool GRAD, bool F32, bool FAST, bool KT, int PM>
 __device__ __forceinline__ void sym_one_row(bool s, bool tail, int mode, const double *__restrict__ sub, int lane,
                                             const double (&xi)[2][D], const double (&ti)[2], const double (&tn)[2],
                                             const double (&vm)[2], const PairConst &c, double (&acc)[2][6],
@@ -289,9 +291,9 @@ __device__ __forceinline__ void sym_one_row(bool s, bool tail, int mode, const d
 #pragma unroll
     for (int q = 0; q < 6; ++q) a1[0][q] = q < NB ? (s ? acc[1][q] : acc[0][q]) : 0.0;
     if (tail)
-        sym_dispatch<D, GRAD, 1, true, F32, FAST, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
+        sym_dispatch<D, GRAD, 1, true, F32, FAST, KT, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
     else
-        sym_dispatch<D, GRAD, 1, false, F32, FAST, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
+        sym_dispatch<D, GRAD, 1, false, F32, FAST, KT, PM>(mode, sub, lane, x1, t1, n1, v1, c, a1, cs);
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         acc[0][q] = s ? acc[0][q] : a1[0][q];
@@ -436,7 +438,7 @@ __device__ __forceinline__ double box_background_exponent(const double (&alo)[D 
 // skips, events of a chunk need not be earlier than the block's). FLUSH: chunks longer than
 // kFlushTiles tiles sum their rows per group of kFlushTiles tiles (below); a separate variant,
 // because the second set of row sums costs registers that short chunks do not need.
-template <int D, bool GRAD, int A, bool F32, bool FAST, int PM, bool FLUSH>
+template <int D, bool GRAD, int A, bool F32, bool FAST, bool KT, int PM, bool FLUSH>
 __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, const double *__restrict__ params,
                                          double *__restrict__ rowpart, double *__restrict__ colpart,
                                          double *__restrict__ tile, double *__restrict__ colbuf, double *cs,
@@ -547,7 +549,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
             if (wave_live) {
 #pragma unroll
                 for (int a = 0; a < A; ++a)
-                    diag_pass<D, GRAD, F32, FAST, PM>(tile, cnt, xi[a], ti[a], tn[a], c, acc[a]);
+                    diag_pass<D, GRAD, F32, FAST, KT, PM>(tile, cnt, xi[a], ti[a], tn[a], c, acc[a]);
             }
         }
     } else {
@@ -680,7 +682,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                         const bool near0 = !L.skip_zero || !((farmask >> sb) & 1);
                         const bool near1 = !L.skip_zero || !((farmask >> (4 + sb)) & 1);
                         if (near0 != near1) {
-                            sym_one_row<D, GRAD, F32, FAST, PM>(near1, tail, mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
+                            sym_one_row<D, GRAD, F32, FAST, KT, PM>(near1, tail, mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
                             mode = 0;
                         } else if (!near0) {
                             mode = 0;
@@ -690,9 +692,9 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                     }
                     if (mode != 0) {
                         if (tail)
-                            sym_dispatch<D, GRAD, A, true, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
+                            sym_dispatch<D, GRAD, A, true, F32, FAST, KT, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
                         else
-                            sym_dispatch<D, GRAD, A, false, F32, FAST, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
+                            sym_dispatch<D, GRAD, A, false, F32, FAST, KT, PM>(mode, sub, lane, xi, ti, tn, vm, c, ac, cs);
                     }
                     // column lane of the subtile: its slots before and after the wrap; the slots
                     // are zeroed again for the next subtile (the wave's LDS operations take effect
@@ -766,7 +768,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 // Persistent pass: every workgroup takes work items from the queue until it is empty. FM = 0:
 // every item in the natural form; 1: every item in the base-2 form; 2: each item in the base-2
 // form where its dataset allows it (SymData::fast), else natural (both paths in one kernel, at
-// some register cost, so it is used only for mixed dataset tables).
+// some register cost, so it is used only for mixed dataset tables); 3: as 1, with every
+// background exponent of the launch in exp2_neg_ktab's range (SymLaunch::ktab; FP64 only).
 // LDS of sym_kernel in doubles (below): subtiles or the diagonal tile, column slots, the
 // exponential's table (base-2 form in FP64), per-wave column sums.
 template <int D, bool GRAD, bool F32, int FM> struct SymLds {
@@ -792,6 +795,8 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     using Lds = SymLds<D, GRAD, F32, FM>;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr bool TAB = FM != 0 && !F32; // the base-2 form in FP64: exp2_neg_tab
+    constexpr bool KT = FM == 3;          // exp2_neg_ktab and its table
+    static_assert(!KT || !F32, "the exponent-in-table form is FP64's");
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x NCOMP x 128 doubles
     constexpr int kTileD = Lds::tile;
     constexpr int kSlotD = Lds::slot;
@@ -808,7 +813,7 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     // the table is complete before the first item: the queue loop's barrier follows
-    if (TAB) fill_exp2_table(lds + kTileD + kSlotD, tid);
+    if (TAB) fill_exp2_table(lds + kTileD + kSlotD, tid, KT);
     // this wave's column slots (sym_subtile), zero between subtiles
     double *cs = colslot + w * NC * 128;
 #pragma unroll
@@ -820,18 +825,18 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
         const int idx = queue_next(L, &s_item);
         if (idx >= L.items) break;
         const int4 it = L.item[idx];
-        const bool fast = FM == 1 || (FM == 2 && L.data[it.x].fast);
+        const bool fast = FM == 1 || FM == 3 || (FM == 2 && L.data[it.x].fast);
         const bool flush = it.z >= 0 && L.data[it.x].L > kFlushTiles * kTile;
         if (fast) {
             if (flush)
-                sym_item<D, GRAD, A, F32, true, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
+                sym_item<D, GRAD, A, F32, true, KT, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
             else
-                sym_item<D, GRAD, A, F32, true, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
-        } else {
+                sym_item<D, GRAD, A, F32, true, KT, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
+        } else if constexpr (FM != 1 && FM != 3) {
             if (flush)
-                sym_item<D, GRAD, A, F32, false, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
+                sym_item<D, GRAD, A, F32, false, false, PM, true>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
             else
-                sym_item<D, GRAD, A, F32, false, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
+                sym_item<D, GRAD, A, F32, false, false, PM, false>(L, it, params, rowpart, colpart, tile, colbuf, cs, etab, tid, lane, w);
         }
     }
 }

@@ -161,6 +161,14 @@ class Engine:
         check(_lib.lib().hph_get_last_pass(self._h, out, 3))
         return int(out[2])
 
+    def exponent_in_table(self):
+        """True when the last full evaluation's time-sorted symmetric pass ran the FP64 table
+        exponential with the power of two folded into the table entry (every background exponent
+        of the call within its range; results bitwise those of the table exponential with ldexp)."""
+        out = (ctypes.c_int * 4)()
+        check(_lib.lib().hph_get_last_pass(self._h, out, 4))
+        return bool(out[3])
+
     def timing(self):
         """(pair-pass ms, finalize + reduction ms) of the last evaluation."""
         out = np.zeros(2)

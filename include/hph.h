@@ -153,8 +153,11 @@ int hph_get_last_evaluation(hph_engine *engine, int *kind);
  * of a sum of squares; chosen only where the rounding of the scaled values stays below 1e-13
  * relative on the terms), 0 for the natural form; with length >= 3, out[2] = parameter vectors
  * per launch (a batch whose symmetric or split partial sums do not fit in device memory at once
- * runs as consecutive launches of that many vectors). length must be >= 2;
- * HPH_ERR_NOT_INITIALIZED before the first evaluation. */
+ * runs as consecutive launches of that many vectors); with length >= 4, out[3] = 1 when the
+ * time-sorted symmetric pass ran in FP64 with every background exponent of the call at most 700
+ * (from the data's extents and each vector's tauX, tauT), where its table exponential puts the
+ * power of two into the table entry instead of scaling the result (bitwise the same results).
+ * length must be >= 2; HPH_ERR_NOT_INITIALIZED before the first evaluation. */
 int hph_get_last_pass(hph_engine *engine, int *out, size_t length);
 /* The per-event sums behind the last log-likelihood-and-gradient evaluation at the current
  * parameters (dimensions 1..3): out is N x 6 row-major in the caller's order, length = 6 N, with

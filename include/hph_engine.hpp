@@ -82,7 +82,9 @@ class HawkesEngine : public AbstractHawkes {
     virtual int getLastEvaluation() = 0;
     // Pair pass of the last full evaluation: out[0] = HPH_PAIR_KERNEL_ROWWISE, _SYMMETRIC or
     // _SPLIT, out[1] = 1 when it ran the base-2 form (scaled coordinates and times), else 0;
-    // with length >= 3, out[2] = parameter vectors per launch.
+    // with length >= 3, out[2] = parameter vectors per launch; with length >= 4, out[3] = 1 when
+    // the time-sorted symmetric pass ran the table exponential with the power of two in the table
+    // entry (every background exponent of the call at most 700; bitwise the same results).
     virtual void getLastPass(int *out, size_t length) = 0;
     // Per-event sums (sum B, B r^2, B dt^2, S, S r^2, S dt; natural units, N x 6 row-major in the
     // caller's order) of the last evaluation with the gradient at the current parameters.

@@ -4,7 +4,8 @@ reductions, Horner with correctly rounded fma, emulated at 60 digits) against mp
 digits. exp_fp64 (Cody-Waite, natural form), exp2_fp64 (shifter, exponents of either sign) and
 exp2_neg_fp64 (2^-y for y >= 0 from trunc and fract) and exp2_neg_tab (2^-y from 32 y: a 32-entry
 table of 2^(b/32), a degree-5 polynomial and v_ldexp_f64; the FP64 background terms of the base-2
-form).
+form), and exp2_neg_ktab (the same with the power of two added into the table entry's exponent
+field) bitwise against exp2_neg_tab over its range.
 A dropped or mistyped coefficient, a wrong sign of the 2^-f polynomial or an off-by-one in the
 reduction moves the error far above the bound.
 """
@@ -146,6 +147,50 @@ def test_exp2_neg_tab():
     assert exp2_neg_tab(c, tab, 96.0) == 0.125  # 2^-3
     assert exp2_neg_tab(c, tab, 32.0 * 1100) == 0.0  # below the FP64 range, as IEEE exp
     assert exp2_neg_tab(c, tab, 2.0 ** 31 - 1) == 0.0
+
+
+def exp2_ktable(tab):
+    """fill_exp2_table with ktab: the bit pattern of entry b with b << 15 taken off its high word."""
+    return [int(np.array([t]).view(np.uint64)[0]) - (b << 47) for b, t in enumerate(tab)]
+
+
+def exp2_neg_ktab(c, ktab, y32):
+    """exp2_neg_ktab: as exp2_neg_tab, with (-trunc(y32)) << 15 added to the table entry's high
+    word (32-bit wrap-around) in place of the final ldexp."""
+    ni = -int(y32)
+    f = y32 - np.floor(y32)
+    q = c["HPH_EXP2T_C5"]
+    for k in (4, 3, 2, 1):
+        q = fma(q, f, c["HPH_EXP2T_C%d" % k])
+    bits = ktab[ni & 31]
+    hi = ((bits >> 32) + ((ni << 15) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    t = float(np.array([(hi << 32) | (bits & 0xFFFFFFFF)], dtype=np.uint64).view(np.float64)[0])
+    return fma(t, float(mp.mpf(f) * mp.mpf(q)), t)
+
+
+def test_exp2_neg_ktab_is_bitwise_exp2_neg_tab():
+    """Over its whole range (0 <= y32 <= kExp2KTabMaxY32) the exponent-in-table variant returns
+    exp2_neg_tab's doubles bit for bit: every entry b, both ends of the range, integer and
+    fractional y32. Its range covers every background exponent up to the host's bound (ktab_ok in
+    hph_engine.cpp), and the smallest result is a normal double."""
+    c = constants()
+    tab = exp2_table(c)
+    ktab = exp2_ktable(tab)
+    ymax = device_constants()["kExp2KTabMaxY32"]
+    assert ymax == 32.0 * 1011
+    with open(os.path.join(HERE, "..", "hawkes_amd", "csrc", "hph_engine.cpp")) as f:
+        src = f.read()
+    body = src[src.index("static bool ktab_ok("):]
+    bound = float(re.search(r"if \(!\(aB <= ([0-9.]+)\)\) return false;", body).group(1))
+    assert bound * 32 * float(mp.log(mp.e, 2)) < ymax
+    rng = np.random.default_rng(9)
+    ys = np.concatenate([rng.uniform(0.0, ymax, 400), rng.uniform(0.0, 64.0, 200),
+                         np.arange(0.0, 70.0), [ymax, ymax - 1e-9, ymax - 31.5, 0.5, 1e-300, 31.999999]])
+    for y32 in ys:
+        y32 = float(y32)
+        a, b = exp2_neg_ktab(c, ktab, y32), exp2_neg_tab(c, tab, y32)
+        assert a == b and np.float64(a).tobytes() == np.float64(b).tobytes(), (y32, a, b)
+    assert exp2_neg_ktab(c, ktab, ymax) >= 2.0 ** -1022
 
 
 # ---- double-double constants of the base-2 form (hph_device.h) ----

@@ -14,6 +14,11 @@ against max(|g|, 1e-3 x the sum of |per-row terms|)), and 1e-6 in FP32-bulk mode
 The split passes' background pass runs on the culled order, whose blocks span a whole window of
 consecutive events in time; the base-2 form is then also gated on st x (window time span) <= 300.
 test_split_pass_wide_window_takes_the_natural_form builds data past that gate.
+
+In FP64 the time-sorted pass's table exponential folds the power of two into the table entry when
+every background exponent of the call is at most 700 (ktab_gate; Engine.exponent_in_table): every
+FP64 base-2 case here asserts which variant ran, and
+test_exponent_in_table_variant_is_bitwise_the_ldexp_variant compares the two on one vector.
 """
 import numpy as np
 import pytest
@@ -66,6 +71,15 @@ def base2_gate(locs, times, p, split=False):
     return bool(ok)
 
 
+def ktab_gate(locs, times, params):
+    """The engine's bound for the exponent-in-table exponential (ktab_ok), restated: every vector's
+    tauX^2 r2max / 2 + tauT^2 span^2 / 2 <= 700."""
+    P = np.atleast_2d(params)
+    r2max = float(np.sum((locs.max(axis=0) - locs.min(axis=0)) ** 2))
+    span = float(np.max(times) - np.min(times))
+    return bool(np.all(0.5 * P[:, 1] ** 2 * r2max + 0.5 * P[:, 2] ** 2 * span ** 2 <= 700.0))
+
+
 def events_at_scale(n, sx_diag, seed):
     """n events, uniform on a square whose diagonal times sx is sx_diag (params P0), times uniform
     on [0, HORIZON) h and sorted."""
@@ -99,12 +113,14 @@ def check(locs, times, params, rows_per_lane, flags, tol_ll, tol_g, tol_p, kind=
     e.set_parameters(params)
     ll, g = e.log_likelihood_and_gradient()
     assert e.last_pass() == (kind, base2)
+    ktab = kind == _lib.PAIR_KERNEL_SYMMETRIC and base2 and not (flags & _lib.FLOAT) and ktab_gate(locs, times, params)
+    assert e.exponent_in_table() == ktab
     gerr = np.abs(g - ref["grad"]) / np.maximum(np.abs(ref["grad"]), 1e-3 * ref["scale"][1:])
     p = e.probs_self_excite()
     assert e.last_pass() == (kind, base2)
     perr = np.max(np.abs(p - ref["probs"]) / np.maximum(np.abs(ref["probs"]), 1e-300))
-    print("N=%d A=%d flags=%d kind=%d base2=%d: ll %.2e grad %.2e probs %.2e" % (
-        len(times), rows_per_lane, flags, kind, base2, rel_err(ll, ref["ll"]), gerr.max(), perr), flush=True)
+    print("N=%d A=%d flags=%d kind=%d base2=%d ktab=%d: ll %.2e grad %.2e probs %.2e" % (
+        len(times), rows_per_lane, flags, kind, base2, ktab, rel_err(ll, ref["ll"]), gerr.max(), perr), flush=True)
     e.close()
     assert rel_err(ll, ref["ll"]) <= tol_ll
     assert gerr.max() <= tol_g, gerr
@@ -148,3 +164,37 @@ def test_split_pass_wide_window_takes_the_natural_form():
     assert base2_gate(locs, times, p, split=False)
     check(locs, times, p, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SPLIT, base2=False)
     check(locs, times, p, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SYMMETRIC, base2=True)
+
+
+def test_exponent_in_table_variant_is_bitwise_the_ldexp_variant():
+    """One vector (P0 on 20 km x 6,000 h: largest background exponent ~315) in the two kernel
+    variants: alone, its call runs the exponent-in-table exponential; in a batch with a vector
+    whose exponents reach ~1,300 (tauT = 0.008 / h) the launch runs the table exponential with its
+    ldexp. Same chunking (one plan at both K), so the log-likelihood and gradient must agree bit
+    for bit; the single call is also checked against the FP64 oracle."""
+    rng = np.random.Generator(np.random.PCG64(672))
+    n = 16384
+    locs = rng.uniform(0.0, 20.0, size=(n, 2))
+    times = np.sort(rng.uniform(0.0, HORIZON, size=n))
+    pa = np.array(synth.P0, dtype=float)
+    pb = pa.copy()
+    pb[2] = 0.008
+    assert base2_gate(locs, times, pa) and base2_gate(locs, times, pb)
+    assert ktab_gate(locs, times, pa) and not ktab_gate(locs, times, pb)
+    e = Engine(2, n, flags=_lib.OPENCL)
+    e.set_pair_config(_lib.PAIR_KERNEL_SYMMETRIC, 2, 4096)
+    e.update_locations(locs)
+    e.set_times(times)
+    e.set_parameters(pa)
+    ll_a, g_a = e.log_likelihood_and_gradient()
+    assert e.last_pass() == (_lib.PAIR_KERNEL_SYMMETRIC, True) and e.exponent_in_table()
+    e.set_parameters_batch(np.stack([pa, pb]))
+    ll_b, g_b = e.log_likelihood_batch()
+    assert e.last_pass() == (_lib.PAIR_KERNEL_SYMMETRIC, True) and not e.exponent_in_table()
+    print("ll single %r batch %r" % (ll_a, ll_b[0]), flush=True)
+    assert ll_b[0] == ll_a
+    assert np.array_equal(g_b[0], g_a)
+    ref = oracle.total(locs, times, pa, precision="double")
+    assert rel_err(ll_a, ref["ll"]) <= 1e-12
+    assert np.all(np.abs(g_a - ref["grad"]) <= 1e-12 * np.maximum(np.abs(ref["grad"]), 1e-3 * ref["scale"][1:]))
+    e.close()
