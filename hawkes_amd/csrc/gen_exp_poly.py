@@ -12,17 +12,18 @@ on [-1/2, 1/2].
 exp2 of a non-negative exponent (background terms of the base-2 pair form, 2^-y with y >= 0):
     y = i + f,   i = trunc(y),   f = fract(y) in [0, 1) exact,   2^-y = 2^-i * s(f),   s ~ 2^-f
 on [0, 1] (the same relative error as q: 2^-f on [0, 1] is 2^-1/2 times 2^g on [-1/2, 1/2]).
-exp2 of a non-negative exponent from a table (the FP64 background terms of the base-2 pair form,
-whose exponent arrives scaled by 32: y32 = 32 y >= 0):
-    -trunc(y32) = 32 a + b (b in [0, 31]),   f = fract(y32) in [0, 1) exact,
-    2^-y = 2^a * M[b] * u(f),   M[b] = 2^(b/32) correctly rounded,   u ~ 2^(-f/32) on [0, 1].
-u has degree 5: the interval is 1/32 of the one above.
+exp2 of a non-negative exponent from a table of S = 2^TAB_BITS entries (the FP64 background terms
+of the base-2 pair form, whose exponent arrives scaled by S: yS = S y >= 0):
+    -trunc(yS) = S a + b (b in [0, S - 1]),   f = fract(yS) in [0, 1) exact,
+    2^-y = 2^a * M[b] * u(f),   M[b] = 2^(b/S) correctly rounded,   u ~ 2^(-f/S) on [0, 1].
+With S = 256, u has degree 4 (the interval is 1/256 of the one above); S = 32 needs degree 5.
 All four polynomials have value 1 at 0 exactly and the other coefficients minimise
 the weighted (relative) least-squares error on 3000 Chebyshev nodes at 40 digits, which lands
 close to the minimax polynomial. The script then measures the max relative error of the
 double-precision Horner evaluation with correctly rounded fma at 20000 points.
 
-Usage: python hawkes_amd/csrc/gen_exp_poly.py [degree]   (writes hph_exp_coeffs.h beside itself)
+Usage: python hawkes_amd/csrc/gen_exp_poly.py [degree [table bits [table degree]]]
+(defaults 10, 8, 4; writes hph_exp_coeffs.h beside itself)
 """
 import os
 import random
@@ -64,6 +65,9 @@ def horner(coef_d, r):
 
 def main():
     degree = int(sys.argv[1]) if len(sys.argv) > 1 else 10
+    tbits = int(sys.argv[2]) if len(sys.argv) > 2 else 8
+    tdeg = int(sys.argv[3]) if len(sys.argv) > 3 else 4
+    S = 1 << tbits
     ln2 = mp.log(2)
     ln2_hi = float(mp.nint(ln2 * 2 ** 32) / 2 ** 32)
     ln2_lo = float(ln2 - mp.mpf(ln2_hi))
@@ -95,22 +99,21 @@ def main():
         i = int(y)
         v = mp.ldexp(mp.mpf(horner(coef3_d, y - i)), -i)
         worst3 = max(worst3, abs(float((v - mp.power(2, -y)) / mp.power(2, -y))))
-    tdeg = 5
-    coef4 = fit(tdeg, mp.mpf(1), -ln2 / 32, mp.mpf(0))
+    coef4 = fit(tdeg, mp.mpf(1), -ln2 / S, mp.mpf(0))
     coef4_d = [float(c) for c in coef4]
-    mant = [float(mp.power(2, mp.mpf(b) / 32)) for b in range(32)]
+    mant = [float(mp.power(2, mp.mpf(b) / S)) for b in range(S)]
     worst4 = 0.0
     for _ in range(20000):
         y = random.choice((random.uniform(0.0, 4.0), random.uniform(0.0, 1020.0)))
-        y32 = float(mp.mpf(y) * 32)
-        ni = -int(y32)
-        f = y32 + ni
-        t = mant[ni & 31]
+        yS = float(mp.mpf(y) * S)
+        ni = -int(yS)
+        f = yS + ni
+        t = mant[ni & (S - 1)]
         q = coef4_d[-1]
         for c in reversed(coef4_d[1:-1]):
             q = fma_d(q, f, c)
-        v = mp.ldexp(mp.mpf(fma_d(t, float(mp.mpf(f) * mp.mpf(q)), t)), ni >> 5)
-        ref = mp.power(2, -mp.mpf(y32) / 32)
+        v = mp.ldexp(mp.mpf(fma_d(t, float(mp.mpf(f) * mp.mpf(q)), t)), ni >> tbits)
+        ref = mp.power(2, -mp.mpf(yS) / S)
         worst4 = max(worst4, abs(float((v - ref) / ref)))
     path = os.path.join(HERE, "hph_exp_coeffs.h")
     with open(path, "w") as f:
@@ -128,12 +131,16 @@ def main():
             f.write("#define HPH_EXP2_C%d %r\n" % (k, c))
         for k, c in enumerate(coef3_d):
             f.write("#define HPH_EXP2N_C%d %r\n" % (k, c))
+        f.write("#define HPH_EXP2T_BITS %d\n" % tbits)
         f.write("#define HPH_EXP2T_DEGREE %d\n" % tdeg)
         for k, c in enumerate(coef4_d):
             f.write("#define HPH_EXP2T_C%d %r\n" % (k, c))
+        # u(f) = 1 + f q(f): the coefficients of q, lowest first
+        f.write("#define HPH_EXP2T_Q %s\n" % ", ".join("HPH_EXP2T_C%d" % k for k in range(1, tdeg + 1)))
         for j, m in enumerate(mant):
             f.write("#define HPH_EXP2T_M%d %r\n" % (j, m))
-    print("degree", degree, "max rel err exp %.3e exp2 %.3e exp2 of -y %.3e table %.3e" % (worst, worst2, worst3, worst4),
+        f.write("#define HPH_EXP2T_TABLE %s\n" % ", ".join("HPH_EXP2T_M%d" % j for j in range(S)))
+    print("degree", degree, "table", S, "x degree", tdeg, "max rel err exp %.3e exp2 %.3e exp2 of -y %.3e table %.3e" % (worst, worst2, worst3, worst4),
           "->", path)
 
 

@@ -89,65 +89,64 @@ __device__ __forceinline__ double exp2_neg_fp64(double y) {
     return __builtin_amdgcn_ldexp(p, mi);
 }
 
-// 2^-y for y >= 0 from a 32-entry table: the FP64 background terms of the base-2 form, whose
-// exponent arrives scaled by 32 (y32 = 32 y, a sum of squares of coordinates and times scaled by
-// time_scale32_dd; the host keeps y32 below 2^31). -trunc(y32) = 32 a + b with a = its arithmetic
-// shift by 5 and b in [0, 31] its low bits, f = fract(y32) in [0, 1) (exact for y32 >= 0):
-//   2^-y = 2^a tab[b] u(f),   tab[b] = 2^(b/32),   u(f) = 1 + f q(f) ~ 2^(-f/32)
-// with u the degree-5 polynomial of gen_exp_poly.py, applied as tab[b] + tab[b] (f q(f)), and
-// v_ldexp_f64 for 2^a. Eleven VALU operations (convert, fract, two for the address, shift, 4 fma,
-// mul, fma, ldexp) and one LDS read where exp2_neg_fp64 takes 13. The table is 256 bytes, one
-// entry per pair of LDS banks, so the 64 lanes' reads never conflict (lanes reading one entry
-// share it). Max relative error 2.7e-16 (table entries correctly rounded, so their errors do not
-// share a sign as a polynomial's do); exactly 0 below the FP64 range, as IEEE exp.
-constexpr int kExp2TabSize = 32;
-__device__ __forceinline__ double exp2_neg_tab(double y32, const double *tab) {
-    const int ni = (int)(-y32); // -trunc(y32)
-    const double f = __builtin_amdgcn_fract(y32);
-    double q = HPH_EXP2T_C5;
-    q = __builtin_fma(q, f, HPH_EXP2T_C4);
-    q = __builtin_fma(q, f, HPH_EXP2T_C3);
-    q = __builtin_fma(q, f, HPH_EXP2T_C2);
-    q = __builtin_fma(q, f, HPH_EXP2T_C1);
-    const double t = tab[ni & 31];
-    return __builtin_amdgcn_ldexp(__builtin_fma(t, f * q, t), ni >> 5);
+// 2^-y for y >= 0 from a table of S = 256 entries: the FP64 background terms of the base-2 form,
+// whose exponent arrives scaled by S (yS = S y, a sum of squares of coordinates and times scaled
+// by time_scale_tab_dd; the host keeps yS below 2^31). -trunc(yS) = S a + b with a = its
+// arithmetic shift by 8 and b in [0, S - 1] its low byte, f = fract(yS) in [0, 1) (exact for
+// yS >= 0):
+//   2^-y = 2^a tab[b] u(f),   tab[b] = 2^(b/S),   u(f) = 1 + f q(f) ~ 2^(-f/S)
+// with u the degree-4 polynomial of gen_exp_poly.py, applied as tab[b] + tab[b] (f q(f)), and
+// v_ldexp_f64 for 2^a. Ten VALU operations (convert, fract, the byte address - one shift whose
+// operand is the low byte, an SDWA select -, shift, 3 fma, mul, fma, ldexp) and one LDS read where
+// exp2_neg_fp64 takes 13. The table is 2 KB. Max relative error 2.0e-16 (table entries correctly
+// rounded, so their errors do not share a sign as a polynomial's do); exactly 0 below the FP64
+// range, as IEEE exp.
+constexpr int kExp2TabBits = HPH_EXP2T_BITS;
+constexpr int kExp2TabSize = 1 << kExp2TabBits;
+static_assert(kExp2TabBits == 8, "the byte index (ni & 255, one SDWA operand select) and the scale constants");
+// f q(f), q = HPH_EXP2T_Q by Horner
+__device__ __forceinline__ double exp2t_fq(double f) {
+    constexpr double Q[] = {HPH_EXP2T_Q};
+    constexpr int n = (int)(sizeof(Q) / sizeof(Q[0]));
+    double q = Q[n - 1];
+#pragma unroll
+    for (int k = n - 2; k >= 0; --k) q = __builtin_fma(q, f, Q[k]);
+    return f * q;
+}
+__device__ __forceinline__ double exp2_neg_tab(double yS, const double *tab) {
+    const int ni = (int)(-yS); // -trunc(yS)
+    const double fq = exp2t_fq(__builtin_amdgcn_fract(yS));
+    const double t = tab[ni & (kExp2TabSize - 1)];
+    return __builtin_amdgcn_ldexp(__builtin_fma(t, fq, t), ni >> kExp2TabBits);
 }
 // exp2_neg_tab where every argument keeps the result in the normal range with margin
-// (0 <= y32 <= kExp2KTabMaxY32, checked by the caller for all pairs of a launch): the power 2^k,
-// k = ni >> 5, goes into the table entry's exponent field by one integer add instead of a shift
-// and an ldexp after the fma. ni = 32 k + b, so adding ni << 15 to the high word adds k << 20 (the
-// exponent) and b << 15; the table (fill_exp2_table, ktab) holds 2^(b/32) with b << 15 taken off
-// its high word, so the sum is exactly the bit pattern of 2^(b/32) 2^k. With k >= -1011 every
+// (0 <= yS <= kExp2KTabMaxY, checked by the caller for all pairs of a launch): the power 2^k,
+// k = ni >> 8, goes into the table entry's exponent field by one integer add instead of a shift
+// and an ldexp after the fma. ni = 256 k + b, so adding ni << 12 to the high word adds k << 20 (the
+// exponent) and b << 12; the table (fill_exp2_table, ktab) holds 2^(b/256) with b << 12 taken off
+// its high word, so the sum is exactly the bit pattern of 2^(b/256) 2^k. With k >= -1011 every
 // t 2^k and the fma's result are normal doubles, where scaling by 2^k commutes with rounding: the
 // result is bitwise exp2_neg_tab's.
-constexpr double kExp2KTabMaxY32 = 32352.0; // 32 x 1011: k = ni >> 5 >= -1011
-__device__ __forceinline__ double exp2_neg_ktab(double y32, const double *tab) {
-    const int ni = (int)(-y32); // -trunc(y32)
-    const double f = __builtin_amdgcn_fract(y32);
-    double q = HPH_EXP2T_C5;
-    q = __builtin_fma(q, f, HPH_EXP2T_C4);
-    q = __builtin_fma(q, f, HPH_EXP2T_C3);
-    q = __builtin_fma(q, f, HPH_EXP2T_C2);
-    q = __builtin_fma(q, f, HPH_EXP2T_C1);
-    uint2 w = __builtin_bit_cast(uint2, tab[ni & 31]);
-    w.y += (unsigned)ni << 15;
+constexpr double kExp2KTabMaxY = 258816.0; // 256 x 1011: k = ni >> 8 >= -1011
+static_assert(kExp2KTabMaxY == kExp2TabSize * 1011.0, "k >= -1011");
+__device__ __forceinline__ double exp2_neg_ktab(double yS, const double *tab) {
+    const int ni = (int)(-yS); // -trunc(yS)
+    const double fq = exp2t_fq(__builtin_amdgcn_fract(yS));
+    uint2 w = __builtin_bit_cast(uint2, tab[ni & (kExp2TabSize - 1)]);
+    w.y += (unsigned)ni << (20 - kExp2TabBits);
     const double t = __builtin_bit_cast(double, w);
-    return __builtin_fma(t, f * q, t);
+    return __builtin_fma(t, fq, t);
 }
-// Fills exp2_neg_tab's table (LDS): entry b = 2^(b/32) correctly rounded (gen_exp_poly.py); with
-// ktab, exp2_neg_ktab's: the same entry with b << 15 taken off its high word (a bit pattern, not
-// used as a number). Threads tid < 32 of the workgroup write it; the caller barriers before use.
-__device__ __forceinline__ void fill_exp2_table(double *tab, int tid, bool ktab = false) {
-    constexpr double M[kExp2TabSize] = {
-        HPH_EXP2T_M0,  HPH_EXP2T_M1,  HPH_EXP2T_M2,  HPH_EXP2T_M3,  HPH_EXP2T_M4,  HPH_EXP2T_M5,  HPH_EXP2T_M6,
-        HPH_EXP2T_M7,  HPH_EXP2T_M8,  HPH_EXP2T_M9,  HPH_EXP2T_M10, HPH_EXP2T_M11, HPH_EXP2T_M12, HPH_EXP2T_M13,
-        HPH_EXP2T_M14, HPH_EXP2T_M15, HPH_EXP2T_M16, HPH_EXP2T_M17, HPH_EXP2T_M18, HPH_EXP2T_M19, HPH_EXP2T_M20,
-        HPH_EXP2T_M21, HPH_EXP2T_M22, HPH_EXP2T_M23, HPH_EXP2T_M24, HPH_EXP2T_M25, HPH_EXP2T_M26, HPH_EXP2T_M27,
-        HPH_EXP2T_M28, HPH_EXP2T_M29, HPH_EXP2T_M30, HPH_EXP2T_M31};
-    if (tid < kExp2TabSize) {
-        uint2 w = __builtin_bit_cast(uint2, M[tid]);
-        if (ktab) w.y -= (unsigned)tid << 15;
-        tab[tid] = __builtin_bit_cast(double, w);
+// Fills exp2_neg_tab's table (LDS): entry b = 2^(b/256) correctly rounded (gen_exp_poly.py); with
+// ktab, exp2_neg_ktab's: the same entry with b << 12 taken off its high word (a bit pattern, not
+// used as a number). Threads tid < 256 of the workgroup write it (a workgroup of 256 threads
+// writes every entry; smaller ones loop); the caller barriers before use.
+__device__ __forceinline__ void fill_exp2_table(double *tab, int tid, int nthreads, bool ktab = false) {
+    constexpr double M[kExp2TabSize] = {HPH_EXP2T_TABLE};
+    for (int b = tid; b < kExp2TabSize; b += nthreads) {
+        uint2 w = __builtin_bit_cast(uint2, M[b]);
+        if (ktab) w.y -= (unsigned)b << (20 - kExp2TabBits);
+        tab[b] = __builtin_bit_cast(double, w);
     }
 }
 
@@ -162,8 +161,9 @@ __device__ __forceinline__ double time_scale(double tauT) { return fabs(tauT) * 
 constexpr double kSqrtHalfLog2eHi = 0.8493218002880191, kSqrtHalfLog2eLo = -2.5959464104597403e-17;
 constexpr double kTwoLn2Hi = 1.3862943611198906, kTwoLn2Lo = 4.638093627692599e-17;
 constexpr double kLog2eHi = 1.4426950408889634, kLog2eLo = 2.0355273740931033e-17;
-// 4 sqrt(log2(e)) = sqrt(32) sqrt(log2(e) / 2): the scale of the table form (exponents times 32).
-constexpr double kSqrt16Log2eHi = 4.804489635145799, kSqrt16Log2eLo = -1.1835725830478566e-16;
+// sqrt(128 log2(e)) = sqrt(256) sqrt(log2(e) / 2): the scale of the table form (exponents times
+// 256 = kExp2TabSize).
+constexpr double kSqrt128Log2eHi = 13.589148804608305, kSqrt128Log2eLo = -4.1535142567355846e-16;
 
 // The base-2 form's scales as double-doubles: a rounded scale st would make every scaled time
 // st (t - t0) carry the same relative error, i.e. the pass would evaluate the background kernel at
@@ -177,17 +177,17 @@ __device__ __forceinline__ double2 time_scale_dd(double tau) {
     const double err = __builtin_fma(a, kSqrtHalfLog2eHi, -hi); // exact error of the product
     return make_double2(hi, __builtin_fma(a, kSqrtHalfLog2eLo, err));
 }
-// The same with |tau| 4 sqrt(log2(e)) (exponents scaled by 32, for exp2_neg_tab).
-__device__ __forceinline__ double2 time_scale32_dd(double tau) {
+// The same with |tau| sqrt(128 log2(e)) (exponents scaled by 256, for exp2_neg_tab).
+__device__ __forceinline__ double2 time_scale_tab_dd(double tau) {
     const double a = fabs(tau);
-    const double hi = a * kSqrt16Log2eHi;
-    const double err = __builtin_fma(a, kSqrt16Log2eHi, -hi);
-    return make_double2(hi, __builtin_fma(a, kSqrt16Log2eLo, err));
+    const double hi = a * kSqrt128Log2eHi;
+    const double err = __builtin_fma(a, kSqrt128Log2eHi, -hi);
+    return make_double2(hi, __builtin_fma(a, kSqrt128Log2eLo, err));
 }
 // v (s.hi + s.lo), rounded once.
 __device__ __forceinline__ double scale_by(double v, double2 s) { return __builtin_fma(v, s.x, v * s.y); }
-// A moment summed in the base-2 form's squared units (r'^2 = r^2 tau^2 log2(e) / 2, or 32 times
-// that in the table form: unit = 1/32) back to natural units: m unit 2 ln2 / tau2 with
+// A moment summed in the base-2 form's squared units (r'^2 = r^2 tau^2 log2(e) / 2, or 256 times
+// that in the table form: unit = 1/256) back to natural units: m unit 2 ln2 / tau2 with
 // tau2 = tau * tau as the finalize's gradient formulas round it, so that tau2 times the result is
 // m unit 2 ln 2 to one rounding. unit is a power of two (the constants' products are exact).
 __device__ __forceinline__ double from_base2_sq(double m, double tau2, double unit = 1.0) {

@@ -132,7 +132,7 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
 }
 
 // True when the pass may use the base-2 form (scaled coordinates and times, clamp-free exp2):
-// every background (tauX^2 r^2/2 + tauT^2 dt^2/2) exponent of the K vectors is below 4e7 and
+// every background (tauX^2 r^2/2 + tauT^2 dt^2/2) exponent of the K vectors is below 5.5e6 and
 // every excitation (|omega| dt + sigma^2 r^2/2) exponent below 1e9 in size, the scales st = |tauT| sqrt(log2(e)/2) and
 // sx = |tauX| sqrt(log2(e)/2) lie in [1e-100, 1e100] with omega log2(e) / st and sigma^2 / tauX^2
 // finite, and distinct times are at least 8 ulp(max |t|) apart: then (t - t0) st, rounded, keeps
@@ -156,9 +156,9 @@ static bool fast_ok(const Extents &e, const double *params, int K, bool split = 
         if (!(std::fabs(p[3]) * e.block_span <= 300.0)) return false;
         const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;
-        // background: 32 log2(e) aB < 2^31 (the FP64 table exponential converts 32 x its base-2
-        // exponent to an int)
-        if (!(aB < 4e7 && aS < 1e9)) return false;
+        // background: 256 log2(e) aB < 2^31 (the FP64 table exponential converts 256 x its base-2
+        // exponent to an int: aB < 5.81e6)
+        if (!(aB < 5.5e6 && aS < 1e9)) return false;
         const double st = std::fabs(p[2]) * 0.84932180028801904272;
         const double sx = std::fabs(p[1]) * 0.84932180028801904272;
         if (!(st >= 1e-100 && st <= 1e100 && sx >= 1e-100 && sx <= 1e100)) return false;
@@ -172,8 +172,8 @@ static bool fast_ok(const Extents &e, const double *params, int K, bool split = 
 
 // True when every background exponent tauX^2 r^2/2 + tauT^2 dt^2/2 that the K vectors can form
 // over the data's extents is at most 700: the FP64 base-2 time-sorted pass may then run
-// exp2_neg_ktab, whose argument (32 log2(e) times the exponent, formed from the scaled values)
-// must stay below kern::kExp2KTabMaxY32 = 32 x 1011; 700 gives 32,316, the rest a margin far
+// exp2_neg_ktab, whose argument (256 log2(e) times the exponent, formed from the scaled values)
+// must stay below kern::kExp2KTabMaxY = 256 x 1011 = 258,816; 700 gives 258,525, the rest a margin far
 // beyond the scaled values' rounding. (The split passes' background pass spans every window of
 // the culled order and keeps exp2_neg_tab.)
 static bool ktab_ok(const Extents &e, const double *params, int K) {

@@ -200,8 +200,8 @@ struct SymLaunch {
                             // pass skipped every tile of the item (partials unwritten)
     int bsum_shared;        // finalize from per-event sums: 1 when bsum holds one set of background
                             // moments for every vector (vectors sharing tauX and tauT), 0 one per vector
-    int b2tab;              // 1: the base-2 form ran in FP64 with exponents scaled by 32 (the table
-                            // exponential exp2_neg_tab); its squared moments are 32 times the
+    int b2tab;              // 1: the base-2 form ran in FP64 with exponents scaled by 256 (the table
+                            // exponential exp2_neg_tab); its squared moments are 256 times the
                             // unscaled ones. 0: FP32-bulk (unscaled). Set from the precision by the
                             // host; launch_sym checks that it matches.
     int ktab;               // 1: every background exponent of every pair of the launch lies in

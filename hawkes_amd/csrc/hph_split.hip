@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_bsum_kernel(SymLaunch L, co
     for (int q = 0; q < QS; ++q) b[q] = a[q] + cs[q];
     if (GRAD && ds.fast) { // the base-2 pass summed e y (y = r'^2 + dt'^2) and e dt'^2, r' = sx r, dt' = st dt
         const double *pk = params + 6 * (size_t)(ds.param0 + k);
-        const double unit = L.b2tab ? 0.03125 : 1.0;
+        const double unit = L.b2tab ? 1.0 / kern::kExp2TabSize : 1.0;
         b[1] = from_base2_sq(b[1] - b[2], pk[1] * pk[1], unit);
         b[2] = from_base2_sq(b[2], pk[2] * pk[2], unit);
     }

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void sym_finalize_kernel(
                 // r' = sx r, dt' = st dt (the excitation's dt is natural)
                 const double *pk = params + 6 * (size_t)(ds.param0 + k);
                 const double x2 = pk[1] * pk[1], t2 = pk[2] * pk[2];
-                const double unit = L.b2tab ? 0.03125 : 1.0;
+                const double unit = L.b2tab ? 1.0 / kern::kExp2TabSize : 1.0;
                 s[1] = from_base2_sq(s[1] - s[2], x2, unit);
                 s[2] = from_base2_sq(s[2], t2, unit);
                 s[4] = from_base2_sq(s[4], x2, unit);

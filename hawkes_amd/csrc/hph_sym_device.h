@@ -46,9 +46,9 @@ namespace kern {
 //      by exp2_fp64. The background moments are summed as e y and e dt'^2, the excitation ones
 //      as e r'^2 and e dt; the finalize kernels turn them into e r^2 = (e y - e dt'^2) / sx^2,
 //      e dt^2, e r^2 and e dt.
-//      In FP64 (F32 false) the scales are sqrt(32) times larger (time_scale32_dd), so that y
-//      arrives as 32 times the exponent for the table exponential exp2_neg_tab; cS0 is divided
-//      by 32 to match, and the finalize divides the squared moments by 32 (SymLaunch::b2tab).
+//      In FP64 (F32 false) the scales are sqrt(256) times larger (time_scale_tab_dd), so that y
+//      arrives as 256 times the exponent for the table exponential exp2_neg_tab; cS0 is divided
+//      by 256 to match, and the finalize divides the squared moments by 256 (SymLaunch::b2tab).
 struct PairConst {
     double cB0, cB1, cS0, cS1;
     double cS0lo, cS1lo; // base 2: the low parts of cS0 and cS1 (double-doubles), else 0
@@ -93,7 +93,7 @@ __device__ __forceinline__ double background_term(const PairConst &c, double r2,
                                                   const double *__restrict__ tab) {
     if (FAST) {
         if (F32) return (double)exp2_f32((float)(-y));
-        // y = 32 x the exponent >= 0: a sum of squares
+        // y = 256 x the exponent >= 0: a sum of squares
         return KT ? exp2_neg_ktab(y, tab) : exp2_neg_tab(y, tab);
     }
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
@@ -114,6 +114,14 @@ __device__ __forceinline__ void lds_add(double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Doubles of one staged 64-column subtile: the D coordinates and the time as rows of 128 (each
+// 64-column run stored twice, so that the per-lane read index lane + s needs no wrap) and, in the
+// base-2 form, the natural time as one row of 64, read only by subtiles that form excitation
+// terms (a small share of the pass), with a wrapped index.
+template <int D, bool FAST> struct SubStride {
+    static constexpr int value = (D + 1) * 128 + (FAST ? 64 : 0);
+};
+
 // One 64-column subtile against this lane's A rows; the column contributions are added to the
 // wave's LDS slots cs (moment c, slot u in [0, 128) at cs[c * 128 + u]).
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
@@ -130,9 +138,10 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
     // components read per column: D coordinates and the time, plus the natural time where the
     // base-2 form forms excitation terms
     constexpr int NQ = (FAST && SMODE != kSNone) ? D + 2 : D + 1;
-    // structure-of-arrays subtile: component d of entry u at sub[d * 128 + u]; the next step's
-    // entry is read before this step's slot additions are issued (entry lane + 64 of the last
-    // step's read is in the stored copy)
+    // structure-of-arrays subtile (SubStride): component d <= D of entry u at sub[d * 128 + u],
+    // the natural time at sub[(D + 1) * 128 + (u & 63)]; the next step's entry is read before this
+    // step's slot additions are issued (entry lane + 64 of the last step's read is in the stored
+    // copy, or wraps)
     double qn[NQ];
 #pragma unroll
     for (int d = 0; d < NQ; ++d) qn[d] = sub[d * 128 + lane];
@@ -141,7 +150,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
 #pragma unroll
         for (int d = 0; d < NQ; ++d) {
             q[d] = qn[d];
-            qn[d] = sub[d * 128 + lane + s + 1];
+            qn[d] = d <= D ? sub[d * 128 + lane + s + 1] : sub[d * 128 + ((lane + s + 1) & 63)];
         }
         double v[NC]; // this step's column contributions of the A rows
 #pragma unroll
@@ -447,9 +456,9 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     constexpr int NR = GRAD ? 6 : 2;
     constexpr int NC = GRAD ? 3 : 1;
     constexpr int R = kSymWaves * 64 * A;
-    // components per column of a chunk subtile: D coordinates, the time and, in the base-2 form,
-    // the natural time
-    constexpr int NCOMP = FAST ? D + 2 : D + 1;
+    // doubles per staged chunk subtile: D coordinates, the time and, in the base-2 form, the
+    // natural time
+    constexpr int SUB = SubStride<D, FAST>::value;
     const SymData &ds = L.data[it.x];
     const int P = it.y, chunk = it.z, k = it.w;
     const double *pk = params + 6 * (size_t)(ds.param0 + k);
@@ -461,8 +470,8 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 
     // base-2 form: coordinates and times relative to the block's first event, scaled by sx and st
     // (in FP64 sqrt(32) times larger: exponents times 32 for the table exponential)
-    const double2 st = FAST ? (F32 ? time_scale_dd(tauT) : time_scale32_dd(tauT)) : make_double2(1.0, 0.0);
-    const double2 sx = FAST ? (F32 ? time_scale_dd(tauX) : time_scale32_dd(tauX)) : make_double2(1.0, 0.0);
+    const double2 st = FAST ? (F32 ? time_scale_dd(tauT) : time_scale_tab_dd(tauT)) : make_double2(1.0, 0.0);
+    const double2 sx = FAST ? (F32 ? time_scale_dd(tauX) : time_scale_tab_dd(tauX)) : make_double2(1.0, 0.0);
     const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
     double x0[D];
 #pragma unroll
@@ -480,7 +489,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         const double s2 = sigma * sigma, x2 = tauX * tauX;
         const double q = s2 / x2;
         const double qlo = __builtin_fma(-q, x2, s2) / x2;
-        const double unit = F32 ? 1.0 : 0.03125;
+        const double unit = F32 ? 1.0 : 1.0 / kExp2TabSize;
         c.cS0 = -q * unit;
         c.cS0lo = -qlo * unit;
         const double w = -omega * kLog2eHi;
@@ -639,9 +648,9 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 farmask = (int)(__ballot(far && lane < 4 * A) & ((1ULL << (4 * A)) - 1ULL));
             }
             {
-                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 as NCOMP arrays of 128
-                // doubles (each 64-column run stored twice); lanes then read consecutive doubles.
-                const int e0 = (tid >> 6) * NCOMP * 128 + (tid & 63);
+                // subtile sb = tid / 64 holds columns j0 + 64 sb .. + 63 (SubStride); lanes then
+                // read consecutive doubles.
+                const int e0 = (tid >> 6) * SUB + (tid & 63);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double u = ex[(size_t)d * npad + j0 + tid];
@@ -653,17 +662,14 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 const double tv = FAST ? scale_by(tnat - t0, st) : tnat;
                 tile[e0 + D * 128] = tv;
                 tile[e0 + D * 128 + 64] = tv;
-                if (FAST) {
-                    tile[e0 + (D + 1) * 128] = tnat;
-                    tile[e0 + (D + 1) * 128 + 64] = tnat;
-                }
+                if (FAST) tile[e0 + (D + 1) * 128] = tnat;
             }
             __syncthreads();
             if (wave_live) {
                 // time-sorted passes: columns are earlier than the rows, tl <= t_lo; equal times
                 // need the mask
                 // the tile's last column, natural time (staged beside the scaled one in base 2)
-                const double tl = tile[(3 * NCOMP + (FAST ? D + 1 : D)) * 128 + 63];
+                const double tl = tile[3 * SUB + (FAST ? D + 1 : D) * 128 + 63];
                 int smode = PM == kPassCulledB ? kSNone : (tl < t_lo ? kSFull : kSMixed);
                 int dob = 1;
                 if (PM != kPassCulledB && L.skip_zero) {
@@ -674,7 +680,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
                 const int mode0 = __builtin_amdgcn_readfirstlane(dob * 4 + smode);
                 for (int sb = 0; sb < 4; ++sb) {
                     int mode = mode0;
-                    const double *sub = tile + sb * NCOMP * 128;
+                    const double *sub = tile + sb * SUB;
                     auto &ac = FLUSH ? tacc : acc;
                     if constexpr (PM == kPassCulledB && A == 2) {
                         // culled order: the subtile against the wave's row groups that are near it
@@ -773,10 +779,10 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
 // LDS of sym_kernel in doubles (below): subtiles or the diagonal tile, column slots, the
 // exponential's table (base-2 form in FP64), per-wave column sums.
 template <int D, bool GRAD, bool F32, int FM> struct SymLds {
-    static constexpr int NCOMP = FM != 0 ? D + 2 : D + 1;
+    static constexpr int SUB = SubStride<D, FM != 0>::value;
     static constexpr int ST = DiagStride<D, FM != 0>::value;
     static constexpr int NC = GRAD ? 3 : 1;
-    static constexpr int tile = (2 * kTile * NCOMP > kTile * ST) ? 2 * kTile * NCOMP : kTile * ST;
+    static constexpr int tile = (4 * SUB > kTile * ST) ? 4 * SUB : kTile * ST;
     static constexpr int slot = kSymWaves * NC * 128;
     static constexpr int tab = (FM != 0 && !F32) ? kExp2TabSize : 0;
     static constexpr int total = tile + slot + tab + kSymWaves * NC * kTile;
@@ -797,7 +803,7 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     constexpr bool TAB = FM != 0 && !F32; // the base-2 form in FP64: exp2_neg_tab
     constexpr bool KT = FM == 3;          // exp2_neg_ktab and its table
     static_assert(!KT || !F32, "the exponent-in-table form is FP64's");
-    // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x NCOMP x 128 doubles
+    // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x SubStride doubles
     constexpr int kTileD = Lds::tile;
     constexpr int kSlotD = Lds::slot;
     constexpr int kTabD = Lds::tab;
@@ -813,7 +819,7 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     // the table is complete before the first item: the queue loop's barrier follows
-    if (TAB) fill_exp2_table(lds + kTileD + kSlotD, tid, KT);
+    if (TAB) fill_exp2_table(lds + kTileD + kSlotD, tid, kSymWaves * 64, KT);
     // this wave's column slots (sym_subtile), zero between subtiles
     double *cs = colslot + w * NC * 128;
 #pragma unroll

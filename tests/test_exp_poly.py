@@ -2,10 +2,10 @@
 on the CPU: the constants of hph_exp_coeffs.h evaluated as the kernels evaluate them (the same
 reductions, Horner with correctly rounded fma, emulated at 60 digits) against mpmath at 60
 digits. exp_fp64 (Cody-Waite, natural form), exp2_fp64 (shifter, exponents of either sign) and
-exp2_neg_fp64 (2^-y for y >= 0 from trunc and fract) and exp2_neg_tab (2^-y from 32 y: a 32-entry
-table of 2^(b/32), a degree-5 polynomial and v_ldexp_f64; the FP64 background terms of the base-2
-form), and exp2_neg_ktab (the same with the power of two added into the table entry's exponent
-field) bitwise against exp2_neg_tab over its range.
+exp2_neg_fp64 (2^-y for y >= 0 from trunc and fract) and exp2_neg_tab (2^-y from 256 y: a
+256-entry table of 2^(b/256), a degree-4 polynomial and v_ldexp_f64; the FP64 background terms of
+the base-2 form), and exp2_neg_ktab (the same with the power of two added into the table entry's
+exponent field) bitwise against exp2_neg_tab over its range.
 A dropped or mistyped coefficient, a wrong sign of the 2^-f polynomial or an off-by-one in the
 reduction moves the error far above the bound.
 """
@@ -24,7 +24,7 @@ def constants():
     c = {}
     with open(HEADER) as f:
         for line in f:
-            m = re.match(r"#define (HPH_\w+) (\S+)", line)
+            m = re.match(r"#define (HPH_\w+) ([-+0-9.eE]+)$", line.strip())
             if m:
                 c[m.group(1)] = float(m.group(2))
     return c
@@ -75,20 +75,31 @@ def exp2_neg_fp64(c, y):
     return ldexp(horner(poly(c, "HPH_EXP2N_C"), f), -i)
 
 
+def tab_bits(c):
+    return int(c["HPH_EXP2T_BITS"])
+
+
 def exp2_table(c):
-    """fill_exp2_table: entry b = 2^(b/32)."""
-    return [c["HPH_EXP2T_M%d" % b] for b in range(32)]
+    """fill_exp2_table: entry b = 2^(b/S), S = 2^HPH_EXP2T_BITS."""
+    return [c["HPH_EXP2T_M%d" % b] for b in range(1 << tab_bits(c))]
 
 
-def exp2_neg_tab(c, tab, y32):
-    """2^(-y32/32): -trunc(y32) = 32 a + b, f = fract(y32), ldexp(tab[b] + tab[b] (f q(f)), a)."""
-    ni = -int(y32)  # v_cvt_i32_f64 of -y32
-    f = y32 - np.floor(y32)
-    q = c["HPH_EXP2T_C5"]
-    for k in (4, 3, 2, 1):
+def exp2t_fq(c, f):
+    """f q(f), q by Horner from its top coefficient (hph_device.h exp2t_fq)."""
+    deg = int(c["HPH_EXP2T_DEGREE"])
+    q = c["HPH_EXP2T_C%d" % deg]
+    for k in range(deg - 1, 0, -1):
         q = fma(q, f, c["HPH_EXP2T_C%d" % k])
-    t = tab[ni & 31]  # Python's & and >> on negative ints: two's complement, arithmetic shift
-    return ldexp(fma(t, float(mp.mpf(f) * mp.mpf(q)), t), ni >> 5)
+    return float(mp.mpf(f) * mp.mpf(q))
+
+
+def exp2_neg_tab(c, tab, yS):
+    """2^(-yS/S): -trunc(yS) = S a + b, f = fract(yS), ldexp(tab[b] + tab[b] (f q(f)), a)."""
+    bits = tab_bits(c)
+    ni = -int(yS)  # v_cvt_i32_f64 of -yS
+    f = yS - np.floor(yS)
+    t = tab[ni & ((1 << bits) - 1)]  # Python's & and >> on negative ints: two's complement, arithmetic shift
+    return ldexp(fma(t, exp2t_fq(c, f), t), ni >> bits)
 
 
 def rel(v, ref):
@@ -126,70 +137,76 @@ def test_exp2_neg_fp64():
 
 
 def test_exp2_table_mantissas():
-    """The 32 table entries are 2^(b/32) correctly rounded."""
+    """The 256 table entries are 2^(b/256) correctly rounded, and hph_device.h's fixed choices
+    (the byte index, the scale constant) match the header's table size."""
     c = constants()
-    for b in range(32):
-        assert c["HPH_EXP2T_M%d" % b] == float(mp.power(2, mp.mpf(b) / 32)), b
+    assert tab_bits(c) == 8
+    S = 1 << tab_bits(c)
+    for b in range(S):
+        assert c["HPH_EXP2T_M%d" % b] == float(mp.power(2, mp.mpf(b) / S)), b
 
 
 def test_exp2_neg_tab():
     c = constants()
     tab = exp2_table(c)
+    S = float(len(tab))
     rng = np.random.default_rng(4)
     ys = np.concatenate([rng.uniform(0.0, 1020.0, 300), rng.uniform(0.0, 4.0, 300),
-                         [0.0, 1.0 / 32, 1.0, 2.0, 95.0, 1e-300, 1.0 - 2.0 ** -53, 31.0 / 32, 33.0 / 32]])
+                         [0.0, 1.0 / S, 1.0, 2.0, 95.0, 1e-300, 1.0 - 2.0 ** -53, (S - 1) / S, (S + 1) / S,
+                          1.0 / 32, 31.0 / 32]])
     worst = 0.0
     for y in ys:
-        y32 = float(mp.mpf(float(y)) * 32)
-        worst = max(worst, rel(exp2_neg_tab(c, tab, y32), mp.power(2, -mp.mpf(y32) / 32)))
-    assert worst < 4e-16, worst
+        yS = float(mp.mpf(float(y)) * mp.mpf(S))
+        worst = max(worst, rel(exp2_neg_tab(c, tab, yS), mp.power(2, -mp.mpf(yS) / mp.mpf(S))))
+    assert worst < 3e-16, worst
     assert exp2_neg_tab(c, tab, 0.0) == 1.0  # the self pair's term is exactly 1
-    assert exp2_neg_tab(c, tab, 96.0) == 0.125  # 2^-3
-    assert exp2_neg_tab(c, tab, 32.0 * 1100) == 0.0  # below the FP64 range, as IEEE exp
+    assert exp2_neg_tab(c, tab, 3.0 * S) == 0.125  # 2^-3
+    assert exp2_neg_tab(c, tab, S * 1100) == 0.0  # below the FP64 range, as IEEE exp
     assert exp2_neg_tab(c, tab, 2.0 ** 31 - 1) == 0.0
 
 
-def exp2_ktable(tab):
-    """fill_exp2_table with ktab: the bit pattern of entry b with b << 15 taken off its high word."""
-    return [int(np.array([t]).view(np.uint64)[0]) - (b << 47) for b, t in enumerate(tab)]
+def exp2_ktable(c, tab):
+    """fill_exp2_table with ktab: the bit pattern of entry b with b << (20 - bits) taken off its
+    high word."""
+    sh = 32 + 20 - tab_bits(c)
+    return [int(np.array([t]).view(np.uint64)[0]) - (b << sh) for b, t in enumerate(tab)]
 
 
-def exp2_neg_ktab(c, ktab, y32):
-    """exp2_neg_ktab: as exp2_neg_tab, with (-trunc(y32)) << 15 added to the table entry's high
-    word (32-bit wrap-around) in place of the final ldexp."""
-    ni = -int(y32)
-    f = y32 - np.floor(y32)
-    q = c["HPH_EXP2T_C5"]
-    for k in (4, 3, 2, 1):
-        q = fma(q, f, c["HPH_EXP2T_C%d" % k])
-    bits = ktab[ni & 31]
-    hi = ((bits >> 32) + ((ni << 15) & 0xFFFFFFFF)) & 0xFFFFFFFF
+def exp2_neg_ktab(c, ktab, yS):
+    """exp2_neg_ktab: as exp2_neg_tab, with (-trunc(yS)) << (20 - bits) added to the table entry's
+    high word (32-bit wrap-around) in place of the final ldexp."""
+    tb = tab_bits(c)
+    ni = -int(yS)
+    f = yS - np.floor(yS)
+    bits = ktab[ni & ((1 << tb) - 1)]
+    hi = ((bits >> 32) + ((ni << (20 - tb)) & 0xFFFFFFFF)) & 0xFFFFFFFF
     t = float(np.array([(hi << 32) | (bits & 0xFFFFFFFF)], dtype=np.uint64).view(np.float64)[0])
-    return fma(t, float(mp.mpf(f) * mp.mpf(q)), t)
+    return fma(t, exp2t_fq(c, f), t)
 
 
 def test_exp2_neg_ktab_is_bitwise_exp2_neg_tab():
-    """Over its whole range (0 <= y32 <= kExp2KTabMaxY32) the exponent-in-table variant returns
+    """Over its whole range (0 <= yS <= kExp2KTabMaxY) the exponent-in-table variant returns
     exp2_neg_tab's doubles bit for bit: every entry b, both ends of the range, integer and
     fractional y32. Its range covers every background exponent up to the host's bound (ktab_ok in
     hph_engine.cpp), and the smallest result is a normal double."""
     c = constants()
     tab = exp2_table(c)
-    ktab = exp2_ktable(tab)
-    ymax = device_constants()["kExp2KTabMaxY32"]
-    assert ymax == 32.0 * 1011
+    ktab = exp2_ktable(c, tab)
+    S = len(tab)
+    ymax = device_constants()["kExp2KTabMaxY"]
+    assert ymax == S * 1011.0
     with open(os.path.join(HERE, "..", "hawkes_amd", "csrc", "hph_engine.cpp")) as f:
         src = f.read()
     body = src[src.index("static bool ktab_ok("):]
     bound = float(re.search(r"if \(!\(aB <= ([0-9.]+)\)\) return false;", body).group(1))
-    assert bound * 32 * float(mp.log(mp.e, 2)) < ymax
+    assert bound * S * float(mp.log(mp.e, 2)) < ymax
     rng = np.random.default_rng(9)
-    ys = np.concatenate([rng.uniform(0.0, ymax, 400), rng.uniform(0.0, 64.0, 200),
-                         np.arange(0.0, 70.0), [ymax, ymax - 1e-9, ymax - 31.5, 0.5, 1e-300, 31.999999]])
-    for y32 in ys:
-        y32 = float(y32)
-        a, b = exp2_neg_ktab(c, ktab, y32), exp2_neg_tab(c, tab, y32)
-        assert a == b and np.float64(a).tobytes() == np.float64(b).tobytes(), (y32, a, b)
+    ys = np.concatenate([rng.uniform(0.0, ymax, 400), rng.uniform(0.0, 2.0 * S, 200),
+                         np.arange(0.0, S + 70.0), [ymax, ymax - 1e-9, ymax - S + 0.5, 0.5, 1e-300, S - 1e-6]])
+    for yS in ys:
+        yS = float(yS)
+        a, b = exp2_neg_ktab(c, ktab, yS), exp2_neg_tab(c, tab, yS)
+        assert a == b and np.float64(a).tobytes() == np.float64(b).tobytes(), (yS, a, b)
     assert exp2_neg_ktab(c, ktab, ymax) >= 2.0 ** -1022
 
 
@@ -212,7 +229,7 @@ DD_VALUES = {
     "kSqrtHalfLog2e": lambda: mp.sqrt(mp.log(mp.e, 2) / 2),
     "kTwoLn2": lambda: 2 * mp.log(2),
     "kLog2e": lambda: 1 / mp.log(2),
-    "kSqrt16Log2e": lambda: 4 * mp.sqrt(mp.log(mp.e, 2)),
+    "kSqrt128Log2e": lambda: mp.sqrt(128 * mp.log(mp.e, 2)),
 }
 
 
@@ -228,12 +245,12 @@ def test_double_double_constants():
         assert lo == float(v - mp.mpf(hi)), name
         assert abs(mp.mpf(hi) + mp.mpf(lo) - v) <= mp.mpf(2) ** -104 * abs(v), name
     assert abs(DD_VALUES["kTwoLn2"]() * DD_VALUES["kSqrtHalfLog2e"]() ** 2 - 1) < mp.mpf(10) ** -50
-    # the table form's scale: 32 times the squared scale, so 2 ln 2 / 32 undoes it
-    assert abs(DD_VALUES["kTwoLn2"]() / 32 * DD_VALUES["kSqrt16Log2e"]() ** 2 - 1) < mp.mpf(10) ** -50
+    # the table form's scale: 256 times the squared scale, so 2 ln 2 / 256 undoes it
+    assert abs(DD_VALUES["kTwoLn2"]() / 256 * DD_VALUES["kSqrt128Log2e"]() ** 2 - 1) < mp.mpf(10) ** -50
 
 
 def time_scale_dd(c, tau, name="kSqrtHalfLog2e"):
-    """hph_device.h time_scale_dd (time_scale32_dd with name kSqrt16Log2e): |tau| times the
+    """hph_device.h time_scale_dd (time_scale_tab_dd with name kSqrt128Log2e): |tau| times the
     constant as a double-double (two-product)."""
     a = abs(tau)
     hi = float(mp.mpf(a) * mp.mpf(c[name + "Hi"]))
@@ -248,7 +265,7 @@ def test_scaling_round_trip():
     c = device_constants()
     rng = np.random.default_rng(11)
     # (scale constant, unit of from_base2_sq): FP32-bulk's base-2 form and the FP64 table form
-    for name, unit in (("kSqrtHalfLog2e", 1.0), ("kSqrt16Log2e", 1.0 / 32)):
+    for name, unit in (("kSqrtHalfLog2e", 1.0), ("kSqrt128Log2e", 1.0 / 256)):
         worst_scale = worst_back = 0.0
         for tau in np.concatenate([rng.uniform(1e-4, 10.0, 200), [1 / 336.0, 1 / 1.6, 1.0]]):
             hi, lo = time_scale_dd(c, float(tau), name)

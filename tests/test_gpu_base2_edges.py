@@ -44,7 +44,7 @@ def cull_window(n):
 def base2_gate(locs, times, p, split=False):
     """The engine's conditions for the base-2 form (fast_ok in the engine), restated: distinct
     times at least 8 ulp apart, |omega| x (largest span of 512 sorted events) <= 300, every
-    background exponent below 4e7 and excitation exponent below 1e9, sx x diagonal <= 300,
+    background exponent below 5.5e6 and excitation exponent below 1e9, sx x diagonal <= 300,
     sigma sqrt(log2(e)/2) x diagonal <= 300, and for the split passes st x (largest window time
     span) <= 300."""
     t = np.sort(np.asarray(times, dtype=float))
@@ -63,7 +63,7 @@ def base2_gate(locs, times, p, split=False):
     sigma, tauX, tauT, omega = p[0], p[1], p[2], p[3]
     st, sx = abs(tauT) * SX_PER_TAUX, abs(tauX) * SX_PER_TAUX
     ok = abs(omega) * block_span <= 300.0
-    ok &= 0.5 * tauX ** 2 * r2max + 0.5 * tauT ** 2 * span ** 2 < 4e7
+    ok &= 0.5 * tauX ** 2 * r2max + 0.5 * tauT ** 2 * span ** 2 < 5.5e6
     ok &= abs(omega) * span + 0.5 * sigma ** 2 * r2max < 1e9
     ok &= sx * diag <= 300.0 and abs(sigma) * SX_PER_TAUX * diag <= 300.0
     if split:
@@ -152,14 +152,14 @@ def test_base2_form_in_the_split_passes():
 
 
 def test_split_pass_wide_window_takes_the_natural_form():
-    """tauT = 5 / h on 8,000 events over 1,000 h: the time-sorted blocks (512 events, ~64 h)
-    pass every gate of the base-2 form, but a window of the culled order (1,024 events, ~128 h)
-    has st x span ~ 550, so the split passes must take the natural form; the time-sorted pass
-    keeps the base-2 form."""
+    """tauT = 5 / h on 8,000 events over 650 h: the time-sorted blocks (512 events, ~42 h) pass
+    every gate of the base-2 form (largest background exponent 5.3e6), but a window of the culled
+    order (1,024 events, ~83 h) has st x span ~ 350, so the split passes must take the natural
+    form; the time-sorted pass keeps the base-2 form."""
     rng = np.random.Generator(np.random.PCG64(671))
     n = 8000
     locs = rng.uniform(0.0, 4.0, size=(n, 2))
-    times = np.sort(rng.uniform(0.0, 1000.0, size=n))
+    times = np.sort(rng.uniform(0.0, 650.0, size=n))
     p = np.array([1.0, 1 / 1.6, 5.0, 1.0, 1.0, 1.0])
     assert base2_gate(locs, times, p, split=False)
     check(locs, times, p, 2, _lib.OPENCL, 1e-12, 1e-12, 1e-12, kind=_lib.PAIR_KERNEL_SPLIT, base2=False)
