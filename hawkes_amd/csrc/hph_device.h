@@ -113,11 +113,28 @@ __device__ __forceinline__ double exp2t_fq(double f) {
     for (int k = n - 2; k >= 0; --k) q = __builtin_fma(q, f, Q[k]);
     return f * q;
 }
+// The exponential in two halves, so that a pass can issue the table read of one term and finish
+// another meanwhile: exp2_tab_split reduces yS (ni = -trunc(yS), f = fract(yS)), exp2_tab_entry
+// reads tab[b], exp2_tab_finish forms the result (KT: exp2_neg_ktab's table and bound, below).
+struct Exp2TabArg {
+    int ni;
+    double f;
+};
+__device__ __forceinline__ Exp2TabArg exp2_tab_split(double yS) { return {(int)(-yS), __builtin_amdgcn_fract(yS)}; }
+__device__ __forceinline__ double exp2_tab_entry(const double *tab, int ni) { return tab[ni & (kExp2TabSize - 1)]; }
+template <bool KT> __device__ __forceinline__ double exp2_tab_finish(double t, Exp2TabArg a) {
+    const double fq = exp2t_fq(a.f);
+    if (KT) {
+        uint2 w = __builtin_bit_cast(uint2, t);
+        w.y += (unsigned)a.ni << (20 - kExp2TabBits);
+        t = __builtin_bit_cast(double, w);
+        return __builtin_fma(t, fq, t);
+    }
+    return __builtin_amdgcn_ldexp(__builtin_fma(t, fq, t), a.ni >> kExp2TabBits);
+}
 __device__ __forceinline__ double exp2_neg_tab(double yS, const double *tab) {
-    const int ni = (int)(-yS); // -trunc(yS)
-    const double fq = exp2t_fq(__builtin_amdgcn_fract(yS));
-    const double t = tab[ni & (kExp2TabSize - 1)];
-    return __builtin_amdgcn_ldexp(__builtin_fma(t, fq, t), ni >> kExp2TabBits);
+    const Exp2TabArg a = exp2_tab_split(yS);
+    return exp2_tab_finish<false>(exp2_tab_entry(tab, a.ni), a);
 }
 // exp2_neg_tab where every argument keeps the result in the normal range with margin
 // (0 <= yS <= kExp2KTabMaxY, checked by the caller for all pairs of a launch): the power 2^k,
@@ -130,12 +147,8 @@ __device__ __forceinline__ double exp2_neg_tab(double yS, const double *tab) {
 constexpr double kExp2KTabMaxY = 258816.0; // 256 x 1011: k = ni >> 8 >= -1011
 static_assert(kExp2KTabMaxY == kExp2TabSize * 1011.0, "k >= -1011");
 __device__ __forceinline__ double exp2_neg_ktab(double yS, const double *tab) {
-    const int ni = (int)(-yS); // -trunc(yS)
-    const double fq = exp2t_fq(__builtin_amdgcn_fract(yS));
-    uint2 w = __builtin_bit_cast(uint2, tab[ni & (kExp2TabSize - 1)]);
-    w.y += (unsigned)ni << (20 - kExp2TabBits);
-    const double t = __builtin_bit_cast(double, w);
-    return __builtin_fma(t, fq, t);
+    const Exp2TabArg a = exp2_tab_split(yS);
+    return exp2_tab_finish<true>(exp2_tab_entry(tab, a.ni), a);
 }
 // Fills exp2_neg_tab's table (LDS): entry b = 2^(b/256) correctly rounded (gen_exp_poly.py); with
 // ktab, exp2_neg_ktab's: the same entry with b << 12 taken off its high word (a bit pattern, not

@@ -122,6 +122,92 @@ template <int D, bool FAST> struct SubStride {
     static constexpr int value = (D + 1) * 128 + (FAST ? 64 : 0);
 };
 
+// Background-only subtile in the FP64 table form (the bulk of a likelihood pass), pipelined by one
+// step: step s + 1's geometry, its reduction and its table reads are issued before step s's
+// exponential is finished, so each term's LDS read waits out a step of arithmetic instead of
+// standing in its term's dependency chain. The pairs, the operations on each and the order of
+// every addition are those of sym_subtile: the results are bitwise the same.
+template <int D, bool GRAD, int A, bool TAIL, bool KT>
+__device__ __forceinline__ void sym_subtile_bg_tab(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
+                                                   const double (&ti)[A], const double (&vm)[A], double (&acc)[A][6],
+                                                   double *cs, const double *__restrict__ tab) {
+    constexpr int NC = GRAD ? 3 : 1;
+    constexpr int NQ = D + 1;
+    // one term in flight: y, dt'^2, the reduced exponent and the table entry
+    struct Term {
+        double y, dt2, t;
+        Exp2TabArg a;
+    };
+    auto issue = [&](const double (&q)[NQ], Term (&p)[A]) {
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            double r2, dt, dt2, y;
+            pair_geometry<D, true, false>(xi[a], ti[a], q, r2, dt, dt2, y);
+            p[a].y = y;
+            p[a].dt2 = dt2;
+            p[a].a = exp2_tab_split(y);
+            p[a].t = exp2_tab_entry(tab, p[a].a.ni);
+        }
+    };
+    Term cur[A];
+    double qn[NQ];
+    {
+        double q[NQ];
+#pragma unroll
+        for (int d = 0; d < NQ; ++d) {
+            q[d] = sub[d * 128 + lane];
+            qn[d] = sub[d * 128 + lane + 1];
+        }
+        issue(q, cur);
+    }
+    // step s: columns lane + s (cur, issued) and lane + s + 1 (qn, read). Every step issues the
+    // next one, the last step too: its read-ahead entry, lane + 65 <= 128, is the first of the
+    // subtile's next row (inside the staged tile) and its term is discarded, which costs one step's
+    // geometry per subtile and keeps the loop free of a branch.
+    auto step = [&](int s) {
+        Term nxt[A];
+        {
+            double q[NQ];
+#pragma unroll
+            for (int d = 0; d < NQ; ++d) {
+                q[d] = qn[d];
+                qn[d] = sub[d * 128 + lane + s + 2];
+            }
+            issue(q, nxt);
+        }
+        double v[NC];
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const double eb = exp2_tab_finish<KT>(cur[a].t, cur[a].a);
+            acc[a][0] += eb;
+            if (GRAD) {
+                acc[a][1] = __builtin_fma(eb, cur[a].y, acc[a][1]);
+                acc[a][2] = __builtin_fma(eb, cur[a].dt2, acc[a][2]);
+            }
+            const double ec = TAIL ? eb * vm[a] : eb;
+            if (a == 0) {
+                v[0] = ec;
+                if (GRAD) {
+                    v[1] = ec * cur[a].y;
+                    v[2] = ec * cur[a].dt2;
+                }
+            } else {
+                v[0] += ec;
+                if (GRAD) {
+                    v[1] = __builtin_fma(ec, cur[a].y, v[1]);
+                    v[2] = __builtin_fma(ec, cur[a].dt2, v[2]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) lds_add(cs + c * 128 + lane + s, v[c]);
+#pragma unroll
+        for (int a = 0; a < A; ++a) cur[a] = nxt[a];
+    };
+#pragma unroll 8
+    for (int s = 0; s < 64; ++s) step(s);
+}
+
 // One 64-column subtile against this lane's A rows; the column contributions are added to the
 // wave's LDS slots cs (moment c, slot u in [0, 128) at cs[c * 128 + u]).
 // DOB: form the background term; SMODE: excitation absent / unmasked / masked; TAIL: the block
@@ -200,7 +286,9 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
     };
     // the background-only loop (the bulk of a likelihood pass) is unrolled further: its
     // per-iteration address and constant moves are then shared by more pairs
-    if constexpr (DOB && SMODE == kSNone) {
+    if constexpr (DOB && SMODE == kSNone && FAST && !F32) {
+        sym_subtile_bg_tab<D, GRAD, A, TAIL, KT>(sub, lane, xi, ti, vm, acc, cs, tab);
+    } else if constexpr (DOB && SMODE == kSNone) {
 #pragma unroll 8
         for (int s = 0; s < 64; ++s) step(s);
     } else {
