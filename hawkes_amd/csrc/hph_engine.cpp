@@ -312,7 +312,7 @@ class SymRunner {
 
     ~SymRunner() {
         for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_,
-                          &d_flag_})
+                          &d_flag_, &d_src_, &d_qbase_, &d_oscr_})
             b->release();
         h_items_.release();
     }
@@ -322,27 +322,28 @@ class SymRunner {
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     // sums (may be null): the per-event moments of dataset 0, vector 0 (natural units, [6][npad]).
-    // host_t, host_params (both or neither): each dataset's time-sorted times and the parameter
-    // vectors on the host; with them the work queue is ordered by each item's expected cost for
-    // these parameters (order_queue) and copied to the device queue before the pass.
+    // order: the work queue is ordered on the device by each item's expected cost for these
+    // parameters (kern::launch_sym_order) before the pass; without it the pass runs whichever
+    // permutation of the plan's items the device queue holds (the order is a matter of time only).
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
                  bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, int cut_s,
                  const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
                  hipStream_t s, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr,
-                 const std::vector<const double *> *host_t = nullptr, const double *host_params = nullptr) {
+                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr, bool order = false) {
         if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
         kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
         L.cut_s = skip_zero && cut_s ? 1 : 0;
-        if (host_t != nullptr && host_params != nullptr) {
-            // (a call without host data runs whichever permutation of the plan's items the device
-            // queue holds: the order is a matter of time only)
-            if (order_queue(*host_t, host_params, dim, skip_zero, L.cut_s) || !dev_order_) {
-                check(hipMemcpyAsync(d_items_.ptr, h_items_.ptr, sizeof(int4) * plan_.items.size(),
-                                     hipMemcpyHostToDevice, s),
-                      "upload work queue order");
-                dev_order_ = true;
-            }
+        if (order) {
+            kern::SymOrder O{};
+            O.src = d_src_.as<int4>();
+            O.qbase = d_qbase_.as<int>();
+            O.thr = d_oscr_.as<int>();
+            O.key = d_oscr_.as<int>() + 2 * (size_t)onthr_;
+            O.out = d_items_.as<int4>();
+            O.nd = (int)plan_.data.size();
+            O.nthr = onthr_;
+            kern::launch_sym_order(L, O, params, s);
+            check(hipGetLastError(), "work-queue order kernel launch");
         }
         if (ev0) check(hipEventRecord(ev0, s), "hipEventRecord");
         kern::launch_sym(L, params, grad, fp32, d_row_.as<double>(), d_col_.as<double>(), s);
@@ -375,7 +376,6 @@ class SymRunner {
             check(hipMemcpyAsync(d_items_.ptr, h_items_.ptr, sizeof(int4) * plan_.items.size(), hipMemcpyHostToDevice,
                                  s),
                   "upload work queue order");
-            dev_order_ = false; // the device queue holds the culled order
         }
         // the background pass's table: the sorted table with the culled order's arrays
         kern::SymData b = plan_.data[0];
@@ -461,155 +461,36 @@ class SymRunner {
 
     const kern::SymPlan &plan() const { return plan_; }
 
-    // Orders the current plan's work queue (prepare has run for this shape) by each item's
-    // expected cost under the given parameters, heaviest first, into the pinned staging buffer
-    // that enqueue copies to the device. Items are independent (each writes its own partial-sum
-    // slots, which the finalize adds in a fixed order), so the order changes the pass's time and
-    // not its results. Plan order alone (by nominal size: chunk rows x columns) puts blocks whose
-    // earlier columns are all within the background's reach -- every tile computed -- among
-    // blocks whose chunks are mostly skipped; the queue's tail then waits on the former.
+    // The work queue's order for each call's parameters is formed on the device
+    // (kern::launch_sym_order, enqueue with order). Items are independent (each writes its own
+    // partial-sum slots, which the finalize adds in a fixed order), so the order changes the pass's
+    // time and not its results. Plan order alone (by nominal size: chunk rows x columns) puts
+    // blocks whose earlier columns are all within the background's reach -- every tile computed --
+    // among blocks whose chunks are mostly skipped; the queue's tail then waits on the former.
     // Cost of a chunk item, in tenths of a (block rows x kTile)-pair tile: the tiles that pass the
     // workgroup-wide skip test of sym_item (the same test, on the block's first time), weighted
     // 210 where the block's first row also forms excitation terms and 100 where it forms the
     // background only; skipped tiles count 1 (a flag write). Diagonal items: rows^2 pairs of both
     // terms, 210 rows^2 / (R kTile). Both tests are monotone in the tile index (times are sorted),
-    // so each (dataset, vector, block) takes two binary searches; each item's tile range and block
-    // are computed once per plan (oq_*), so a call is a pass of integer arithmetic over the items
-    // and a counting sort of the integer costs (stable: equal costs keep the plan order; costs
-    // past 65,535 are first shifted down to that range).
-    // The order is reused -- the staging buffer kept, false returned -- while the plan, the data
-    // (reset_order), the skip settings and every vector's tauT and omega (the gap scales of the
-    // two skip tests) stay within kOrderReuse of the ones it was computed for, as a sampler's
-    // steps do: recomputing and uploading it takes ~70 us per call at N = 1e5 (2.4% of the call),
-    // while a pass run in the order of vectors up to 25% away measured 0.8% slower than in its own
-    // order. Returns true when it wrote a new order.
-    static constexpr double kOrderReuse = 0.05;
-    bool order_queue(const std::vector<const double *> &ht, const double *hparams, int dim, int skip_zero, int cut_s) {
-        const size_t m = plan_.items.size();
-        if (ht.size() != plan_.data.size()) throw std::invalid_argument("order_queue: one time array per dataset");
-        {
-            size_t nv = 0;
-            for (const kern::SymData &sd : plan_.data) nv += (size_t)sd.K;
-            bool reuse = oq_valid_ && oq_items_ == m && oq_skip_ == skip_zero && oq_cut_ == cut_s &&
-                         oq_ref_.size() == 2 * nv;
-            for (size_t v = 0; reuse && v < nv; ++v) {
-                const double tt = hparams[6 * v + 2], om = hparams[6 * v + 3];
-                reuse = std::fabs(tt - oq_ref_[2 * v]) <= kOrderReuse * std::fabs(oq_ref_[2 * v]) &&
-                        std::fabs(om - oq_ref_[2 * v + 1]) <= kOrderReuse * std::fabs(oq_ref_[2 * v + 1]);
-            }
-            if (reuse) return false;
-            oq_ref_.resize(2 * nv);
-            for (size_t v = 0; v < nv; ++v) {
-                oq_ref_[2 * v] = hparams[6 * v + 2];
-                oq_ref_[2 * v + 1] = hparams[6 * v + 3];
-            }
-            oq_skip_ = skip_zero;
-            oq_cut_ = cut_s;
-        }
-        h_items_.ensure(sizeof(int4) * m);
-        const int R = plan_.R;
-        const int T = kern::kTile;
-        // first tile of each (dataset, vector, block) that is not skipped workgroup-wide, and the
-        // first with excitation terms for the block's first row
-        std::vector<size_t> base(plan_.data.size());
+    // so each (dataset, vector, block) takes two binary searches. plan_order uploads, once per
+    // plan, the plan's items and each dataset's first (vector, block) index.
+    void plan_order() {
+        const size_t m = plan_.items.size(), nd = plan_.data.size();
+        std::vector<int> qbase(nd + 1);
         size_t nthr = 0;
-        for (size_t d = 0; d < plan_.data.size(); ++d) {
-            base[d] = nthr;
+        for (size_t d = 0; d < nd; ++d) {
+            qbase[d] = (int)nthr;
             nthr += (size_t)plan_.data[d].K * plan_.data[d].nb;
         }
-        thr_.resize(2 * nthr);
-        for (size_t d = 0; d < plan_.data.size(); ++d) {
-            const kern::SymData &sd = plan_.data[d];
-            const double *t = ht[d];
-            for (int k = 0; k < sd.K; ++k) {
-                const double *pk = hparams + 6 * (size_t)(sd.param0 + k);
-                const double gb = 0.5 * pk[2] * pk[2], gs = pk[3];
-                const double zb = sd.zero_arg_b;
-                const double zs = cut_s ? kern::excitation_skip_arg(pk, dim, sd.n) : kern::kZeroExpArg;
-                for (int P = 0; P < sd.nb; ++P) {
-                    const double tb = t[std::min(P * R, sd.n - 1)];
-                    const int J = P * R / T; // earlier tiles of block P
-                    auto gap_of = [&](int j) { return tb - t[j * T + T - 1]; };
-                    auto skipped = [&](int j) {
-                        const double tl = t[j * T + T - 1], gap = tb - tl;
-                        return skip_zero && tl < tb && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs;
-                    };
-                    auto no_exc = [&](int j) {
-                        const double gap = gap_of(j);
-                        return skip_zero && gap > 0.0 && gs >= 0.0 && gs * gap >= zs;
-                    };
-                    int lo = 0, hi = J; // first j in [0, J] with !skipped(j)
-                    while (lo < hi) {
-                        const int mid = (lo + hi) / 2;
-                        if (skipped(mid)) lo = mid + 1; else hi = mid;
-                    }
-                    const size_t q = base[d] + (size_t)k * sd.nb + P;
-                    thr_[2 * q] = lo;
-                    lo = 0;
-                    hi = J;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) / 2;
-                        if (no_exc(mid)) lo = mid + 1; else hi = mid;
-                    }
-                    thr_[2 * q + 1] = lo;
-                }
-            }
-        }
-        if (oq_items_ != m) {
-            // per item: its (dataset, vector, block) index into thr_ and its tile range, or for a
-            // diagonal item a = -1 and b = its cost
-            oq_q_.resize(m);
-            oq_a_.resize(m);
-            oq_b_.resize(m);
-            for (size_t i = 0; i < m; ++i) {
-                const int4 it = plan_.items[i];
-                const kern::SymData &sd = plan_.data[it.x];
-                if (it.z < 0) {
-                    const double rows = (double)std::min(R, sd.n - it.y * R);
-                    oq_q_[i] = 0;
-                    oq_a_[i] = -1;
-                    oq_b_[i] = (int)(210.0 * rows * rows / ((double)R * T) + 0.5);
-                } else {
-                    oq_q_[i] = (int)(base[it.x] + (size_t)it.w * sd.nb + it.y);
-                    oq_a_[i] = it.z * sd.L / T;
-                    oq_b_[i] = std::min((it.z + 1) * sd.L, it.y * R) / T;
-                }
-            }
-            oq_items_ = m;
-        }
-        key_.resize(m);
-        int cmax = 0;
-        for (size_t i = 0; i < m; ++i) {
-            int c;
-            const int a = oq_a_[i], b = oq_b_[i];
-            if (a < 0) {
-                c = b;
-            } else {
-                const int q = oq_q_[i];
-                const int j1 = std::min(std::max(thr_[2 * q], a), b);      // first computed tile
-                const int j2 = std::min(std::max(thr_[2 * q + 1], j1), b); // first with excitation
-                c = (j1 - a) + 100 * (j2 - j1) + 210 * (b - j2);
-            }
-            key_[i] = c;
-            cmax = std::max(cmax, c);
-        }
-        int shift = 0;
-        while ((cmax >> shift) > 65535) ++shift;
-        const int top = cmax >> shift;
-        count_.assign(top + 2, 0);
-        for (size_t i = 0; i < m; ++i) {
-            key_[i] = top - (key_[i] >> shift); // ascending key = descending cost
-            ++count_[key_[i] + 1];
-        }
-        for (int b = 0; b <= top; ++b) count_[b + 1] += count_[b];
-        int4 *out = h_items_.as<int4>();
-        for (size_t i = 0; i < m; ++i) out[count_[key_[i]]++] = plan_.items[i];
-        oq_valid_ = true;
-        return true;
+        qbase[nd] = (int)nthr;
+        d_src_.ensure(sizeof(int4) * m);
+        check(hipMemcpy(d_src_.ptr, plan_.items.data(), sizeof(int4) * m, hipMemcpyHostToDevice), "upload plan items");
+        d_qbase_.ensure(sizeof(int) * (nd + 1));
+        check(hipMemcpy(d_qbase_.ptr, qbase.data(), sizeof(int) * (nd + 1), hipMemcpyHostToDevice),
+              "upload dataset indices");
+        d_oscr_.ensure(sizeof(int) * (2 * nthr + m));
+        onthr_ = (int)nthr;
     }
-
-    // Forgets the time-sorted passes' queue order (the data changed).
-    void reset_order() { oq_valid_ = false; }
 
     // Forgets the culled pass's order (its box table changed).
     void reset_culled_order() { culled_order_.clear(); }
@@ -715,7 +596,6 @@ class SymRunner {
         }
         h_items_.ensure(sizeof(int4) * m);
         std::memcpy(h_items_.ptr, culled_order_.data(), sizeof(int4) * m);
-        oq_valid_ = false; // the staging buffer holds the culled order now
     }
 
   private:
@@ -730,7 +610,6 @@ class SymRunner {
         const bool same = valid_ && n == n_ && K == K_ && rows_per_lane == A_ && target_items == target_ && grad == grad_;
         if (!same) {
             plan_ = kern::plan_sym(n, K, rows_per_lane, target_items, grad);
-            oq_items_ = 0;
             n_ = n;
             K_ = K;
             A_ = rows_per_lane;
@@ -739,8 +618,7 @@ class SymRunner {
             d_items_.ensure(sizeof(int4) * plan_.items.size());
             check(hipMemcpy(d_items_.ptr, plan_.items.data(), sizeof(int4) * plan_.items.size(), hipMemcpyHostToDevice),
                   "upload work items");
-            dev_order_ = false;
-            oq_valid_ = false;
+            plan_order();
             d_fin_.ensure(sizeof(int4) * plan_.fin_items.size());
             check(hipMemcpy(d_fin_.ptr, plan_.fin_items.data(), sizeof(int4) * plan_.fin_items.size(),
                             hipMemcpyHostToDevice),
@@ -828,17 +706,14 @@ class SymRunner {
     kern::SymData b_data_{};
     bool b_valid_ = false;
     DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_, d_flag_;
-    // work-queue order (order_queue): pinned staging buffer and scratch
+    // the device work-queue order (plan_order, enqueue with order): the plan's items, the
+    // datasets' first (vector, block) indices and scratch (thresholds, then the items' classes)
+    DevBuf d_src_, d_qbase_, d_oscr_;
+    int onthr_ = 0;
+    // the culled pass's order (order_queue_culled): pinned staging buffer and scratch
     HostBuf h_items_;
-    std::vector<int> thr_, count_, key_;
+    std::vector<int> count_, key_;
     std::vector<double> cost_;
-    std::vector<int> oq_q_, oq_a_, oq_b_; // per plan item (order_queue); valid for oq_items_ items
-    size_t oq_items_ = 0;                 // of the current plan (0 after a new plan)
-    // order_queue's reuse state: h_items_ holds a time-sorted order computed for oq_ref_ ((tauT,
-    // omega) per vector) and the skip settings below; dev_order_: the device queue holds h_items_
-    bool oq_valid_ = false, dev_order_ = false;
-    int oq_skip_ = -1, oq_cut_ = -1;
-    std::vector<double> oq_ref_;
     // order_queue_culled: the last order, the plan items and the (tauX, tauT) per vector it was
     // computed for
     std::vector<int4> culled_order_, culled_items_;
@@ -1321,11 +1196,9 @@ class HipHawkes : public HawkesEngine {
         check(hipMemcpyAsync(d_perm_.ptr, hp, sizeof(int) * npad_, hipMemcpyHostToDevice, stream_), "upload perm");
         check(hipStreamSynchronize(stream_), "upload sync");
         extents_ = extents_of(locs_, ht, (size_t)N, D);
-        tsorted_.assign(ht, ht + N);
         data_dirty_ = false;
         culled_valid_ = false;
         cache_valid_ = false;
-        sym_.reset_order();
     }
 
     // The culled order of the split passes (make_culled), uploaded with its group boxes.
@@ -1496,11 +1369,10 @@ class HipHawkes : public HawkesEngine {
                                        d_bsum_.as<double>(), d_sband_.as<double>(), probs_dev, w_dev, rdev, stream_,
                                        e0, e1, e2, sums, gstat_host_.data(), phost);
                 } else {
-                    const std::vector<const double *> host_t{tsorted_.data()};
                     sym_.enqueue({ref}, {kk}, embeddingDimension, pdev, grad, fp32_, cfg.rows_per_lane,
                                  cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
                                  {last_ktab_ ? (char)2 : last_fast_ ? (char)1 : (char)0},
-                                 probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums, &host_t, phost);
+                                 probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums, true);
                 }
             }
             last_sub_batch_ = sub;
@@ -1780,7 +1652,6 @@ class HipHawkes : public HawkesEngine {
     int kernel_ = HPH_PAIR_KERNEL_AUTO;
     SymRunner sym_;
     Extents extents_;
-    std::vector<double> tsorted_; // the times in ascending order (host), for the work-queue order
     std::vector<double> gstat_host_; // the culled order's box table (host), for its queue order
     int last_sub_batch_ = 0;         // vectors per launch of the last symmetric / split evaluation
     int last_k_ = 0;                 // vectors of the last full evaluation
@@ -1860,7 +1731,6 @@ class HipHawkesBatch {
         locs_.resize(nd);
         times_.resize(nd);
         sorted_.resize(nd);
-        tsorted_.resize(nd);
         split_.resize(nd);
         ext_.resize(nd);
         have_.assign(nd, false);
@@ -1913,7 +1783,6 @@ class HipHawkesBatch {
         }
         ext_[d] = extents_of(locs_[d], ht.data(), (size_t)n, D_);
         sorted_[d] = perm;
-        tsorted_[d].assign(ht.begin(), ht.begin() + n);
         if (split_[d]) split_[d]->valid = false;
         // x planes of dataset d live at d_x_ + D * off_[d]: plane q at + q * npad_[d]
         check(hipMemcpy(d_x_.as<double>() + (size_t)D_ * off_[d], hx.data(), sizeof(double) * hx.size(),
@@ -2073,12 +1942,8 @@ class HipHawkesBatch {
                 graph_ = g;
                 check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
             }
-            // the work queue's order for these parameters, into the staging buffer the graph copies
-            std::vector<const double *> host_t;
-            for (int d = 0; d < nd; ++d)
-                if (!split[d]) host_t.push_back(tsorted_[d].data());
-            if (!host_t.empty())
-                runner_.order_queue(host_t, h_params_.as<double>(), D_, (flags_ & ALL_PAIRS) ? 0 : 1, 1);
+            // the culled passes' work-queue order for these parameters, into the staging buffers
+            // the graph copies (the time-sorted pass orders its queue on the device)
             if (!(flags_ & ALL_PAIRS))
                 for (int d = 0; d < nd; ++d)
                     if (split[d])
@@ -2226,13 +2091,10 @@ class HipHawkesBatch {
                 f.push_back(fast[d]);
             }
         if (!refs.empty()) {
-            // the queue's order for this call's parameters (a captured graph copies the staging
-            // buffer that evaluate refills before each replay)
-            std::vector<const double *> host_t;
-            for (size_t d = 0; d < counts_.size(); ++d)
-                if (!split[d]) host_t.push_back(tsorted_[d].data());
+            // the queue's order for this call's parameters, formed on the device from the uploaded
+            // parameters (inside a captured graph too)
             runner_.enqueue(refs, K, D_, params, grad_, fp32_, A_, target_, skip_zero, 1, f, nullptr, nullptr, result,
-                            stream_, nullptr, nullptr, nullptr, nullptr, &host_t, h_params_.as<double>());
+                            stream_, nullptr, nullptr, nullptr, nullptr, true);
         }
         for (size_t d = 0; d < counts_.size(); ++d)
             if (split[d]) {
@@ -2274,7 +2136,6 @@ class HipHawkesBatch {
     std::vector<int> npad_;
     std::vector<std::vector<double>> locs_, times_;
     std::vector<std::vector<int>> sorted_; // per dataset: time-sorted position -> caller's index
-    std::vector<std::vector<double>> tsorted_; // per dataset: its times in ascending order
     std::vector<Extents> ext_;
     std::vector<bool> have_;
     std::vector<int> K_;

@@ -259,6 +259,24 @@ void launch_sym_finalize(const SymLaunch &L, const double *params, bool grad, co
 void launch_sym_bsums(const SymLaunch &L, const double *params, bool grad, const double *rowpart,
                       const double *colpart, double *bsum, hipStream_t s);
 void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, double *result, hipStream_t s);
+
+// Work-queue order of a time-sorted pass for the call's parameters, formed on the device before
+// the pass: each item's expected cost counts its tiles that pass the pass's own workgroup-wide
+// skip tests (1 per skipped tile, 100 per tile with background terms only, 210 per tile with both
+// terms; a diagonal item's cost is fixed by its rows), found by binary search on the block's first
+// time (the tests are monotone in the tile index), and the items are counting-sorted by cost,
+// heaviest first, into L.item. Items of one cost class may land in any order: every item writes
+// fixed locations, so the results do not depend on the order (only the pass's tail does).
+struct SymOrder {
+    const int4 *src;  // the plan's items, in plan order
+    const int *qbase; // per dataset: its first (vector, block) index (nd + 1 entries, the last = nthr)
+    int *thr;         // scratch, 2 ints per (dataset, vector, block): the first tile not skipped and
+                      // the first with excitation terms (used when they do not fit in LDS)
+    int *key;         // scratch, 1 int per item (plans of more than 16 x 1024 items)
+    int4 *out;        // the queue the pass reads (L.item)
+    int nd, nthr;
+};
+void launch_sym_order(const SymLaunch &L, const SymOrder &O, const double *params, hipStream_t s);
 // The work-queue counter (SymLaunch::counter) is zero before every persistent pass and is
 // returned to zero by the pass itself (queue_next, hph_sym_device.h): no reset between launches.
 // Symmetric location-gradient pass over dataset 0, vector 0 of a plan made with grad = true

@@ -307,5 +307,202 @@ void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, d
                            block_out, result);
 }
 
+// The work-queue order (SymOrder), one workgroup: per (dataset, vector, block) the two tile
+// thresholds, per item its cost, then a counting sort over cost classes. Only a few rounds of
+// global loads may stand in its path (the pass waits for it): the searches run on the tiles' last
+// times staged in LDS (up to kOrderTiles of them across the datasets; beyond that they read the
+// times in device memory), the thresholds stay in LDS (up to kOrderThr (dataset, vector, block)s),
+// and with PER > 0 every thread holds its PER items (i = tid + j kOrderThreads) in registers from
+// one round of loads to the final scatter; PER = 0 keeps the classes in device memory (plans of
+// more than 16 x 1024 items: K-batches, dataset batches). Classes: kOrderClasses linear cost
+// classes, each split by the lane that holds the item, so that the 64 lanes of one LDS atomic
+// never add to one counter (items of one cost would otherwise serialise it 64-fold).
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderClasses = 64;
+constexpr int kOrderBins = kOrderClasses * 64;
+constexpr int kOrderTiles = 4096;
+constexpr int kOrderThr = 1920;
+template <int PER>
+__global__ __launch_bounds__(kOrderThreads) void sym_order_kernel(SymLaunch L, SymOrder O,
+                                                                  const double *__restrict__ params) {
+    __shared__ int hist[kOrderBins];
+    __shared__ double tend[kOrderTiles];
+    __shared__ int lthr[2 * kOrderThr];
+    __shared__ int wred[kOrderThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int T = kTile, R = L.R, m = L.items;
+    // the tables do not overlap: restrict lets the loads of later items go ahead of earlier stores
+    const int4 *__restrict__ src = O.src;
+    int *__restrict__ gthr = O.thr;
+    int *__restrict__ key = O.key;
+    int4 *__restrict__ out = O.out;
+    // this thread's items, loaded before anything waits on them
+    int4 it[PER > 0 ? PER : 1];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = tid + j * kOrderThreads;
+        if (i < m) it[j] = src[i];
+    }
+    // tile-end times: dataset d's tile j (j < n_d / T) at tend[tb_d + j]
+    int tiles = 0;
+    for (int d = 0; d < O.nd; ++d) tiles += L.data[d].n / T;
+    const bool staged = tiles <= kOrderTiles;
+    if (staged) {
+        int tb = 0;
+        for (int d = 0; d < O.nd; ++d) {
+            const SymData &sd = L.data[d];
+            const int nt = sd.n / T;
+            for (int j = tid; j < nt; j += kOrderThreads) tend[tb + j] = sd.t[j * T + T - 1];
+            tb += nt;
+        }
+    }
+    for (int b = tid; b < kOrderBins; b += kOrderThreads) hist[b] = 0;
+    int *const thr = O.nthr <= kOrderThr ? lthr : gthr;
+    __syncthreads();
+    for (int q = tid; q < O.nthr; q += kOrderThreads) {
+        int d = 0, tb = 0;
+        while (d + 1 < O.nd && O.qbase[d + 1] <= q) {
+            tb += L.data[d].n / T;
+            ++d;
+        }
+        const SymData &sd = L.data[d];
+        const int local = q - O.qbase[d];
+        const int k = local / sd.nb, P = local - k * sd.nb;
+        const double *pk = params + 6 * (size_t)(sd.param0 + k);
+        const double gb = 0.5 * pk[2] * pk[2], gs = pk[3];
+        const double zb = sd.zero_arg_b;
+        const double zs = L.cut_s ? excitation_skip_arg(pk, L.dim, sd.n) : kZeroExpArg;
+        const double *t = sd.t;
+        const double tb0 = t[min(P * R, sd.n - 1)];
+        // the block's earlier tiles j < J; their last events j T + T - 1 < P R <= n - 1
+        const int J = P * R / T;
+        auto tile_end = [&](int j) { return staged ? tend[tb + j] : t[j * T + T - 1]; };
+        // the pass's workgroup-wide tests (sym_item), both monotone in j: skipped for j below the
+        // first threshold, excitation terms from the second on
+        int lo = 0, hi = J;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const double tl = tile_end(mid), gap = tb0 - tl;
+            const bool skipped = L.skip_zero && tl < tb0 && gb * gap * gap >= zb && gs >= 0.0 && gs * gap >= zs;
+            if (skipped) lo = mid + 1; else hi = mid;
+        }
+        thr[2 * q] = lo;
+        lo = 0;
+        hi = J;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const double gap = tb0 - tile_end(mid);
+            const bool no_exc = L.skip_zero && gap > 0.0 && gs >= 0.0 && gs * gap >= zs;
+            if (no_exc) lo = mid + 1; else hi = mid;
+        }
+        thr[2 * q + 1] = lo;
+    }
+    __syncthreads();
+    // an item's cost: its chunk's tiles against its block's thresholds (tenths of a tile: 1 per
+    // skipped tile, 10 x 10 with background terms only, 10 x 21 with both); a diagonal item
+    // 210 rows^2 / (R kTile)
+    auto item_cost = [&](int4 v) {
+        const SymData &sd = L.data[v.x];
+        if (v.z < 0) {
+            const double rows = (double)min(R, sd.n - v.y * R);
+            return (int)(210.0 * rows * rows / ((double)R * T) + 0.5);
+        }
+        const int q = O.qbase[v.x] + v.w * sd.nb + v.y;
+        const int a = v.z * sd.L / T, b = min((v.z + 1) * sd.L, v.y * R) / T;
+        const int j1 = min(max(thr[2 * q], a), b);     // first computed tile
+        const int j2 = min(max(thr[2 * q + 1], j1), b); // first with excitation terms
+        return (j1 - a) + 100 * (j2 - j1) + 210 * (b - j2);
+    };
+    int cmax = 0;
+    int c[PER > 0 ? PER : 1];
+    if (PER > 0) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            c[j] = tid + j * kOrderThreads < m ? item_cost(it[j]) : 0;
+            cmax = max(cmax, c[j]);
+        }
+    } else {
+        for (int i = tid; i < m; i += kOrderThreads) {
+            const int ci = item_cost(src[i]);
+            key[i] = ci;
+            cmax = max(cmax, ci);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o));
+    if (lane == 0) wred[w] = cmax;
+    __syncthreads();
+    cmax = 0;
+    for (int v = 0; v < kOrderThreads / 64; ++v) cmax = max(cmax, wred[v]);
+    // bin = (class, lane); ascending class = descending cost
+    auto bin_of = [&](int ci) {
+        const int cls = (int)((long long)ci * (kOrderClasses - 1) / (cmax > 0 ? cmax : 1));
+        return (kOrderClasses - 1 - cls) * 64 + lane;
+    };
+    if (PER > 0) {
+        // independent LDS additions, issued back to back
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            c[j] = bin_of(c[j]);
+            if (tid + j * kOrderThreads < m) atomicAdd(&hist[c[j]], 1);
+        }
+    } else {
+        for (int i = tid; i < m; i += kOrderThreads) {
+            const int b = bin_of(key[i]);
+            key[i] = b;
+            atomicAdd(&hist[b], 1);
+        }
+    }
+    __syncthreads();
+    // exclusive prefix sums of the bins: kOrderBins / kOrderThreads consecutive bins per thread,
+    // an inclusive scan across each wave, then across the waves
+    {
+        constexpr int per = kOrderBins / kOrderThreads;
+        int v[per], sum = 0;
+#pragma unroll
+        for (int b = 0; b < per; ++b) {
+            v[b] = hist[tid * per + b];
+            sum += v[b];
+        }
+        int incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
+        }
+        __syncthreads(); // every wave has read the maxima
+        if (lane == 63) wred[w] = incl;
+        __syncthreads();
+        int run = incl - sum;
+        for (int v2 = 0; v2 < w; ++v2) run += wred[v2];
+#pragma unroll
+        for (int b = 0; b < per; ++b) {
+            hist[tid * per + b] = run;
+            run += v[b];
+        }
+    }
+    __syncthreads();
+    if (PER > 0) {
+        // every slot taken first (independent LDS atomics, one wait), then the stores
+        int slot[PER > 0 ? PER : 1];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) slot[j] = tid + j * kOrderThreads < m ? atomicAdd(&hist[c[j]], 1) : 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (tid + j * kOrderThreads < m) out[slot[j]] = it[j];
+    } else {
+        for (int i = tid; i < m; i += kOrderThreads) out[atomicAdd(&hist[key[i]], 1)] = src[i];
+    }
+}
+
+void launch_sym_order(const SymLaunch &L, const SymOrder &O, const double *params, hipStream_t s) {
+    if (L.items < 1 || O.nd < 1 || O.out == nullptr) throw std::invalid_argument("work-queue order: empty plan");
+    const int per = (L.items + kOrderThreads - 1) / kOrderThreads;
+    const dim3 g(1), b(kOrderThreads);
+    if (per <= 4) hipLaunchKernelGGL(sym_order_kernel<4>, g, b, 0, s, L, O, params);
+    else if (per <= 8) hipLaunchKernelGGL(sym_order_kernel<8>, g, b, 0, s, L, O, params);
+    else if (per <= 12) hipLaunchKernelGGL(sym_order_kernel<12>, g, b, 0, s, L, O, params);
+    else if (per <= 16) hipLaunchKernelGGL(sym_order_kernel<16>, g, b, 0, s, L, O, params);
+    else hipLaunchKernelGGL(sym_order_kernel<0>, g, b, 0, s, L, O, params);
+}
+
 } // namespace kern
 } // namespace hph

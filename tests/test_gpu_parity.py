@@ -288,9 +288,10 @@ def test_repeat_calls_are_bitwise_identical():
 
 
 def test_reused_queue_order_leaves_results_unchanged():
-    """Full passes at tauT steps of 1-2% (the work-queue order of the first is reused: every
-    vector within 5%), then a 40% jump (a new order), then a step back: each result bitwise equal
-    to a fresh engine's first call at the same parameters, and to the oracle at 1e-12."""
+    """Full passes at tauT steps of 1-2%, then a 40% jump, then a step back: every call orders its
+    work queue on the device for its parameters (items of one cost class in any order), and each
+    result is bitwise equal to a fresh engine's first call at the same parameters, and to the
+    oracle at 1e-12."""
     locs, times = synth.cluster_process(6000, side=8.0, seed=23)
     base = np.array(synth.P0, dtype=float)
     e = make_engine(locs, times, base, kernel="symmetric")
