@@ -1043,8 +1043,10 @@ class HipHawkes : public HawkesEngine {
 
     void setTiming(bool enabled) override {
         check(hipSetDevice(device_), "hipSetDevice");
+        // the events only time the stream's work (read after a stream synchronisation): no
+        // system-scope cache write-back at each of them
         if (enabled && !ev_[0])
-            for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+            for (auto &e : ev_) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
         timing_ = enabled;
     }
 
@@ -1737,7 +1739,8 @@ class HipHawkesBatch {
         d_x_.ensure(sizeof(double) * (size_t)D * off_[nd]);
         d_t_.ensure(sizeof(double) * (size_t)off_[nd]);
         d_perm_.ensure(sizeof(int) * (size_t)off_[nd]);
-        for (auto &e : ev_) check(hipEventCreate(&e), "hipEventCreate");
+        // timing only (read after a stream synchronisation): no system-scope fence
+        for (auto &e : ev_) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
     }
 
     ~HipHawkesBatch() {
