@@ -150,6 +150,16 @@ __device__ __forceinline__ double exp2_neg_ktab(double yS, const double *tab) {
     const Exp2TabArg a = exp2_tab_split(yS);
     return exp2_tab_finish<true>(exp2_tab_entry(tab, a.ni), a);
 }
+// exp2_neg_tab's result from exp2_neg_ktab's table, for any yS >= 0 (also past exp2_neg_ktab's
+// range): the entry's b << 12, taken off its high word in that table, is added back first (two
+// integer operations), then the ldexp form. Kernels that hold the ktab table use it for the
+// terms whose exponents the host does not bound (the excitation terms).
+__device__ __forceinline__ double exp2_neg_tab_on_ktab(double yS, const double *tab) {
+    const Exp2TabArg a = exp2_tab_split(yS);
+    uint2 w = __builtin_bit_cast(uint2, exp2_tab_entry(tab, a.ni));
+    w.y += (unsigned)(a.ni & (kExp2TabSize - 1)) << (20 - kExp2TabBits);
+    return exp2_tab_finish<false>(__builtin_bit_cast(double, w), a);
+}
 // Fills exp2_neg_tab's table (LDS): entry b = 2^(b/256) correctly rounded (gen_exp_poly.py); with
 // ktab, exp2_neg_ktab's: the same entry with b << 12 taken off its high word (a bit pattern, not
 // used as a number). Threads tid < 256 of the workgroup write it (a workgroup of 256 threads

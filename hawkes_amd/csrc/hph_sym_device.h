@@ -43,12 +43,14 @@ namespace kern {
 //      cS1 = -omega log2(e) and dt the difference of the natural times (staged beside the
 //      scaled ones: a scaled difference carries the rounding of st (t - t0), which an
 //      excitation term would feel as a relative error of order eps omega |t - t0|), evaluated
-//      by exp2_fp64. The background moments are summed as e y and e dt'^2, the excitation ones
+//      by exp2_fp64 in FP32-bulk. The background moments are summed as e y and e dt'^2, the excitation ones
 //      as e r'^2 and e dt; the finalize kernels turn them into e r^2 = (e y - e dt'^2) / sx^2,
 //      e dt^2, e r^2 and e dt.
 //      In FP64 (F32 false) the scales are sqrt(256) times larger (time_scale_tab_dd), so that y
-//      arrives as 256 times the exponent for the table exponential exp2_neg_tab; cS0 is divided
-//      by 256 to match, and the finalize divides the squared moments by 256 (SymLaunch::b2tab).
+//      arrives as 256 times the exponent for the table exponential exp2_neg_tab, and the finalize
+//      divides the squared moments by 256 (SymLaunch::b2tab); the excitation constants are then
+//      -256 times the base-2 ones against r'^2 (cS0 = sigma^2 / tauX^2, cS1 = 256 omega log2(e)),
+//      so that the excitation exponent too arrives for exp2_neg_tab (excitation_term).
 struct PairConst {
     double cB0, cB1, cS0, cS1;
     double cS0lo, cS1lo; // base 2: the low parts of cS0 and cS1 (double-doubles), else 0
@@ -99,12 +101,22 @@ __device__ __forceinline__ double background_term(const PairConst &c, double r2,
     return background_exp<F32>(c.cB0, c.cB1, r2, dt2);
 }
 
-template <bool FAST>
-__device__ __forceinline__ double excitation_term(const PairConst &c, double r2, double dt) {
+// Excitation term of one pair. Base 2 (double-double constants: the exponent's scale matches the
+// finalize's sigma and omega): FP32-bulk evaluates 2^(cS0 r'^2 + cS1 dt) with exp2_fp64; FP64 (the
+// table form, TAB) forms yS = cS0 r'^2 + cS1 dt = -256 x the base-2 exponent and evaluates
+// exp2_neg_tab(yS), 2^(-yS/256) (KT: the table is exp2_neg_ktab's, whose range bounds only the
+// background exponents: exp2_neg_tab_on_ktab). For a causal pair yS >= 0; pairs with dt <= 0,
+// whose yS may be negative (the result then finite or +inf, not 2^(-yS/256)), are masked by a
+// select by every caller; yS past 2^31 (terms far below the FP64 range) converts to INT_MIN and
+// gives +0.
+template <bool FAST, bool TAB, bool KT>
+__device__ __forceinline__ double excitation_term(const PairConst &c, double r2, double dt,
+                                                  const double *__restrict__ tab) {
     if (FAST) {
-        // double-double constants: the exponent's scale matches the finalize's sigma and omega
         const double u = __builtin_fma(c.cS1, dt, c.cS1lo * dt);
-        return exp2_fp64(__builtin_fma(c.cS0, r2, __builtin_fma(c.cS0lo, r2, u)));
+        const double y = __builtin_fma(c.cS0, r2, __builtin_fma(c.cS0lo, r2, u));
+        if (!TAB) return exp2_fp64(y);
+        return KT ? exp2_neg_tab_on_ktab(y, tab) : exp2_neg_tab(y, tab);
     }
     return exp_fp64(__builtin_fma(c.cS0, r2, c.cS1 * dt));
 }
@@ -270,7 +282,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                 }
             }
             if (SMODE != kSNone) {
-                double es = excitation_term<FAST>(c, r2, dte);
+                double es = excitation_term<FAST, FAST && !F32, KT>(c, r2, dte, tab);
                 if (SMODE == kSMixed) es = dte > 0.0 ? es : 0.0;
                 acc[a][3] += es;
                 if (GRAD) {
@@ -329,7 +341,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
             }
         }
         if (PM != kPassCulledB) {
-            double es = excitation_term<FAST>(c, r2, dte);
+            double es = excitation_term<FAST, FAST && !F32, KT>(c, r2, dte, c.tab);
             es = dte > 0.0 ? es : 0.0;
             acc[3] += es;
             if (GRAD) {
@@ -557,7 +569,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     const int P0 = P * R;
 
     // base-2 form: coordinates and times relative to the block's first event, scaled by sx and st
-    // (in FP64 sqrt(32) times larger: exponents times 32 for the table exponential)
+    // (in FP64 sqrt(256) times larger: exponents times 256 for the table exponential)
     const double2 st = FAST ? (F32 ? time_scale_dd(tauT) : time_scale_tab_dd(tauT)) : make_double2(1.0, 0.0);
     const double2 sx = FAST ? (F32 ? time_scale_dd(tauX) : time_scale_tab_dd(tauX)) : make_double2(1.0, 0.0);
     const double t0 = FAST ? et[min(P0, n - 1)] : 0.0;
@@ -572,17 +584,27 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         c.cB0 = 1.0;
         c.cB1 = 0.0;
         // -sigma^2 / tauX^2 with sigma^2 and tauX^2 rounded as the finalize rounds them, the
-        // quotient carried as a double-double (divided by 32, exactly, against the table form's
-        // r'^2); -omega log2(e) per natural time difference, the same
+        // quotient carried as a double-double; -omega log2(e) per natural time difference, the
+        // same. In FP64 (the table form) both are multiplied by -256, exactly: against the
+        // table form's r'^2 (256 times the base-2 units) the first becomes sigma^2 / tauX^2 itself,
+        // and the excitation exponent arrives as yS = -256 x (its base-2 value) >= 0 for
+        // exp2_neg_tab (excitation_term)
         const double s2 = sigma * sigma, x2 = tauX * tauX;
         const double q = s2 / x2;
         const double qlo = __builtin_fma(-q, x2, s2) / x2;
-        const double unit = F32 ? 1.0 : 1.0 / kExp2TabSize;
-        c.cS0 = -q * unit;
-        c.cS0lo = -qlo * unit;
         const double w = -omega * kLog2eHi;
-        c.cS1 = w;
-        c.cS1lo = __builtin_fma(-omega, kLog2eHi, -w) + (-omega * kLog2eLo);
+        const double wlo = __builtin_fma(-omega, kLog2eHi, -w) + (-omega * kLog2eLo);
+        if (F32) {
+            c.cS0 = -q;
+            c.cS0lo = -qlo;
+            c.cS1 = w;
+            c.cS1lo = wlo;
+        } else {
+            c.cS0 = q;
+            c.cS0lo = qlo;
+            c.cS1 = -w * kExp2TabSize;
+            c.cS1lo = -wlo * kExp2TabSize;
+        }
     } else {
         c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
         c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);

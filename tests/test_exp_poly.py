@@ -210,6 +210,36 @@ def test_exp2_neg_ktab_is_bitwise_exp2_neg_tab():
     assert exp2_neg_ktab(c, ktab, ymax) >= 2.0 ** -1022
 
 
+def exp2_neg_tab_on_ktab(c, ktab, yS):
+    """exp2_neg_tab_on_ktab: the ktab entry with b << (20 - bits) added back to its high word, then
+    exp2_neg_tab's ldexp form."""
+    tb = tab_bits(c)
+    ni = -int(yS)
+    f = yS - np.floor(yS)
+    b = ni & ((1 << tb) - 1)
+    bits = ktab[b]
+    hi = ((bits >> 32) + (b << (20 - tb))) & 0xFFFFFFFF
+    t = float(np.array([(hi << 32) | (bits & 0xFFFFFFFF)], dtype=np.uint64).view(np.float64)[0])
+    return ldexp(fma(t, exp2t_fq(c, f), t), ni >> tb)
+
+
+def test_exp2_neg_tab_on_ktab_is_bitwise_exp2_neg_tab():
+    """The excitation terms of the exponent-in-table kernels read exp2_neg_ktab's table but may
+    lie far past its range: restoring each entry gives exp2_neg_tab's doubles bit for bit, past
+    the range too, down to the subnormals and +0."""
+    c = constants()
+    tab = exp2_table(c)
+    ktab = exp2_ktable(c, tab)
+    S = float(len(tab))
+    rng = np.random.default_rng(10)
+    ys = np.concatenate([rng.uniform(0.0, S * 1100.0, 300), rng.uniform(0.0, 2.0 * S, 200),
+                         np.arange(0.0, S + 8.0), [S * 1011.0, S * 1021.5, S * 1050.0, S * 1074.9, 2.0 ** 31 - 1]])
+    for yS in ys:
+        yS = float(yS)
+        a, b = exp2_neg_tab_on_ktab(c, ktab, yS), exp2_neg_tab(c, tab, yS)
+        assert np.float64(a).tobytes() == np.float64(b).tobytes(), (yS, a, b)
+
+
 # ---- double-double constants of the base-2 form (hph_device.h) ----
 
 DEVICE_H = os.path.join(HERE, "..", "hawkes_amd", "csrc", "hph_device.h")
