@@ -312,7 +312,7 @@ class SymRunner {
 
     ~SymRunner() {
         for (DevBuf *b : {&d_data_, &d_data_b_, &d_items_, &d_fin_, &d_res_, &d_counter_, &d_row_, &d_col_, &d_block_,
-                          &d_flag_, &d_src_, &d_qbase_, &d_oscr_})
+                          &d_flag_, &d_src_, &d_qbase_, &d_oscr_, &d_xs_})
             b->release();
         h_items_.release();
     }
@@ -322,24 +322,24 @@ class SymRunner {
     // dataset 0, vector 0.
     // ev0/ev1/ev2 (may be null) are recorded before the pass, after it, and after the reduction.
     // sums (may be null): the per-event moments of dataset 0, vector 0 (natural units, [6][npad]).
-    // order: the work queue is ordered on the device by each item's expected cost for these
-    // parameters (kern::launch_sym_order) before the pass; without it the pass runs whichever
-    // permutation of the plan's items the device queue holds (the order is a matter of time only).
+    // Before the pass the work queue is ordered on the device by each item's expected cost for
+    // these parameters, and each vector's excitation skip point formed (kern::launch_sym_order).
     void enqueue(const std::vector<DataRef> &refs, const std::vector<int> &K, int dim, const double *params,
                  bool grad, bool fp32, int rows_per_lane, int target_items, int skip_zero, int cut_s,
                  const std::vector<char> &fast, double *probs_dev, double *inv_lam_dev, double *result,
                  hipStream_t s, hipEvent_t ev0 = nullptr,
-                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr, bool order = false) {
+                 hipEvent_t ev1 = nullptr, hipEvent_t ev2 = nullptr, double *sums = nullptr) {
         if (fast.size() != refs.size()) throw std::invalid_argument("one base-2 flag per dataset");
         kern::SymLaunch L = prepare(refs, K, dim, grad, fp32, rows_per_lane, target_items, skip_zero, fast);
         L.cut_s = skip_zero && cut_s ? 1 : 0;
-        if (order) {
+        {
             kern::SymOrder O{};
             O.src = d_src_.as<int4>();
             O.qbase = d_qbase_.as<int>();
             O.thr = d_oscr_.as<int>();
             O.key = d_oscr_.as<int>() + 2 * (size_t)onthr_;
             O.out = d_items_.as<int4>();
+            O.xs = d_xs_.as<double>();
             O.nd = (int)plan_.data.size();
             O.nthr = onthr_;
             kern::launch_sym_order(L, O, params, s);
@@ -490,6 +490,9 @@ class SymRunner {
               "upload dataset indices");
         d_oscr_.ensure(sizeof(int) * (2 * nthr + m));
         onthr_ = (int)nthr;
+        size_t nv = 0;
+        for (size_t d = 0; d < nd; ++d) nv += (size_t)plan_.data[d].K;
+        d_xs_.ensure(sizeof(double) * std::max<size_t>(nv, 1));
     }
 
     // Forgets the culled pass's order (its box table changed).
@@ -676,6 +679,7 @@ class SymRunner {
         L.item = d_items_.as<int4>();
         L.items = (int)plan_.items.size();
         L.counter = d_counter_.as<int>();
+        L.xs = d_xs_.as<double>(); // written by the order kernel before a pass in time order
         L.grid = std::min(L.items, std::max(cus_, 1) * 4);
         L.dim = dim;
         L.A = plan_.A;
@@ -708,7 +712,7 @@ class SymRunner {
     DevBuf d_data_, d_data_b_, d_items_, d_fin_, d_res_, d_counter_, d_row_, d_col_, d_block_, d_flag_;
     // the device work-queue order (plan_order, enqueue with order): the plan's items, the
     // datasets' first (vector, block) indices and scratch (thresholds, then the items' classes)
-    DevBuf d_src_, d_qbase_, d_oscr_;
+    DevBuf d_src_, d_qbase_, d_oscr_, d_xs_;
     int onthr_ = 0;
     // the culled pass's order (order_queue_culled): pinned staging buffer and scratch
     HostBuf h_items_;
@@ -1374,7 +1378,7 @@ class HipHawkes : public HawkesEngine {
                     sym_.enqueue({ref}, {kk}, embeddingDimension, pdev, grad, fp32_, cfg.rows_per_lane,
                                  cfg.target_blocks, skip_zero, exact_s_ ? 0 : 1,
                                  {last_ktab_ ? (char)2 : last_fast_ ? (char)1 : (char)0},
-                                 probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums, true);
+                                 probs_dev, w_dev, rdev, stream_, e0, e1, e2, sums);
                 }
             }
             last_sub_batch_ = sub;
@@ -2097,7 +2101,7 @@ class HipHawkesBatch {
             // the queue's order for this call's parameters, formed on the device from the uploaded
             // parameters (inside a captured graph too)
             runner_.enqueue(refs, K, D_, params, grad_, fp32_, A_, target_, skip_zero, 1, f, nullptr, nullptr, result,
-                            stream_, nullptr, nullptr, nullptr, nullptr, true);
+                            stream_, nullptr, nullptr, nullptr, nullptr);
         }
         for (size_t d = 0; d < counts_.size(); ++d)
             if (split[d]) {

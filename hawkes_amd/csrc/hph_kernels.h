@@ -204,6 +204,9 @@ struct SymLaunch {
                             // exponential exp2_neg_tab); its squared moments are 256 times the
                             // unscaled ones. 0: FP32-bulk (unscaled). Set from the precision by the
                             // host; launch_sym checks that it matches.
+    const double *xs;       // per parameter vector (index SymData::param0 + k): its excitation skip
+                            // point excitation_skip_arg, written by launch_sym_order before a
+                            // pass in time order (read where cut_s is set)
     int ktab;               // 1: every background exponent of every pair of the launch lies in
                             // exp2_neg_ktab's range (host bound from the data's extents and each
                             // vector's tauX, tauT; FP64 base-2 form, time-sorted pass only): the
@@ -266,7 +269,9 @@ void launch_sym_reduce(const SymLaunch &L, bool grad, const double *block_out, d
 // terms; a diagonal item's cost is fixed by its rows), found by binary search on the block's first
 // time (the tests are monotone in the tile index), and the items are counting-sorted by cost,
 // heaviest first, into L.item. Items of one cost class may land in any order: every item writes
-// fixed locations, so the results do not depend on the order (only the pass's tail does).
+// fixed locations, so the results do not depend on the order (only the pass's tail does). It also
+// writes each parameter vector's excitation skip point (O.xs), which the pass then reads instead
+// of evaluating its logarithms per work item.
 struct SymOrder {
     const int4 *src;  // the plan's items, in plan order
     const int *qbase; // per dataset: its first (vector, block) index (nd + 1 entries, the last = nthr)
@@ -274,6 +279,7 @@ struct SymOrder {
                       // the first with excitation terms (used when they do not fit in LDS)
     int *key;         // scratch, 1 int per item (plans of more than 16 x 1024 items)
     int4 *out;        // the queue the pass reads (L.item)
+    double *xs;       // per parameter vector: its excitation skip point (the pass's L.xs)
     int nd, nthr;
 };
 void launch_sym_order(const SymLaunch &L, const SymOrder &O, const double *params, hipStream_t s);

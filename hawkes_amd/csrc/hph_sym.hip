@@ -371,7 +371,9 @@ __global__ __launch_bounds__(kOrderThreads) void sym_order_kernel(SymLaunch L, S
         const double *pk = params + 6 * (size_t)(sd.param0 + k);
         const double gb = 0.5 * pk[2] * pk[2], gs = pk[3];
         const double zb = sd.zero_arg_b;
-        const double zs = L.cut_s ? excitation_skip_arg(pk, L.dim, sd.n) : kZeroExpArg;
+        const double xs = excitation_skip_arg(pk, L.dim, sd.n);
+        if (P == 0) O.xs[sd.param0 + k] = xs; // one writer per vector
+        const double zs = L.cut_s ? xs : kZeroExpArg;
         const double *t = sd.t;
         const double tb0 = t[min(P * R, sd.n - 1)];
         // the block's earlier tiles j < J; their last events j T + T - 1 < P R <= n - 1

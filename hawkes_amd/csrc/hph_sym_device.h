@@ -615,7 +615,7 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
     // gap alone drives every background exponent tauT^2 gap^2 / 2 past the negligible point or
     // every excitation exponent omega gap past the cut
     const double zb = ds.zero_arg_b;
-    const double zs = L.cut_s ? excitation_skip_arg(pk, D, n) : L.zero_arg_s;
+    const double zs = L.cut_s ? L.xs[ds.param0 + k] : L.zero_arg_s; // (launch_sym_order's)
     const double gb = 0.5 * tauT * tauT; // background exponent per gap^2
     const double gs = omega;             // excitation exponent per gap
     // the lane's row a lies in 64-row group row_group(a) of the block: the wave's A groups are
