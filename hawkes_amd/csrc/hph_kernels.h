@@ -141,8 +141,9 @@ void launch_locgrad(const Events &ev, const double *params, const double *w, con
 // has its own events, parameter vectors and slices of the partial buffers. Work items
 // (dataset, row block, chunk of earlier columns | -1 for the diagonal block, parameter vector)
 // are taken from a queue by a persistent grid, heaviest first: the plan orders them by nominal
-// size, and a caller with the times and parameters on the host reorders them by expected cost
-// (the engine's SymRunner::order_queue). Any order gives the same results.
+// size, and launch_sym_order reorders them on the device by their expected cost under the call's
+// parameters before each pass in time order (the culled pass's order is formed on the host,
+// SymRunner::order_queue_culled). Any order gives the same results.
 
 // One dataset of a symmetric pass (device table entry).
 struct SymData {
