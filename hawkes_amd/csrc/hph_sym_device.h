@@ -39,22 +39,20 @@ namespace kern {
 //  base 2 (FAST true): coordinates arrive as x' = sx (x - x0) and times as t' = st (t - t0)
 //      (time_scale_dd of tauX and tauT, applied by one fma each; x0, t0 = the row block's first
 //      event), so that the background exponent in 2-units is the plain sum of squares y = r'^2 + dt'^2
-//      (clamp-free exp2_neg_fp64); excitation cS0 r'^2 + cS1 dt with cS0 = -sigma^2 / tauX^2,
-//      cS1 = -omega log2(e) and dt the difference of the natural times (staged beside the
-//      scaled ones: a scaled difference carries the rounding of st (t - t0), which an
-//      excitation term would feel as a relative error of order eps omega |t - t0|), evaluated
-//      by exp2_fp64 in FP32-bulk. The background moments are summed as e y and e dt'^2, the excitation ones
-//      as e r'^2 and e dt; the finalize kernels turn them into e r^2 = (e y - e dt'^2) / sx^2,
-//      e dt^2, e r^2 and e dt.
+//      (clamp-free exp2_neg_fp64); excitation -(cS0 r'^2 + cS1 dt) / 256 in 2-units, with
+//      cS0 = 256 sigma^2 / tauX^2, cS1 = 256 omega log2(e) (double-doubles) and dt the difference
+//      of the natural times (staged beside the scaled ones: a scaled difference carries the
+//      rounding of st (t - t0), which an excitation term would feel as a relative error of order
+//      eps omega |t - t0|), evaluated by exp2_neg_tab (excitation_term). The background moments
+//      are summed as e y and e dt'^2, the excitation ones as e r'^2 and e dt; the finalize kernels
+//      turn them into e r^2 = (e y - e dt'^2) / sx^2, e dt^2, e r^2 and e dt.
 //      In FP64 (F32 false) the scales are sqrt(256) times larger (time_scale_tab_dd), so that y
 //      arrives as 256 times the exponent for the table exponential exp2_neg_tab, and the finalize
-//      divides the squared moments by 256 (SymLaunch::b2tab); the excitation constants are then
-//      -256 times the base-2 ones against r'^2 (cS0 = sigma^2 / tauX^2, cS1 = 256 omega log2(e)),
-//      so that the excitation exponent too arrives for exp2_neg_tab (excitation_term).
+//      divides the squared moments by 256 (SymLaunch::b2tab); cS0 is then sigma^2 / tauX^2.
 struct PairConst {
     double cB0, cB1, cS0, cS1;
     double cS0lo, cS1lo; // base 2: the low parts of cS0 and cS1 (double-doubles), else 0
-    const double *tab;   // base 2 in FP64: exp2_neg_tab's table (LDS), else unused
+    const double *tab;   // base 2: exp2_neg_tab's table (LDS), else unused
 };
 
 // Pair geometry from the row's coordinates xi, ti and the column's q (D coordinates, then the
@@ -102,10 +100,9 @@ __device__ __forceinline__ double background_term(const PairConst &c, double r2,
 }
 
 // Excitation term of one pair. Base 2 (double-double constants: the exponent's scale matches the
-// finalize's sigma and omega): FP32-bulk evaluates 2^(cS0 r'^2 + cS1 dt) with exp2_fp64; FP64 (the
-// table form, TAB) forms yS = cS0 r'^2 + cS1 dt = -256 x the base-2 exponent and evaluates
-// exp2_neg_tab(yS), 2^(-yS/256) (KT: the table is exp2_neg_ktab's, whose range bounds only the
-// background exponents: exp2_neg_tab_on_ktab). For a causal pair yS >= 0; pairs with dt <= 0,
+// finalize's sigma and omega; TAB in both precisions): forms yS = cS0 r'^2 + cS1 dt = -256 x the
+// base-2 exponent and evaluates exp2_neg_tab(yS), 2^(-yS/256) (KT: the table is exp2_neg_ktab's,
+// whose range bounds only the background exponents: exp2_neg_tab_on_ktab). For a causal pair yS >= 0; pairs with dt <= 0,
 // whose yS may be negative (the result then finite or +inf, not 2^(-yS/256)), are masked by a
 // select by every caller; yS past 2^31 (terms far below the FP64 range) converts to INT_MIN and
 // gives +0.
@@ -282,7 +279,7 @@ __device__ __forceinline__ void sym_subtile(const double *__restrict__ sub, int 
                 }
             }
             if (SMODE != kSNone) {
-                double es = excitation_term<FAST, FAST && !F32, KT>(c, r2, dte, tab);
+                double es = excitation_term<FAST, FAST, KT>(c, r2, dte, tab);
                 if (SMODE == kSMixed) es = dte > 0.0 ? es : 0.0;
                 acc[a][3] += es;
                 if (GRAD) {
@@ -341,7 +338,7 @@ __device__ __forceinline__ void diag_pass(const double *__restrict__ tile, int c
             }
         }
         if (PM != kPassCulledB) {
-            double es = excitation_term<FAST, FAST && !F32, KT>(c, r2, dte, c.tab);
+            double es = excitation_term<FAST, FAST, KT>(c, r2, dte, c.tab);
             es = dte > 0.0 ? es : 0.0;
             acc[3] += es;
             if (GRAD) {
@@ -585,26 +582,19 @@ __device__ __forceinline__ void sym_item(const SymLaunch &L, const int4 it, cons
         c.cB1 = 0.0;
         // -sigma^2 / tauX^2 with sigma^2 and tauX^2 rounded as the finalize rounds them, the
         // quotient carried as a double-double; -omega log2(e) per natural time difference, the
-        // same. In FP64 (the table form) both are multiplied by -256, exactly: against the
-        // table form's r'^2 (256 times the base-2 units) the first becomes sigma^2 / tauX^2 itself,
-        // and the excitation exponent arrives as yS = -256 x (its base-2 value) >= 0 for
-        // exp2_neg_tab (excitation_term)
+        // same. Both are multiplied by -256, exactly, so that the excitation exponent arrives as
+        // yS = -256 x (its base-2 value) >= 0 for exp2_neg_tab (excitation_term); against the FP64
+        // table form's r'^2 (256 times the base-2 units) the first becomes sigma^2 / tauX^2 itself
         const double s2 = sigma * sigma, x2 = tauX * tauX;
         const double q = s2 / x2;
         const double qlo = __builtin_fma(-q, x2, s2) / x2;
         const double w = -omega * kLog2eHi;
         const double wlo = __builtin_fma(-omega, kLog2eHi, -w) + (-omega * kLog2eLo);
-        if (F32) {
-            c.cS0 = -q;
-            c.cS0lo = -qlo;
-            c.cS1 = w;
-            c.cS1lo = wlo;
-        } else {
-            c.cS0 = q;
-            c.cS0lo = qlo;
-            c.cS1 = -w * kExp2TabSize;
-            c.cS1lo = -wlo * kExp2TabSize;
-        }
+        const double r2unit = F32 ? (double)kExp2TabSize : 1.0;
+        c.cS0 = q * r2unit;
+        c.cS0lo = qlo * r2unit;
+        c.cS1 = -w * kExp2TabSize;
+        c.cS1lo = -wlo * kExp2TabSize;
     } else {
         c.cB0 = -0.5 * tauX * tauX * (F32 ? kLog2e : 1.0);
         c.cB1 = -0.5 * tauT * tauT * (F32 ? kLog2e : 1.0);
@@ -894,7 +884,7 @@ template <int D, bool GRAD, bool F32, int FM> struct SymLds {
     static constexpr int NC = GRAD ? 3 : 1;
     static constexpr int tile = (4 * SUB > kTile * ST) ? 4 * SUB : kTile * ST;
     static constexpr int slot = kSymWaves * NC * 128;
-    static constexpr int tab = (FM != 0 && !F32) ? kExp2TabSize : 0;
+    static constexpr int tab = FM != 0 ? kExp2TabSize : 0;
     static constexpr int total = tile + slot + tab + kSymWaves * NC * kTile;
     // workgroups of kSymWaves waves that fit one CU's 160 KB, at most 3 (waves per SIMD); 1 KB
     // per workgroup left for the compiler's own (the queue index, __syncthreads_or)
@@ -910,7 +900,7 @@ __global__ __launch_bounds__(kSymWaves * 64) __attribute__((amdgpu_waves_per_eu(
     SymLaunch L, const double *__restrict__ params, double *__restrict__ rowpart, double *__restrict__ colpart) {
     using Lds = SymLds<D, GRAD, F32, FM>;
     constexpr int NC = GRAD ? 3 : 1;
-    constexpr bool TAB = FM != 0 && !F32; // the base-2 form in FP64: exp2_neg_tab
+    constexpr bool TAB = FM != 0; // the base-2 form: exp2_neg_tab (FP64 background, excitation)
     constexpr bool KT = FM == 3;          // exp2_neg_ktab and its table
     static_assert(!KT || !F32, "the exponent-in-table form is FP64's");
     // diagonal items: kTile events x ST doubles; chunk items: 4 subtiles x SubStride doubles
