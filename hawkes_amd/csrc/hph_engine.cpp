@@ -148,11 +148,15 @@ static Extents extents_of(const std::vector<double> &locs, const double *sorted_
 // a space-filling curve inside a window of consecutive sorted events: their times, and so t - t0,
 // span the whole window, not 512 events. split adds the condition st x (window time span) <= 300,
 // which keeps the scaled times' rounding as small as sx x diagonal <= 300 keeps the coordinates'.
+// The table exponential takes exponents of one sign: the excitation exponent
+// -(omega dt + sigma^2 r^2 / 2) of a causal pair is <= 0 only for omega >= 0, so omega < 0
+// (accepted by setParameters, unphysical) takes the natural form.
 static bool fast_ok(const Extents &e, const double *params, int K, bool split = false) {
     if (!e.gaps_ok) return false;
     const double diag = std::sqrt(e.r2max);
     for (int k = 0; k < K; ++k) {
         const double *p = params + 6 * (size_t)k;
+        if (!(p[3] >= 0.0)) return false;
         if (!(std::fabs(p[3]) * e.block_span <= 300.0)) return false;
         const double aB = 0.5 * p[1] * p[1] * e.r2max + 0.5 * p[2] * p[2] * e.span * e.span;
         const double aS = std::fabs(p[3]) * e.span + 0.5 * p[0] * p[0] * e.r2max;

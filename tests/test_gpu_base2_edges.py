@@ -42,8 +42,8 @@ def cull_window(n):
 
 
 def base2_gate(locs, times, p, split=False):
-    """The engine's conditions for the base-2 form (fast_ok in the engine), restated: distinct
-    times at least 8 ulp apart, |omega| x (largest span of 512 sorted events) <= 300, every
+    """The engine's conditions for the base-2 form (fast_ok in the engine), restated: omega >= 0,
+    distinct times at least 8 ulp apart, |omega| x (largest span of 512 sorted events) <= 300, every
     background exponent below 5.5e6 and excitation exponent below 1e9, sx x diagonal <= 300,
     sigma sqrt(log2(e)/2) x diagonal <= 300, and for the split passes st x (largest window time
     span) <= 300."""
@@ -62,7 +62,7 @@ def base2_gate(locs, times, p, split=False):
     span = t[-1] - t[0]
     sigma, tauX, tauT, omega = p[0], p[1], p[2], p[3]
     st, sx = abs(tauT) * SX_PER_TAUX, abs(tauX) * SX_PER_TAUX
-    ok = abs(omega) * block_span <= 300.0
+    ok = omega >= 0.0 and abs(omega) * block_span <= 300.0
     ok &= 0.5 * tauX ** 2 * r2max + 0.5 * tauT ** 2 * span ** 2 < 5.5e6
     ok &= abs(omega) * span + 0.5 * sigma ** 2 * r2max < 1e9
     ok &= sx * diag <= 300.0 and abs(sigma) * SX_PER_TAUX * diag <= 300.0

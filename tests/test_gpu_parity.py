@@ -275,6 +275,24 @@ def test_negative_theta_takes_no_excitation_cut(kernel):
     assert np.all(np.abs(out[0][1] - out[1][1]) <= 1e-15 * ref["scale"][1:])
 
 
+@pytest.mark.parametrize("kernel", ["rowwise", "symmetric", "split"])
+def test_negative_omega_takes_the_natural_form(kernel):
+    """omega < 0 (accepted by setParameters, unphysical; with theta < 0 too, so that theta omega
+    > 0 keeps every rate positive): excitation terms grow with the time gap (exponents of either
+    sign), which the base-2 form's table exponential does not take; the passes run the natural
+    form, and no excitation tile is skipped (the cut needs omega >= 0). Results match the oracle
+    (1e-12)."""
+    locs, times = synth.cluster_process(3000, side=np.sqrt(4300 / 100), seed=92)
+    p = np.array([1.0, 1 / 1.6, 1 / 336.0, -2e-4, -1.0, 1.0])
+    assert_matches_oracle(locs, times, p, name="omega<0", kernel=kernel)
+    if kernel != "rowwise":
+        for params, base2 in ((p, False), (np.abs(p), True)):  # omega > 0: the base-2 form
+            e = make_engine(locs, times, params, kernel=kernel)
+            e.log_likelihood_and_gradient()
+            assert e.last_pass()[1] is base2
+            e.close()
+
+
 def test_repeat_calls_are_bitwise_identical():
     locs, times = synth.cluster_process(3000, seed=9)
     e = make_engine(locs, times, synth.P0)
