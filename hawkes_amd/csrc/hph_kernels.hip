@@ -308,7 +308,10 @@ __global__ __launch_bounds__(kRowsPerBlock) void reduce_kernel(Events ev, int nb
 // events contribute nothing); the wavefronts' sums are added in a fixed order. out: [3 or 1][npad]
 // in natural units (S, S r^2, S dt), per vector k (grid y): out + k NQ npad. cut: 0 the
 // likelihood passes' point X_S, 1 exactly zero terms only (omega gap >= 746, probability
-// requests), 2 none (every earlier event).
+// requests), 2 none (every earlier event). For omega >= 0 (every causal exponent
+// -(sigma^2 r^2 / 2 + omega dt) <= 0) the terms come from the LDS table exponential exp2_neg_tab,
+// the exponent scaled by 256 log2(e) with double-double constants (the scale of sigma^2 / 2 and
+// omega the finalize's): 10 VALU operations against exp_fp64's 16; omega < 0 keeps exp_fp64.
 template <int D, bool GRAD>
 __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events ev, const double *__restrict__ params,
                                                                         int cut, double *__restrict__ out) {
@@ -336,6 +339,14 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
     // excitation sum is one relative to the rate and to the probability
     const bool rel = !GRAD && cut == 3 && pk[4] * omega * sD > 0.0 && pk[5] * xD * pk[2] >= 0.0;
     __shared__ double part[4][kNarrow];
+    __shared__ double etab[kExp2TabSize];
+    fill_exp2_table(etab, tid, kRowsPerBlock); // complete before the first tile's barriers
+    const bool tabf = omega >= 0.0;
+    // yS = c0 r^2 + c1 dt = 256 log2(e) (sigma^2 r^2 / 2 + omega dt), constants as double-doubles
+    const double kh = 256.0 * kLog2eHi, kl = 256.0 * kLog2eLo;
+    const double s2h = -cSx;
+    const double c0 = s2h * kh, c0lo = __builtin_fma(s2h, kh, -c0) + s2h * kl;
+    const double c1 = omega * kh, c1lo = __builtin_fma(omega, kh, -c1) + omega * kl;
     bool done = false;
     out += (size_t)k * NQ * npad;
     double xi[D];
@@ -374,7 +385,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void excitation_band_kernel(Events e
                 r2 = __builtin_fma(dd, dd, r2);
             }
             const double dt = ti - e[D];
-            double es = exp_fp64(__builtin_fma(cSx, r2, -omega * dt));
+            double es;
+            if (tabf) // (uniform) a pair with dt <= 0 may give +inf or a wrong value: masked below
+                es = exp2_neg_tab(__builtin_fma(c0, r2, __builtin_fma(c0lo, r2, __builtin_fma(c1, dt, c1lo * dt))), etab);
+            else
+                es = exp_fp64(__builtin_fma(cSx, r2, -omega * dt));
             es = dt > 0.0 ? es : 0.0;
             acc[0] += es;
             if (GRAD) {
