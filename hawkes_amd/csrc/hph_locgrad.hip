@@ -183,10 +183,28 @@ __global__ __launch_bounds__(kRowsPerBlock) void locgrad_finalize_kernel(Events 
 // this pass is bound by its natural-form exp, and the likelihood pass's LDS column slots measured
 // no faster here).
 // The diagonal block is paired in both orders, each order adding to its row only.
+// The chunk subtiles' exponentials from the LDS table (exp2_neg_tab): each exponent scaled by
+// 256 log2(e), formed from double-double constants (the scale of tauX^2 / 2, tauT^2 / 2,
+// sigma^2 / 2 and omega the finalize's) on the natural differences. The excitation terms take it
+// for omega >= 0 only (exc; their exponent has one sign then), else exp_fp64.
+struct LocTab {
+    const double *tab;
+    double2 b0, b1, s0, s1; // 256 log2(e) x (tauX^2 / 2, tauT^2 / 2, sigma^2 / 2, omega)
+    bool exc;
+};
+__device__ __forceinline__ double2 scale_256log2e(double v) {
+    const double kh = 256.0 * kLog2eHi, kl = 256.0 * kLog2eLo;
+    const double hi = v * kh;
+    return make_double2(hi, __builtin_fma(v, kh, -hi) + v * kl);
+}
+__device__ __forceinline__ double tab_exp(double2 c0, double x0, double2 c1, double x1, const double *tab) {
+    return exp2_neg_tab(__builtin_fma(c0.x, x0, __builtin_fma(c0.y, x0, __builtin_fma(c1.x, x1, c1.y * x1))), tab);
+}
+
 template <int D, int A, bool DOB, int SMODE, bool TAIL>
 __device__ __forceinline__ void loc_subtile(const double *__restrict__ sub, int lane, const double (&xi)[A][D],
                                             const double (&ti)[A], const double (&wi)[A], const double (&vm)[A],
-                                            double cBx, double cBt, double cSx, double nomega, double fB, double fS,
+                                            double cSx, double nomega, double fB, double fS, const LocTab &lt,
                                             double (&acc)[A][D], double (&ca)[D]) {
     const int src = ((lane + 1) & 63) << 2;
 #pragma unroll 2
@@ -206,9 +224,10 @@ __device__ __forceinline__ void loc_subtile(const double *__restrict__ sub, int 
             }
             const double dt = ti[a] - q[D];
             double f = 0.0;
-            if (DOB) f = fB * background_exp<false>(cBx, cBt, r2, dt * dt) * (wi[a] + q[D + 1]);
+            if (DOB) f = fB * tab_exp(lt.b0, r2, lt.b1, dt * dt, lt.tab) * (wi[a] + q[D + 1]);
             if (SMODE != kSNone) {
-                double es = exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
+                // (a pair with dt <= 0 may give +inf or a wrong value: masked below)
+                double es = lt.exc ? tab_exp(lt.s0, r2, lt.s1, dt, lt.tab) : exp_fp64(__builtin_fma(cSx, r2, nomega * dt));
                 if (SMODE == kSMixed) es = dt > 0.0 ? es : 0.0;
                 f = __builtin_fma(fS * es, wi[a], f);
             }
@@ -227,24 +246,23 @@ __device__ __forceinline__ void loc_subtile(const double *__restrict__ sub, int 
 template <int D, int A, bool TAIL>
 __device__ __forceinline__ void loc_dispatch(int mode, const double *__restrict__ sub, int lane,
                                              const double (&xi)[A][D], const double (&ti)[A], const double (&wi)[A],
-                                             const double (&vm)[A], double cBx, double cBt, double cSx,
-                                             double nomega, double fB, double fS, double (&acc)[A][D],
-                                             double (&ca)[D]) {
+                                             const double (&vm)[A], double cSx, double nomega, double fB,
+                                             double fS, const LocTab &lt, double (&acc)[A][D], double (&ca)[D]) {
     switch (mode) {
     case 4 + kSNone:
-        loc_subtile<D, A, true, kSNone, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        loc_subtile<D, A, true, kSNone, TAIL>(sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc, ca);
         break;
     case 4 + kSFull:
-        loc_subtile<D, A, true, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        loc_subtile<D, A, true, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc, ca);
         break;
     case 4 + kSMixed:
-        loc_subtile<D, A, true, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        loc_subtile<D, A, true, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc, ca);
         break;
     case kSFull:
-        loc_subtile<D, A, false, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        loc_subtile<D, A, false, kSFull, TAIL>(sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc, ca);
         break;
     case kSMixed:
-        loc_subtile<D, A, false, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS, acc, ca);
+        loc_subtile<D, A, false, kSMixed, TAIL>(sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc, ca);
         break;
     default: break; // every term of the subtile is exactly zero
     }
@@ -261,9 +279,12 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L
     // diagonal items: kTile events x LS doubles; chunk items: 4 subtiles x (D + 2) x 128 doubles
     __shared__ double tile[(2 * kTile * (D + 2) > kTile * LS) ? 2 * kTile * (D + 2) : kTile * LS];
     __shared__ double colbuf[kSymWaves * D * kTile];
+    __shared__ double etab[kExp2TabSize];
     __shared__ int s_item;
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // the table is complete before the first item: the queue loop's barrier follows
+    fill_exp2_table(etab, tid, kSymWaves * 64);
     for (;;) {
         const int idx = queue_next(L, &s_item);
         if (idx >= L.items) break;
@@ -275,6 +296,13 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L
         const double sigma = pk[0], tauX = pk[1], tauT = pk[2], omega = pk[3], theta = pk[4], mu0 = pk[5];
         const double cBx = -0.5 * tauX * tauX, cBt = -0.5 * tauT * tauT, cSx = -0.5 * sigma * sigma;
         const double nomega = -omega;
+        LocTab lt;
+        lt.tab = etab;
+        lt.b0 = scale_256log2e(-cBx);
+        lt.b1 = scale_256log2e(-cBt);
+        lt.s0 = scale_256log2e(-cSx);
+        lt.s1 = scale_256log2e(omega);
+        lt.exc = omega >= 0.0;
         const double fB = mu0 * ipow(tauX, D) * tauT * 0.15915494309189533577 * tauX * tauX;
         const double fS = theta * omega * ipow(sigma, D) * 0.39894228040143267794 * sigma * sigma;
         const double *__restrict__ ex = ds.x;
@@ -375,11 +403,11 @@ __global__ __launch_bounds__(kSymWaves * 64) void sym_locgrad_kernel(SymLaunch L
                         for (int c = 0; c < D; ++c) ca[c] = 0.0;
                         const double *sub = tile + sb * (D + 2) * 128;
                         if (tail)
-                            loc_dispatch<D, A, true>(mode, sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB, fS,
-                                                     acc, ca);
+                            loc_dispatch<D, A, true>(mode, sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc,
+                                                     ca);
                         else
-                            loc_dispatch<D, A, false>(mode, sub, lane, xi, ti, wi, vm, cBx, cBt, cSx, nomega, fB,
-                                                      fS, acc, ca);
+                            loc_dispatch<D, A, false>(mode, sub, lane, xi, ti, wi, vm, cSx, nomega, fB, fS, lt, acc,
+                                                      ca);
 #pragma unroll
                         for (int c = 0; c < D; ++c) colbuf[(wv * D + c) * kTile + sb * 64 + lane] = ca[c];
                     }
